@@ -1,0 +1,292 @@
+"""ctypes binding of libsgraph.so (include/sgraph.h, include/sgraph_io.h).
+
+Host-side harness used by the tests and bench.py. Arrays cross the boundary
+as numpy arrays; every call raises SgError on a non-OK status. The library is
+loaded from this directory; a missing library raises instead of falling back
+to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsgraph.so")
+
+SG_OK, SG_EINVAL, SG_ENOMEM, SG_EBUDGET, SG_EDEVICE, SG_EIO = range(6)
+STATUS_NAMES = {0: "SG_OK", 1: "SG_EINVAL", 2: "SG_ENOMEM", 3: "SG_EBUDGET", 4: "SG_EDEVICE", 5: "SG_EIO"}
+ALGO = {"pr": 0, "sssp": 1, "bfs": 2, "cc": 3}
+WEIGHT = {"hash128": 0, "seeded": 1, "one": 2}
+Q_VALUE, Q_BUFFER, Q_PARENT, Q_DEGREE, Q_OFFSETS, Q_SLOTS = range(6)
+FLAG_KERNEL_TIMING = 0x1
+INF = 0xFFFFFFFF
+DEFAULT_SEED = 27491095  # kRandSeed of the reference (groute/graphs/util.h:68)
+
+
+class SgError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("algo", C.c_uint32), ("weight_mode", C.c_uint32), ("weight_seed", C.c_uint64),
+        ("pr_alpha", C.c_double), ("pr_epsilon", C.c_double), ("pma_density", C.c_double),
+        ("segments", C.c_uint32), ("n_stream", C.c_uint32), ("block_size", C.c_uint32),
+        ("max_rounds", C.c_uint32), ("rounds_per_sync", C.c_uint32), ("flags", C.c_uint32),
+        ("device_budget_gb", C.c_double), ("device", C.c_int32), ("reserved", C.c_uint32 * 7),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("nv", C.c_uint32), ("rounds", C.c_uint32), ("ne", C.c_uint64), ("capacity", C.c_uint64),
+        ("snapshots", C.c_uint64), ("edges_relaxed", C.c_uint64), ("vertices_processed", C.c_uint64),
+        ("invalidated", C.c_uint64), ("rebalances", C.c_uint64), ("resizes", C.c_uint64),
+        ("dropped_updates", C.c_uint64), ("relax_launches", C.c_uint64), ("relax_ms", C.c_double),
+        ("relax_bytes", C.c_uint64), ("last_phase_ms", C.c_double), ("pr_residual_l1", C.c_double),
+        ("pr_residual_max", C.c_double), ("device_bytes", C.c_uint64), ("reserved", C.c_uint64 * 8),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_P = C.c_void_p
+_lib = None
+
+# (name, restype, argtypes) of every exported function
+SIGNATURES = [
+    ("sg_config_default", None, [C.POINTER(Config)]),
+    ("sg_create", C.c_int, [C.POINTER(Config), C.POINTER(_P)]),
+    ("sg_destroy", None, [_P]),
+    ("sg_last_error", C.c_char_p, [_P]),
+    ("sg_load_csr", C.c_int, [_P, C.c_uint32, C.c_uint64, _P, _P]),
+    ("sg_init", C.c_int, [_P, _P, C.c_uint32]),
+    ("sg_apply_snapshot", C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint64]),
+    ("sg_apply_snapshot_device", C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint64]),
+    ("sg_gather", C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
+    ("sg_get_stats", C.c_int, [_P, C.POINTER(Stats)]),
+    ("sg_reset_stats", C.c_int, [_P]),
+    ("sg_device_alloc", C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    ("sg_device_free", C.c_int, [_P, _P]),
+    ("sg_copy_to_device", C.c_int, [_P, _P, _P, C.c_size_t]),
+    ("sg_synchronize", C.c_int, [_P]),
+    ("sg_free", None, [_P]),
+    ("sg_read_edge_list", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(_P), C.POINTER(_P)]),
+    ("sg_write_edge_list", C.c_int, [C.c_char_p, C.c_uint32, _P, _P]),
+    ("sg_read_stream_sizes", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(_P)]),
+    ("sg_read_updates", C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(_P), C.POINTER(C.c_uint64), C.POINTER(_P)]),
+    ("sg_write_updates", C.c_int, [C.c_char_p, C.c_char_p, C.c_uint32, _P, _P, _P]),
+    ("sg_gen_rmat", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_int,
+                              C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(_P), C.POINTER(_P)]),
+    ("sg_gen_updates", C.c_int, [C.c_uint32, _P, _P, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint32,
+                                 C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint64, C.POINTER(_P), C.POINTER(_P)]),
+    ("sg_edge_weight", C.c_uint32, [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
+]
+
+
+def lib():
+    """Load libsgraph.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` or __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(st, eng=None):
+    if st != SG_OK:
+        msg = lib().sg_last_error(eng)
+        raise SgError(st, msg.decode() if msg else "")
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else C.c_void_p(0)
+
+
+def _take(p, count, dtype):
+    """Copy a malloc'd array returned by the library into numpy and free it."""
+    if count == 0:
+        out = np.zeros(0, dtype=dtype)
+    else:
+        buf = (C.c_char * (count * np.dtype(dtype).itemsize)).from_address(p.value)
+        out = np.frombuffer(bytes(buf), dtype=dtype).copy()
+    lib().sg_free(p)
+    return out
+
+
+# ---- formats and generators (no GPU) --------------------------------------
+def read_edge_list(path):
+    nv, ne, row, col = C.c_uint32(), C.c_uint64(), C.c_void_p(), C.c_void_p()
+    _check(lib().sg_read_edge_list(path.encode(), C.byref(nv), C.byref(ne), C.byref(row), C.byref(col)))
+    return nv.value, _take(row, nv.value + 1, np.uint64), _take(col, ne.value, np.uint32)
+
+
+def write_edge_list(path, row, col):
+    row = np.ascontiguousarray(row, dtype=np.uint64)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    _check(lib().sg_write_edge_list(path.encode(), len(row) - 1, _ptr(row), _ptr(col)))
+
+
+def read_stream_sizes(path):
+    n, p = C.c_uint32(), C.c_void_p()
+    _check(lib().sg_read_stream_sizes(path.encode(), C.byref(n), C.byref(p)))
+    return _take(p, 2 * n.value, np.uint64).reshape(-1, 2)
+
+
+def read_updates(path):
+    ni, pi, nd, pd = C.c_uint64(), C.c_void_p(), C.c_uint64(), C.c_void_p()
+    _check(lib().sg_read_updates(path.encode(), C.byref(ni), C.byref(pi), C.byref(nd), C.byref(pd)))
+    return _take(pi, 2 * ni.value, np.uint32).reshape(-1, 2), _take(pd, 2 * nd.value, np.uint32).reshape(-1, 2)
+
+
+def write_updates(update_path, size_path, sizes, ins, dele):
+    sizes = np.ascontiguousarray(sizes, dtype=np.uint64).reshape(-1)
+    ins = np.ascontiguousarray(ins, dtype=np.uint32).reshape(-1)
+    dele = np.ascontiguousarray(dele, dtype=np.uint32).reshape(-1)
+    _check(lib().sg_write_updates(update_path.encode(), size_path.encode(), len(sizes) // 2, _ptr(sizes), _ptr(ins), _ptr(dele)))
+
+
+def gen_rmat(scale, edge_factor, seed=DEFAULT_SEED, a=0.57, b=0.19, c=0.19, permute=False):
+    nv, ne, row, col = C.c_uint32(), C.c_uint64(), C.c_void_p(), C.c_void_p()
+    _check(lib().sg_gen_rmat(scale, edge_factor, seed, a, b, c, int(permute), C.byref(nv), C.byref(ne), C.byref(row), C.byref(col)))
+    return nv.value, _take(row, nv.value + 1, np.uint64), _take(col, ne.value, np.uint32)
+
+
+def gen_updates(row, col, scale, nsnap, n_ins, n_del, seed=DEFAULT_SEED, hub_frac=0.0, hub_top=0.01, a=0.57, b=0.19, c=0.19):
+    row = np.ascontiguousarray(row, dtype=np.uint64)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    pi, pd = C.c_void_p(), C.c_void_p()
+    _check(lib().sg_gen_updates(len(row) - 1, _ptr(row), _ptr(col), scale, a, b, c, nsnap, n_ins, n_del,
+                                hub_frac, hub_top, seed, C.byref(pi), C.byref(pd)))
+    ins = _take(pi, 2 * nsnap * n_ins, np.uint32).reshape(nsnap, n_ins, 2)
+    dele = _take(pd, 2 * nsnap * n_del, np.uint32).reshape(nsnap, n_del, 2)
+    return ins, dele
+
+
+def edge_weight(mode, seed, u, v):
+    return lib().sg_edge_weight(WEIGHT[mode] if isinstance(mode, str) else mode, seed, u, v)
+
+
+# ---- engine -----------------------------------------------------------------
+def default_config(**kw):
+    cfg = Config()
+    lib().sg_config_default(C.byref(cfg))
+    for k, v in kw.items():
+        if k == "algo" and isinstance(v, str):
+            v = ALGO[v]
+        if k == "weight_mode" and isinstance(v, str):
+            v = WEIGHT[v]
+        setattr(cfg, k, v)
+    return cfg
+
+
+class Engine:
+    """One engine handle (single-threaded)."""
+
+    def __init__(self, algo, **cfg):
+        self.algo = algo
+        self.cfg = default_config(algo=algo, **cfg)
+        h = C.c_void_p()
+        _check(lib().sg_create(C.byref(self.cfg), C.byref(h)))
+        self.h = h
+        self.nv = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def load_csr(self, row, col):
+        row = np.ascontiguousarray(row, dtype=np.uint64)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        self.nv = len(row) - 1
+        _check(lib().sg_load_csr(self.h, self.nv, len(col), _ptr(row), _ptr(col)), self.h)
+
+    def init(self, source=None):
+        if source is None:
+            _check(lib().sg_init(self.h, None, 0), self.h)
+        else:
+            s = np.array([source], dtype=np.uint32)
+            _check(lib().sg_init(self.h, _ptr(s), 1), self.h)
+
+    def apply(self, ins, dele):
+        ins = np.ascontiguousarray(ins, dtype=np.uint32).reshape(-1)
+        dele = np.ascontiguousarray(dele, dtype=np.uint32).reshape(-1)
+        _check(lib().sg_apply_snapshot(self.h, _ptr(ins), len(ins) // 2, _ptr(dele), len(dele) // 2), self.h)
+
+    def apply_device(self, d_ins, nins, d_del, ndel):
+        _check(lib().sg_apply_snapshot_device(self.h, C.c_void_p(d_ins), nins, C.c_void_p(d_del), ndel), self.h)
+
+    def device_alloc(self, nbytes):
+        p = C.c_void_p()
+        _check(lib().sg_device_alloc(self.h, nbytes, C.byref(p)), self.h)
+        return p.value
+
+    def device_free(self, p):
+        _check(lib().sg_device_free(self.h, C.c_void_p(p)), self.h)
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        p = self.device_alloc(max(arr.nbytes, 1))
+        if arr.nbytes:
+            _check(lib().sg_copy_to_device(self.h, C.c_void_p(p), _ptr(arr), arr.nbytes), self.h)
+        return p
+
+    def gather(self, q):
+        if q == Q_OFFSETS:
+            out = np.empty(self.nv + 1, dtype=np.uint64)
+        elif q == Q_SLOTS:
+            out = np.empty(self.stats()["capacity"], dtype=np.uint32)
+        elif q in (Q_VALUE, Q_BUFFER) and self.algo == "pr":
+            out = np.empty(self.nv, dtype=np.float32)
+        else:
+            out = np.empty(self.nv, dtype=np.uint32)
+        _check(lib().sg_gather(self.h, q, _ptr(out), out.nbytes), self.h)
+        return out
+
+    def values(self):
+        return self.gather(Q_VALUE)
+
+    def adjacency(self):
+        """Per-vertex adjacency sequences as a CSR (row, col) built from the PMA slots."""
+        off = self.gather(Q_OFFSETS)
+        deg = self.gather(Q_DEGREE)
+        slots = self.gather(Q_SLOTS)
+        row = np.zeros(self.nv + 1, dtype=np.uint64)
+        np.cumsum(deg.astype(np.uint64), out=row[1:])
+        idx = np.repeat(off[:-1], deg.astype(np.int64)) + (np.arange(int(row[-1]), dtype=np.uint64) - np.repeat(row[:-1], deg.astype(np.int64)))
+        return row, slots[idx.astype(np.int64)], off, deg, slots
+
+    def stats(self):
+        s = Stats()
+        _check(lib().sg_get_stats(self.h, C.byref(s)), self.h)
+        return s.as_dict()
+
+    def reset_stats(self):
+        _check(lib().sg_reset_stats(self.h), self.h)
+
+    def synchronize(self):
+        _check(lib().sg_synchronize(self.h), self.h)

@@ -1,0 +1,181 @@
+/*
+ * sgraph.h — C ABI of the MI355X streaming-graph engine (libsgraph.so).
+ *
+ * The engine keeps a dynamic directed graph resident in HBM as a packed-memory
+ * array (PMA) CSR with per-vertex gaps, applies update batches (inserts first,
+ * then deletes) with HIP kernels, and recomputes one of four vertex programs
+ * incrementally per snapshot: PageRank, SSSP, BFS, or directed min-ancestor
+ * component labels.
+ *
+ * Every call returns an sg_status. No call exits the process and no C++
+ * exception crosses this boundary. Host buffers passed in are read (or
+ * written) during the call only; the engine owns all device memory it
+ * allocates. An engine handle is single-threaded.
+ *
+ * Each entry point cites the reference interface it replaces (paths relative
+ * to the C-GpuStreamGraph source tree).
+ */
+#ifndef SGRAPH_H
+#define SGRAPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sg_status {
+    SG_OK = 0,
+    SG_EINVAL = 1,  /* bad argument, id out of range, wrong call order        */
+    SG_ENOMEM = 2,  /* host allocation failed                                  */
+    SG_EBUDGET = 3, /* device allocation would exceed the device budget        */
+    SG_EDEVICE = 4, /* HIP runtime error or no usable device                   */
+    SG_EIO = 5      /* file could not be opened, read or parsed                */
+} sg_status;
+
+typedef enum sg_algo {
+    SG_ALGO_PR = 0,   /* operator PageRank, samples/hybrid_pr/hybrid_pr.cu:25-152   */
+    SG_ALGO_SSSP = 1, /* operator SSSP, samples/hybrid_sssp/hybrid_sssp.cu:28-194   */
+    SG_ALGO_BFS = 2,  /* operator BFS, samples/hybrid_bfs/hybrid_bfs.cu:26-167      */
+    SG_ALGO_CC = 3    /* operator CC, samples/hybrid_cc/hybrid_cc.cu:26-170         */
+} sg_algo;
+
+/* Edge-weight function of SSSP. Weights are computed from (u, v) on the fly
+ * and never stored, as in the reference (push_functor.h:306). */
+typedef enum sg_weight_mode {
+    SG_WEIGHT_HASH128 = 0, /* w = (u + v) % 128 + 1, push_functor.h:306          */
+    SG_WEIGHT_SEEDED = 1,  /* w = 1 + splitmix64(seed ^ (u << 32 | v)) % 63      */
+    SG_WEIGHT_ONE = 2      /* w = 1                                              */
+} sg_weight_mode;
+
+typedef struct sg_config {
+    uint32_t algo;            /* sg_algo                                                  */
+    uint32_t weight_mode;     /* sg_weight_mode (SSSP only)                               */
+    uint64_t weight_seed;     /* seed of SG_WEIGHT_SEEDED                                 */
+    double pr_alpha;          /* damping 0.85, samples/hybrid_pr/hybrid_pr_common.h:38    */
+    double pr_epsilon;        /* residual threshold 1e-6, samples/hybrid_pr/main.cu:37    */
+    double pma_density;       /* initial PMA edge density 0.25, csr_graph.cuh:349-352,379 */
+    uint32_t segments;        /* -SEGMENT vertex segments, framework.cuh:310-384          */
+    uint32_t n_stream;        /* -n_stream streams, app_skeleton.h:89                     */
+    uint32_t block_size;      /* threads per relax block (multiple of 64, <= 1024)        */
+    uint32_t max_rounds;      /* cap on convergence rounds per phase (0 = 1000000)        */
+    uint32_t rounds_per_sync; /* rounds enqueued between host convergence checks (0 = 4)  */
+    uint32_t flags;           /* SG_FLAG_*                                                */
+    double device_budget_gb;  /* cap on engine device memory in GiB (0 = no cap)          */
+    int32_t device;           /* HIP device ordinal                                       */
+    uint32_t reserved[7];
+} sg_config;
+
+/* Time the dominant relax kernel with HIP events recorded on its stream. */
+#define SG_FLAG_KERNEL_TIMING 0x1u
+
+typedef struct sg_stats {
+    uint32_t nv;                  /* vertices                                          */
+    uint32_t rounds;              /* convergence rounds of the last phase              */
+    uint64_t ne;                  /* live edges after the last snapshot                */
+    uint64_t capacity;            /* PMA slots                                         */
+    uint64_t snapshots;           /* snapshots applied                                 */
+    uint64_t edges_relaxed;       /* edges traversed by relax kernels (cumulative)     */
+    uint64_t vertices_processed;  /* frontier vertices expanded (cumulative)           */
+    uint64_t invalidated;         /* vertices reset by deletions (last snapshot)       */
+    uint64_t rebalances;          /* PMA windows re-spread (cumulative)                */
+    uint64_t resizes;             /* PMA capacity doublings (cumulative)               */
+    uint64_t dropped_updates;     /* update pairs with an id >= nv, ignored (cumul.)   */
+    uint64_t relax_launches;      /* launches of the timed relax kernel                */
+    double relax_ms;              /* summed HIP-event time of those launches           */
+    uint64_t relax_bytes;         /* algorithmic bytes of those launches               */
+    double last_phase_ms;         /* host wall time of the last sg_init/sg_apply_* call */
+    double pr_residual_l1;        /* PR: sum |residual| after convergence (FP64 on GPU) */
+    double pr_residual_max;       /* PR: max |residual| after convergence (FP64 on GPU) */
+    uint64_t device_bytes;        /* device memory held by the engine                  */
+    uint64_t reserved[8];
+} sg_stats;
+
+/* What sg_gather copies out. */
+typedef enum sg_query {
+    SG_Q_VALUE = 0,   /* per-vertex result: u32 levels/distances/labels (UINT32_MAX =
+                         unreached), f32 ranks; GatherValue, graph_datum.cuh:297-315   */
+    SG_Q_BUFFER = 1,  /* per-vertex buffer: f32 PR residual, u32 pending dist/label;
+                         GatherBuffer, graph_datum.cuh:337-355                         */
+    SG_Q_PARENT = 2,  /* per-vertex u32 parent (UINT32_MAX = none); GatherParent,
+                         graph_datum.cuh:317-335                                       */
+    SG_Q_DEGREE = 3,  /* per-vertex u32 out-degree                                      */
+    SG_Q_OFFSETS = 4, /* u64[nv + 1] PMA slot offsets (last entry = capacity)           */
+    SG_Q_SLOTS = 5    /* u32[capacity] PMA slots (UINT32_MAX = gap)                     */
+} sg_query;
+
+typedef struct sg_engine sg_engine;
+
+/* Fill *cfg with the reference's defaults (app_skeleton.h:42-100,
+ * samples/hybrid_pr/main.cu:37, hybrid_pr_common.h:38). */
+void sg_config_default(sg_config *cfg);
+
+/* Create an engine on cfg->device: validates the device, creates the streams.
+ * Replaces Engine::Engine (include/framework/framework.cuh:124-140). */
+sg_status sg_create(const sg_config *cfg, sg_engine **out);
+
+/* Release every device and host resource of the engine. NULL is a no-op. */
+void sg_destroy(sg_engine *eng);
+
+/* Text of the last error on this engine (or of the last sg_create failure
+ * when eng is NULL). Never NULL. Errors print nothing by themselves;
+ * replaces GROUTE_CUDA_CHECK's print-and-exit (groute/internal/cuda_utils.h:41-65). */
+const char *sg_last_error(const sg_engine *eng);
+
+/* Load a CSR snapshot G_0 (row[nv+1] offsets, col[ne] destinations, adjacency
+ * order kept) into a device PMA with gaps spread in proportion to deg+1.
+ * Replaces PMAGraph::Bind + spread_weighted_V1 (groute/graphs/csr_graph.cuh:384-511),
+ * Engine::LoadGraph (framework.cuh:310-384) and PMAGraphAllocator
+ * (csr_graph.cuh:1768-1905). */
+sg_status sg_load_csr(sg_engine *eng, uint32_t nv, uint64_t ne,
+                      const uint64_t *row, const uint32_t *col);
+
+/* Initialise vertex state and run the static solve to convergence.
+ * sources: SSSP/BFS source vertex (nsrc must be 1); ignored by PR and CC.
+ * Replaces Engine::InitGraph + Engine::Start (framework.cuh:389-439, 549-584). */
+sg_status sg_init(sg_engine *eng, const uint32_t *sources, uint32_t nsrc);
+
+/* Apply one snapshot from host memory: ins_uv/del_uv are (u, v) pairs, i.e.
+ * 2*nins / 2*ndel uint32 values. Inserts are applied first, then deletes
+ * (every copy of a deleted (u, v) is removed); then the vertex program is
+ * brought to its fixed point on the new graph. Pairs are validated on the host
+ * first: an id >= nv returns SG_EINVAL and changes nothing.
+ * Replaces, for SSSP/BFS/CC, Engine::del_edge + Engine::add_edge
+ * (framework.cuh:975-979, 1025-1038, 1507-1655) and, for PR,
+ * Engine::Cancelation + Engine::Compensate (framework.cuh:1110-1301). */
+sg_status sg_apply_snapshot(sg_engine *eng,
+                            const uint32_t *ins_uv, uint64_t nins,
+                            const uint32_t *del_uv, uint64_t ndel);
+
+/* Same as sg_apply_snapshot with the pairs already in device memory (for
+ * example from sg_device_alloc + sg_copy_to_device). Pairs with an id >= nv
+ * are ignored and counted in sg_stats.dropped_updates. */
+sg_status sg_apply_snapshot_device(sg_engine *eng,
+                                   const uint32_t *d_ins_uv, uint64_t nins,
+                                   const uint32_t *d_del_uv, uint64_t ndel);
+
+/* Copy per-vertex state or the PMA arrays to host memory. bytes must be at
+ * least the size of the requested array (see sg_query).
+ * Replaces GatherValue/GatherParent/GatherBuffer (graph_datum.cuh:297-355). */
+sg_status sg_gather(sg_engine *eng, uint32_t query, void *out, size_t bytes);
+
+/* Counters of the engine (see sg_stats). */
+sg_status sg_get_stats(sg_engine *eng, sg_stats *out);
+
+/* Zero the cumulative counters and kernel timers. */
+sg_status sg_reset_stats(sg_engine *eng);
+
+/* Device-memory helpers for callers without an allocator of their own. The
+ * allocation counts against the engine's device budget. */
+sg_status sg_device_alloc(sg_engine *eng, size_t bytes, void **dptr);
+sg_status sg_device_free(sg_engine *eng, void *dptr);
+sg_status sg_copy_to_device(sg_engine *eng, void *dst, const void *src, size_t bytes);
+
+/* Wait for all work of the engine. */
+sg_status sg_synchronize(sg_engine *eng);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGRAPH_H */

@@ -1,0 +1,143 @@
+"""ctypes binding of the CPU oracle (oracle.c). TEST INFRASTRUCTURE ONLY:
+imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as
+the checker; never by the product."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+INF = 0xFFFFFFFF
+_lib = None
+_P = C.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "or_weight": (C.c_uint32, [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
+            "or_bfs_levels": (C.c_int, [C.c_uint32, _P, _P, C.c_uint32, _P]),
+            "or_sssp_dijkstra": (C.c_int, [C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint64, _P]),
+            "or_sssp_fifo": (C.c_int, [C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint64, _P]),
+            "or_cc_min_ancestor": (C.c_int, [C.c_uint32, _P, _P, _P]),
+            "or_cc_label_prop": (C.c_int, [C.c_uint32, _P, _P, _P]),
+            "or_pr_fp64": (C.c_int, [C.c_uint32, _P, _P, C.c_double, C.c_double, C.c_int, _P]),
+            "or_pr_host_f32": (C.c_int, [C.c_uint32, _P, _P, C.c_double, C.c_double, _P]),
+            "or_apply_updates": (C.c_int, [C.c_uint32, _P, _P, _P, C.c_uint64, _P, C.c_uint64, C.POINTER(_P), C.POINTER(_P)]),
+            "or_free": (None, [_P]),
+            "or_pma_geometry": (None, [C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
+            "or_segment_partition": (C.c_uint32, [C.c_uint32, _P, C.c_uint32, _P]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else C.c_void_p(0)
+
+
+def _csr(row, col):
+    return np.ascontiguousarray(row, dtype=np.uint64), np.ascontiguousarray(col, dtype=np.uint32)
+
+
+WMODE = {"hash128": 0, "seeded": 1, "one": 2, "hash128_noplus1": 3}
+
+
+def weight(mode, seed, u, v):
+    return lib().or_weight(WMODE.get(mode, mode), seed, u, v)
+
+
+def bfs(row, col, src):
+    row, col = _csr(row, col)
+    out = np.empty(len(row) - 1, dtype=np.uint32)
+    assert lib().or_bfs_levels(len(row) - 1, _p(row), _p(col), src, _p(out)) == 0
+    return out
+
+
+def sssp(row, col, src, wmode="hash128", seed=0):
+    row, col = _csr(row, col)
+    out = np.empty(len(row) - 1, dtype=np.uint32)
+    assert lib().or_sssp_dijkstra(len(row) - 1, _p(row), _p(col), src, WMODE.get(wmode, wmode), seed, _p(out)) == 0
+    return out
+
+
+def sssp_fifo(row, col, src, wmode="hash128", seed=0):
+    row, col = _csr(row, col)
+    out = np.empty(len(row) - 1, dtype=np.uint32)
+    assert lib().or_sssp_fifo(len(row) - 1, _p(row), _p(col), src, WMODE.get(wmode, wmode), seed, _p(out)) == 0
+    return out
+
+
+def cc(row, col):
+    row, col = _csr(row, col)
+    out = np.empty(len(row) - 1, dtype=np.uint32)
+    assert lib().or_cc_min_ancestor(len(row) - 1, _p(row), _p(col), _p(out)) == 0
+    return out
+
+
+def cc_label_prop(row, col):
+    row, col = _csr(row, col)
+    out = np.empty(len(row) - 1, dtype=np.uint32)
+    assert lib().or_cc_label_prop(len(row) - 1, _p(row), _p(col), _p(out)) == 0
+    return out
+
+
+def pagerank(row, col, alpha=0.85, tol=1e-13, max_iter=100000):
+    row, col = _csr(row, col)
+    out = np.empty(len(row) - 1, dtype=np.float64)
+    it = lib().or_pr_fp64(len(row) - 1, _p(row), _p(col), alpha, tol, max_iter, _p(out))
+    assert it > 0, "PageRank oracle did not converge"
+    return out
+
+
+def pagerank_host_f32(row, col, alpha=0.85, eps=1e-6):
+    row, col = _csr(row, col)
+    out = np.empty(len(row) - 1, dtype=np.float32)
+    assert lib().or_pr_host_f32(len(row) - 1, _p(row), _p(col), alpha, eps, _p(out)) >= 0
+    return out
+
+
+def apply_updates(row, col, ins, dele):
+    """Adjacency model of one snapshot: returns the new (row, col)."""
+    row, col = _csr(row, col)
+    ins = np.ascontiguousarray(ins, dtype=np.uint32).reshape(-1)
+    dele = np.ascontiguousarray(dele, dtype=np.uint32).reshape(-1)
+    nv = len(row) - 1
+    pr, pc = C.c_void_p(), C.c_void_p()
+    rc = lib().or_apply_updates(nv, _p(row), _p(col), _p(ins), len(ins) // 2, _p(dele), len(dele) // 2, C.byref(pr), C.byref(pc))
+    assert rc == 0, f"or_apply_updates failed ({rc})"
+    nrow = np.frombuffer((C.c_uint64 * (nv + 1)).from_address(pr.value), dtype=np.uint64).copy()
+    ne = int(nrow[-1])
+    ncol = np.frombuffer((C.c_uint32 * max(ne, 1)).from_address(pc.value), dtype=np.uint32)[:ne].copy()
+    lib().or_free(pr)
+    lib().or_free(pc)
+    return nrow, ncol
+
+
+def pma_geometry(nv):
+    s, c = C.c_uint32(), C.c_uint64()
+    lib().or_pma_geometry(nv, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def segment_partition(off, nseg):
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    out = np.zeros(nseg + 1, dtype=np.uint32)
+    n = lib().or_segment_partition(len(off) - 1, _p(off), nseg, _p(out))
+    return out[: n + 1]

@@ -1,0 +1,62 @@
+"""The C ABI library loads and exports every symbol include/*.h declares;
+argument and state errors come back as status codes (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import sgraph
+from conftest import ROOT
+
+
+def declared_functions():
+    names = set()
+    for h in ("sgraph.h", "sgraph_io.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(sg_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    lib = sgraph.lib()
+    names = declared_functions()
+    assert len(names) >= 24
+    for n in names:
+        assert hasattr(lib, n), n
+    # and the binding covers them all
+    assert set(names) == {s[0] for s in sgraph.SIGNATURES}
+
+
+def test_config_defaults_are_the_references():
+    cfg = sgraph.default_config()
+    assert cfg.pr_alpha == 0.85 and cfg.pr_epsilon == 1e-6
+    assert cfg.pma_density == 0.25 and cfg.n_stream == 3 and cfg.block_size == 512
+    assert cfg.segments == 1 and cfg.weight_seed == 27491095
+
+
+def test_null_handles_are_einval():
+    lib = sgraph.lib()
+    assert lib.sg_init(None, None, 0) == sgraph.SG_EINVAL
+    assert lib.sg_apply_snapshot(None, None, 0, None, 0) == sgraph.SG_EINVAL
+    assert lib.sg_create(None, None) == sgraph.SG_EINVAL
+    assert lib.sg_last_error(None)
+    lib.sg_destroy(None)
+
+
+def test_bad_config_is_einval():
+    cfg = sgraph.default_config(algo=9)
+    h = C.c_void_p()
+    assert sgraph.lib().sg_create(C.byref(cfg), C.byref(h)) == sgraph.SG_EINVAL
+    cfg = sgraph.default_config(algo=0, pr_alpha=1.5)
+    assert sgraph.lib().sg_create(C.byref(cfg), C.byref(h)) == sgraph.SG_EINVAL
+
+
+def test_create_without_device_fails_cleanly():
+    cfg = sgraph.default_config(algo=1, device=99)
+    h = C.c_void_p()
+    st = sgraph.lib().sg_create(C.byref(cfg), C.byref(h))
+    assert st == sgraph.SG_EDEVICE
+    assert b"device" in sgraph.lib().sg_last_error(None).lower() or sgraph.lib().sg_last_error(None)

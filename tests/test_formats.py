@@ -1,0 +1,112 @@
+"""Input formats and synthetic inputs of libsgraph.so (host code, no GPU)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sgraph
+from conftest import GOLDEN
+
+
+def test_readme_files_parse():
+    nv, row, col = sgraph.read_edge_list(os.path.join(GOLDEN, "readme_graph.txt"))
+    assert nv == 5
+    assert row.tolist() == [0, 2, 4, 5, 6, 6]
+    assert col.tolist() == [1, 2, 2, 3, 4, 4]
+    sizes = sgraph.read_stream_sizes(os.path.join(GOLDEN, "readme_stream_size.txt"))
+    assert sizes.tolist() == [[2, 3], [2, 3]]
+    ins, dele = sgraph.read_updates(os.path.join(GOLDEN, "readme_update.txt"))
+    assert ins.tolist() == [[0, 3], [0, 4], [1, 4], [2, 3]]
+    assert dele.tolist() == [[0, 1], [0, 2], [1, 2], [1, 3], [2, 4], [3, 4]]
+
+
+def test_edge_list_semantics(tmp_path):
+    p = tmp_path / "g.txt"
+    # file order kept per source, duplicates and self-loops kept, weights ignored,
+    # junk lines skipped, nv from max(src, dst) + 1
+    p.write_text("2 1\n0 5 17\n# comment\n2 0\n0 5\n\n3 3\n")
+    nv, row, col = sgraph.read_edge_list(str(p))
+    assert nv == 6
+    assert row.tolist() == [0, 2, 2, 4, 5, 5, 5]
+    assert col.tolist() == [5, 5, 1, 0, 3]
+
+
+def test_update_file_semantics(tmp_path):
+    p = tmp_path / "u.txt"
+    p.write_text("d 1 2 9\na 0 1\nx 5 5\nd 3 4\na 2 2 7\nab 1 1\n")
+    ins, dele = sgraph.read_updates(str(p))
+    assert ins.tolist() == [[0, 1], [2, 2]]
+    assert dele.tolist() == [[1, 2], [3, 4]]
+
+
+def test_round_trip(tmp_path):
+    nv, row, col = sgraph.gen_rmat(8, 4)
+    g = str(tmp_path / "g.txt")
+    sgraph.write_edge_list(g, row, col)
+    nv2, row2, col2 = sgraph.read_edge_list(g)
+    # nv of a file is max id + 1
+    assert np.array_equal(row2, row[: nv2 + 1]) and np.array_equal(col2, col)
+    ins, dele = sgraph.gen_updates(row, col, 8, 3, 5, 4)
+    sizes = np.array([[5, 4]] * 3, dtype=np.uint64)
+    u, s = str(tmp_path / "u.txt"), str(tmp_path / "s.txt")
+    sgraph.write_updates(u, s, sizes, ins, dele)
+    i2, d2 = sgraph.read_updates(u)
+    assert np.array_equal(i2, ins.reshape(-1, 2)) and np.array_equal(d2, dele.reshape(-1, 2))
+    assert np.array_equal(sgraph.read_stream_sizes(s), sizes)
+
+
+def test_missing_file_is_eio():
+    with pytest.raises(sgraph.SgError) as e:
+        sgraph.read_edge_list("/nonexistent/graph.txt")
+    assert e.value.status == sgraph.SG_EIO
+
+
+def test_rmat_properties():
+    nv, row, col = sgraph.gen_rmat(12, 16)
+    assert nv == 4096
+    nv2, row2, col2 = sgraph.gen_rmat(12, 16)
+    assert np.array_equal(row, row2) and np.array_equal(col, col2)
+    ne = len(col)
+    assert 0.5 * 16 * nv < ne <= 16 * nv
+    src = np.repeat(np.arange(nv, dtype=np.uint64), np.diff(row).astype(np.int64))
+    keys = (src << np.uint64(32)) | col.astype(np.uint64)
+    assert np.all(np.diff(keys.astype(np.int64)) > 0)  # sorted, no duplicates
+    assert not np.any(src == col)  # no self-loops
+    deg = np.diff(row)
+    assert deg[0] == deg.max()  # unpermuted R-MAT: vertex 0 is the hub
+    _, rowp, colp = sgraph.gen_rmat(12, 16, permute=True)
+    assert len(colp) == ne  # permutation preserves the edge count
+
+
+def test_update_stream_properties():
+    nv, row, col = sgraph.gen_rmat(10, 8)
+    ins, dele = sgraph.gen_updates(row, col, 10, 4, 300, 200, seed=5)
+    cur = set(zip(np.repeat(np.arange(nv), np.diff(row).astype(np.int64)).tolist(), col.tolist()))
+    for t in range(4):
+        batch_i = [tuple(x) for x in ins[t].tolist()]
+        batch_d = [tuple(x) for x in dele[t].tolist()]
+        assert len(set(batch_i)) == len(batch_i) and len(set(batch_d)) == len(batch_d)
+        for e in batch_i:
+            assert e not in cur and e[0] != e[1]
+        for e in batch_d:
+            assert e in cur  # deletes come from G_{t-1}
+        cur = (cur | set(batch_i)) - set(batch_d)
+    # hub bias: most inserted sources among the top 1% by out-degree
+    ins_h, _ = sgraph.gen_updates(row, col, 10, 1, 500, 0, seed=6, hub_frac=0.8, hub_top=0.01)
+    deg = np.diff(row)
+    k = max(1, int(0.01 * nv))
+    hubs = set(np.argsort(-deg.astype(np.int64), kind="stable")[:k].tolist())
+    frac = np.mean([u in hubs for u in ins_h[0, :, 0].tolist()])
+    assert frac > 0.7
+
+
+def test_weights_match_oracle():
+    rng = np.random.default_rng(0)
+    for _ in range(2000):
+        u, v = int(rng.integers(0, 2**32 - 1)), int(rng.integers(0, 2**32 - 1))
+        for m in ("hash128", "seeded", "one"):
+            w = sgraph.edge_weight(m, 1234567, u, v)
+            assert w == O.weight(m, 1234567, u, v)
+            assert 1 <= w <= 128
+        assert sgraph.edge_weight("seeded", 99, u, v) <= 63
