@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--seed", type=int, default=sgraph.DEFAULT_SEED)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="do not time the relax kernel with HIP events")
+    ap.add_argument("--wave-budget", type=int, default=0)
+    ap.add_argument("--heavy-degree", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU "
@@ -87,7 +89,8 @@ def main():
     dev_batches = {}
     for a in algos:
         flags = 0 if args.no_timing else sgraph.FLAG_KERNEL_TIMING
-        e = sgraph.Engine(a, weight_mode="seeded", weight_seed=wseed, flags=flags)
+        e = sgraph.Engine(a, weight_mode="seeded", weight_seed=wseed, flags=flags, wave_budget=args.wave_budget,
+                          heavy_degree=args.heavy_degree)
         t = time.perf_counter()
         e.load_csr(row, col)
         e.init(0)
@@ -155,7 +158,7 @@ def main():
         "latency_p50_ms": round(float(np.percentile(all_lat, 50)), 3),
         "latency_p99_ms": round(float(np.percentile(all_lat, 99)), 3),
         "roofline": {
-            "kernel": "k_expand (light-tier relax, convergence rounds)",
+            "kernel": "k_round (convergence round: light items + spilled chunks of the previous round)",
             "bound": "hbm",
             "achieved": round(achieved, 2) if achieved else None,
             "peak": HBM_PEAK_GBS,

@@ -72,7 +72,8 @@ struct Ctl {
     unsigned long long inval;     // invalidated vertices
     unsigned long long pad[2];
     uint32_t tcount;              // touched vertices
-    uint32_t pad32[3];
+    uint32_t nchunks;             // merge chunks
+    uint32_t pad32[2];
 };
 
 __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
@@ -117,9 +118,9 @@ __global__ void k_leaf_init(uint32_t nv, uint32_t L, uint64_t nleaves, const uin
     }
 }
 
-// Split (u, v) pairs into sort keys; mark touched sources; drop ids >= nv.
+// Split (u, v) pairs into sort keys; drop ids >= nv.
 __global__ void k_split(uint32_t nv, const uint32_t *ins_uv, uint32_t nins, const uint32_t *del_uv, uint32_t ndel,
-                        uint32_t *ins_k, uint32_t *ins_v, unsigned long long *del_k, uint8_t *mark, Ctl *ctl) {
+                        uint32_t *ins_k, uint32_t *ins_v, unsigned long long *del_k, Ctl *ctl) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)nins + ndel; i += stride) {
         if (i < nins) {
@@ -127,25 +128,14 @@ __global__ void k_split(uint32_t nv, const uint32_t *ins_uv, uint32_t nins, cons
             const bool ok = u < nv && v < nv;
             ins_k[i] = ok ? u : kGap;
             ins_v[i] = v;
-            if (ok) mark[u] = 1; else atomicAdd(&ctl->dropped, 1ull);
+            if (!ok) atomicAdd(&ctl->dropped, 1ull);
         } else {
             const uint64_t j = i - nins;
             const uint32_t u = del_uv[2 * j], v = del_uv[2 * j + 1];
             const bool ok = u < nv && v < nv;
             del_k[j] = ok ? (((unsigned long long)u << 32) | v) : ~0ull;
-            if (ok) mark[u] = 1; else atomicAdd(&ctl->dropped, 1ull);
+            if (!ok) atomicAdd(&ctl->dropped, 1ull);
         }
-    }
-}
-
-// Compact marked vertices into a list (order unspecified).
-__global__ void __launch_bounds__(kBlock) k_compact_marks(uint32_t nv, const uint8_t *mark, uint32_t *list,
-                                                          uint32_t *cnt, unsigned long long *overflow) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nv; base += stride) {
-        const uint64_t v = base + threadIdx.x;
-        const bool p = v < nv && mark[v];
-        wave_append(p, (uint32_t)v, list, cnt, nv, overflow);
     }
 }
 
@@ -153,96 +143,43 @@ __device__ __forceinline__ uint64_t off_at(const uint64_t *off, uint32_t nv, uin
     return v >= nv ? cap : off[v];
 }
 
-// New degree of each touched vertex; marks overflowing leaves.
-__global__ void __launch_bounds__(kBlock) k_merge_count(GraphView g, BatchView bv, const uint32_t *touched, Ctl *ctl,
-                                                        uint32_t *newdeg, uint32_t L, unsigned long long *leaf_cnt) {
-    const uint32_t n = ctl->tcount;
-    const uint64_t gw = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
-    const uint32_t lane = lane_id();
-    for (uint64_t t = gw; t < n; t += nw) {
-        const uint32_t u = touched[t];
-        uint32_t ib, ie, db, de;
-        equal_range_u32(bv.ins_key, bv.nins, u, ib, ie);
-        src_range_u64(bv.del_key, bv.ndel, u, db, de);
-        const uint32_t d0 = g.deg[u];
-        const uint64_t k0 = (uint64_t)u << 32;
-        uint32_t nd = 0;
-        if (db == de) {
-            nd = d0 + (ie - ib);
-        } else {
-            const uint32_t *old = g.col + g.off[u];
-            for (uint32_t k = 0; k < d0; k += kWave) {
-                const uint32_t i = k + lane;
-                const bool keep = i < d0 && !in_sorted_u64(bv.del_key, db, de, k0 | old[i]);
-                nd += (uint32_t)__popcll(__ballot(keep));
+// New degree of every touched vertex from the chunk prefix; vertices that
+// outgrow their range mark their leaf dirty. Leaf and edge totals follow.
+__global__ void __launch_bounds__(kBlock) k_newdeg(GraphView g, const TInfo *tinfo, const uint32_t *tcount,
+                                                   const unsigned long long *cpos, uint32_t *newdeg, uint8_t *leaf_dirty,
+                                                   uint32_t L, unsigned long long *leaf_cnt, Ctl *ctl) {
+    const uint32_t n = *tcount;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint64_t t = base + threadIdx.x;
+        long long delta = 0;
+        bool over = false;
+        if (t < n) {
+            const TInfo ti = tinfo[t];
+            const uint32_t nd = has_del(ti) ? (uint32_t)(cpos[ti.cbase + ti.nch] - cpos[ti.cbase]) : ti.d0 + ti.ni;
+            newdeg[ti.u] = nd;
+            if (nd > g.off[ti.u + 1] - g.off[ti.u]) {
+                leaf_dirty[ti.u / L] = 1;
+                over = true;
             }
-            for (uint32_t k = ib; k < ie; k += kWave) {
-                const uint32_t i = k + lane;
-                const bool keep = i < ie && !in_sorted_u64(bv.del_key, db, de, k0 | bv.ins_val[i]);
-                nd += (uint32_t)__popcll(__ballot(keep));
-            }
+            delta = (long long)nd - (long long)ti.d0;
+            if (delta) atomicAdd(&leaf_cnt[ti.u / L], (unsigned long long)delta);
         }
-        if (lane == 0) {
-            newdeg[u] = nd;
-            const long long delta = (long long)nd - (long long)d0;
-            if (delta) {
-                atomicAdd(&leaf_cnt[u / L], (unsigned long long)delta);
-                atomicAdd(&ctl->ne_delta, (unsigned long long)delta);
-            }
-        }
+        const unsigned long long wsum = wave_sum_u64((unsigned long long)delta);
+        if (lane_id() == 0 && wsum) atomicAdd(&ctl->ne_delta, wsum);
+        if (__ballot(over) && lane_id() == 0) atomicOr(&ctl->overflow, 1ull);
     }
 }
 
-// Overflow test is separate so the last vertex can use off[nv] = capacity.
-__global__ void k_overflow(GraphView g, const uint32_t *touched, Ctl *ctl, const uint32_t *newdeg,
-                           uint8_t *leaf_dirty, uint32_t L) {
-    const uint32_t n = ctl->tcount;
+// deg[u] = new degree for touched vertices placed in their own range.
+__global__ void k_set_deg(const TInfo *tinfo, const uint32_t *tcount, const uint32_t *newdeg, uint32_t *deg,
+                          const uint8_t *maximal, uint64_t nleaves, uint32_t L, int skip_covered) {
+    const uint32_t n = *tcount;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t u = touched[t];
-        const uint64_t span = g.off[u + 1] - g.off[u];
-        if (newdeg[u] > span) {
-            leaf_dirty[u / L] = 1;
-            atomicOr(&ctl->overflow, 1ull);
-        }
-    }
-}
-
-// covered(u): some ancestor-or-self node of u's leaf is a maximal window.
-__device__ __forceinline__ bool covered(const uint8_t *maximal, uint64_t nleaves, uint32_t L, uint32_t u) {
-    for (uint64_t node = (uint64_t)(u / L) + nleaves; node >= 1; node >>= 1)
-        if (maximal[node]) return true;
-    return false;
-}
-
-// In-place merge: survivors shift left, inserts append, the freed tail becomes gaps.
-__global__ void __launch_bounds__(kBlock) k_merge_inplace(GraphView g, uint32_t *col, uint32_t *deg, BatchView bv,
-                                                          const uint32_t *touched, const Ctl *ctl, const uint32_t *newdeg,
-                                                          const uint8_t *maximal, uint64_t nleaves, uint32_t L,
-                                                          int skip_covered) {
-    const uint32_t n = ctl->tcount;
-    const uint64_t gw = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
-    const uint32_t lane = lane_id();
-    for (uint64_t t = gw; t < n; t += nw) {
-        const uint32_t u = touched[t];
+        const uint32_t u = tinfo[t].u;
         if (skip_covered && covered(maximal, nleaves, L, u)) continue;
-        uint32_t ib, ie, db, de;
-        equal_range_u32(bv.ins_key, bv.nins, u, ib, ie);
-        src_range_u64(bv.del_key, bv.ndel, u, db, de);
-        const uint32_t d0 = deg[u];
-        uint32_t *lst = col + g.off[u];
-        const uint32_t nd = wave_merge_list(lst, d0, bv, u, ib, ie, db, de, lst, db == de);
-        for (uint32_t k = nd + lane; k < d0; k += kWave) lst[k] = kGap;
-        if (lane == 0) deg[u] = nd;
-        (void)newdeg;
+        deg[u] = newdeg[u];
     }
-}
-
-__global__ void k_clear_marks(const uint32_t *touched, const Ctl *ctl, uint8_t *mark) {
-    const uint32_t n = ctl->tcount;
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x)
-        mark[touched[t]] = 0;
 }
 
 // Upper density threshold at height h of a tree of height H
@@ -318,12 +255,12 @@ __device__ __forceinline__ uint32_t find_window(const unsigned long long *wv_off
 }
 
 __global__ void k_win_nd(const Window *win, uint32_t nwin, const unsigned long long *wv_off, uint64_t Wv,
-                         const uint8_t *mark, const uint32_t *newdeg, const uint32_t *deg, unsigned long long *nd) {
+                         const uint32_t *mark, const uint32_t *newdeg, const uint32_t *deg, unsigned long long *nd) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= Wv; i += (uint64_t)gridDim.x * blockDim.x) {
         if (i == Wv) { nd[i] = 0; continue; }
         const uint32_t w = find_window(wv_off, nwin, i);
         const uint32_t v = win[w].vlo + (uint32_t)(i - wv_off[w]);
-        nd[i] = mark[v] ? newdeg[v] : deg[v];
+        nd[i] = bit_test(mark, v) ? newdeg[v] : deg[v];
     }
 }
 
@@ -340,10 +277,15 @@ __global__ void k_win_params(Window *win, uint32_t nwin, const unsigned long lon
     }
 }
 
-// Merged lists of window vertices, packed into tmp at P[i].
+// Merged lists of window vertices, packed into out at P[i]. A touched
+// vertex's merged list is in `merged` (old survivors then inserts when it had
+// deletes; only its inserts otherwise, after its unchanged old list).
 __global__ void __launch_bounds__(kBlock) k_win_gather(GraphView g, BatchView bv, const Window *win, uint32_t nwin,
                                                        const unsigned long long *wv_off, uint64_t Wv,
-                                                       const unsigned long long *P, const uint8_t *mark, uint32_t *tmp) {
+                                                       const unsigned long long *P, const uint32_t *mark,
+                                                       const uint32_t *tidx, const TInfo *tinfo,
+                                                       const unsigned long long *cpos, const uint32_t *merged,
+                                                       uint32_t *out) {
     const uint64_t gw = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
     const uint32_t lane = lane_id();
@@ -351,16 +293,23 @@ __global__ void __launch_bounds__(kBlock) k_win_gather(GraphView g, BatchView bv
         const uint32_t w = find_window(wv_off, nwin, i);
         const uint32_t v = win[w].vlo + (uint32_t)(i - wv_off[w]);
         const uint32_t *old = g.col + g.off[v];
-        const uint32_t d0 = g.deg[v];
-        uint32_t *dst = tmp + P[i];
-        if (mark[v]) {
-            uint32_t ib, ie, db, de;
-            equal_range_u32(bv.ins_key, bv.nins, v, ib, ie);
-            src_range_u64(bv.del_key, bv.ndel, v, db, de);
-            wave_merge_list(old, d0, bv, v, ib, ie, db, de, dst, false);
-        } else {
-            for (uint32_t k = lane; k < d0; k += kWave) dst[k] = old[k];
+        uint32_t *dst = out + P[i];
+        uint32_t keep_old = g.deg[v];
+        const uint32_t *tail = nullptr;
+        uint64_t ntail = 0;
+        if (bit_test(mark, v)) {
+            const TInfo ti = tinfo[tidx[v]];
+            if (ti.de > ti.db) {  // survivors of old list + inserts, packed by the filter
+                keep_old = 0;
+                tail = merged + cpos[ti.cbase];
+                ntail = cpos[ti.cbase + ti.nch] - cpos[ti.cbase];
+            } else {  // old list unchanged, inserts straight from the sorted batch
+                tail = bv.ins_val + ti.ib;
+                ntail = ti.ni;
+            }
         }
+        for (uint32_t k = lane; k < keep_old; k += kWave) dst[k] = old[k];
+        for (uint64_t k = lane; k < ntail; k += kWave) dst[keep_old + k] = tail[k];
     }
 }
 
@@ -430,8 +379,8 @@ __global__ void __launch_bounds__(kBlock) k_seed_active(uint32_t nv, Op op, cons
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_inval_seed(Op op, const unsigned long long *del_k, uint32_t ndel,
                                                        typename Op::value_t *value, typename Op::buffer_t *buffer,
-                                                       uint32_t *inval, uint32_t *list, uint32_t *cnt, uint32_t cap,
-                                                       unsigned long long *stats, unsigned long long *count) {
+                                                       uint32_t *inval, uint32_t *coarse, uint32_t *list, uint32_t *cnt,
+                                                       uint32_t cap, unsigned long long *stats, unsigned long long *count) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < ndel; base += stride) {
         const uint64_t i = base + threadIdx.x;
@@ -444,6 +393,7 @@ __global__ void __launch_bounds__(kBlock) k_inval_seed(Op op, const unsigned lon
             if (Op::Parent(b) == u) {
                 const uint32_t bit = 1u << (v & 31);
                 if (!(atomicOr(&inval[v >> 5], bit) & bit)) {
+                    atomicOr(&coarse[v >> 11], 1u << ((v >> 6) & 31));
                     typename Op::buffer_t bb = b;
                     op.AccumulateBuffer_del(u, v, &bb, &value[v]);
                     buffer[v] = bb;
@@ -458,7 +408,7 @@ __global__ void __launch_bounds__(kBlock) k_inval_seed(Op op, const unsigned lon
 
 // Relax each inserted edge that survived the batch's deletes.
 template <class Op>
-__global__ void __launch_bounds__(kBlock) k_insert_relax(Op op, BatchView bv, typename Op::buffer_t *buffer,
+__global__ void __launch_bounds__(kBlock) k_insert_relax(Op op, BatchView bv, BatchRanges br, typename Op::buffer_t *buffer,
                                                          uint32_t *stamp, uint32_t stamp_val, uint32_t wmode,
                                                          uint64_t wseed, uint32_t *list, uint32_t *cnt, uint32_t cap,
                                                          unsigned long long *stats) {
@@ -471,8 +421,7 @@ __global__ void __launch_bounds__(kBlock) k_insert_relax(Op op, BatchView bv, ty
             const uint32_t u = bv.ins_key[i];
             v = bv.ins_val[i];
             const unsigned long long key = ((unsigned long long)u << 32) | v;
-            uint32_t db, de;
-            src_range_u64(bv.del_key, bv.ndel, u, db, de);
+            const uint32_t db = br.del_b[u], de = br.del_e[u];
             if (db == de || !in_sorted_u64(bv.del_key, db, de, key)) {
                 const uint32_t w = Op::kWeighted ? sg_weight(wmode, wseed, u, v) : 0u;
                 if (op.AccumulateBuffer(u, v, w, &buffer[v], op.PushValue(ld_relaxed(&buffer[u]))))
@@ -608,13 +557,14 @@ public:
     uint32_t *q_[2] = {nullptr, nullptr};
     uint32_t *cnt_ = nullptr;    // [3] rotating round counters
     uint32_t *hcnt_ = nullptr;   // [2] rotating heavy counters
-    HeavyDesc *heavy_ = nullptr;
+    HeavyDesc *heavy_[3] = {nullptr, nullptr, nullptr};  // rotating spill queues
     size_t heavy_cap_ = 0;
     unsigned long long *stats_ = nullptr;  // [0..3] pass stats, [4] inval count
     uint32_t *stamp_ = nullptr;
-    uint32_t *inval_ = nullptr;  // bitmap
+    uint32_t *inval_ = nullptr;   // bitmap of invalidated vertices
+    uint32_t *coarse_ = nullptr;  // one bit per 64 vertices of inval_
+    int num_cus_ = 0;
     uint32_t R_ = 1;             // global round counter (stamps)
-    uint64_t P_ = 0;             // global pass counter (heavy counters)
 
     // --- batch --------------------------------------------------------------
     uint32_t *raw_ins_ = nullptr, *raw_del_ = nullptr;
@@ -625,8 +575,21 @@ public:
     size_t del_cap_ = 0, del_cap2_ = 0;
     uint8_t *cub_tmp_ = nullptr;
     size_t cub_cap_ = 0;
-    uint8_t *mark_ = nullptr;
-    uint32_t *touched_ = nullptr, *newdeg_ = nullptr;
+    uint32_t *mark_ = nullptr;    // bitmap of touched sources
+    BatchRanges br_{};             // per-vertex [begin, end) of its inserts / deletes in the sorted batch
+    unsigned long long *dpre_ = nullptr;  // exclusive prefix of deg (edge-centric sweep)
+    uint32_t *tile_owner_ = nullptr;      // source of the first live edge of each sweep tile
+    size_t tile_owner_cap_ = 0;
+    uint32_t *touched_ = nullptr, *newdeg_ = nullptr, *tidx_ = nullptr;
+    TInfo *tinfo_ = nullptr;
+    size_t tinfo_cap_ = 0;
+    MChunk *chunks_ = nullptr;
+    size_t chunks_cap_ = 0;
+    unsigned long long *ccount_ = nullptr, *cpos_ = nullptr;
+    size_t ccount_cap_ = 0, cpos_cap_ = 0;
+    uint64_t merged_total_ = 0, chunk_bound_ = 0;
+    uint32_t *tmp2_ = nullptr;
+    size_t tmp2_cap_ = 0;
     Ctl *ctl_ = nullptr;
     uint32_t *tmp_ = nullptr;
     size_t tmp_cap_ = 0;
@@ -658,11 +621,12 @@ public:
         hipDeviceProp_t prop;
         SG_HIP(hipGetDeviceProperties(&prop, cfg_.device));
         if (prop.warpSize != 64) fail(SG_EDEVICE, "wave64 device required");
+        num_cus_ = prop.multiProcessorCount;
         SG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         SG_HIP(hipHostMalloc(&pinned_, 65536, hipHostMallocDefault));
         h_ctl_ = (Ctl *)pinned_;
-        h_cnt_ = (uint32_t *)((char *)pinned_ + 4096);
-        h_u64_ = (unsigned long long *)((char *)pinned_ + 8192);
+        h_cnt_ = (uint32_t *)((char *)pinned_ + 4096);             // 256 rounds x 8 counters
+        h_u64_ = (unsigned long long *)((char *)pinned_ + 16384);
     }
 
     void sync() { SG_HIP(hipStreamSynchronize(stream_)); }
@@ -712,12 +676,19 @@ public:
         maximal_ = dalloc<uint8_t>(2 * nleaves_);
         q_[0] = dalloc<uint32_t>(nv);
         q_[1] = dalloc<uint32_t>(nv);
-        cnt_ = dalloc<uint32_t>(4);
-        hcnt_ = dalloc<uint32_t>(2);
+        cnt_ = dalloc<uint32_t>(8);  // [0..2] frontier counters, [3..5] spill counters
+        hcnt_ = cnt_ + 3;
         stats_ = dalloc<unsigned long long>(8);
         stamp_ = dalloc<uint32_t>(nv);
         inval_ = dalloc<uint32_t>(((size_t)nv + 31) / 32);
-        mark_ = dalloc<uint8_t>(nv);
+        coarse_ = dalloc<uint32_t>(((size_t)nv + 2047) / 2048);
+        mark_ = dalloc<uint32_t>(((size_t)nv + 31) / 32);
+        tidx_ = dalloc<uint32_t>(nv);
+        br_.ins_b = dalloc<uint32_t>(nv);
+        br_.ins_e = dalloc<uint32_t>(nv);
+        br_.del_b = dalloc<uint32_t>(nv);
+        br_.del_e = dalloc<uint32_t>(nv);
+        dpre_ = dalloc<unsigned long long>((size_t)nv + 1);
         touched_ = dalloc<uint32_t>(nv);
         newdeg_ = dalloc<uint32_t>(nv);
         ctl_ = dalloc<Ctl>(1);
@@ -740,10 +711,13 @@ public:
         fill<uint8_t>(leaf_dirty_, nleaves_, 0);
         fill<uint8_t>(chosen_, 2 * nleaves_, 0);
         fill<uint8_t>(maximal_, 2 * nleaves_, 0);
-        fill<uint8_t>(mark_, nv, 0);
+        fill<uint32_t>(mark_, ((uint64_t)nv + 31) / 32, 0u);
+        fill<uint32_t>(br_.ins_b, nv, 0u);
+        fill<uint32_t>(br_.ins_e, nv, 0u);
+        fill<uint32_t>(br_.del_b, nv, 0u);
+        fill<uint32_t>(br_.del_e, nv, 0u);
         fill<uint32_t>(stamp_, nv, 0);
-        fill<uint32_t>(cnt_, 4, 0);
-        fill<uint32_t>(hcnt_, 2, 0);
+        fill<uint32_t>(cnt_, 8, 0);
         fill<unsigned long long>(stats_, 8, 0ull);
         fill<uint8_t>((uint8_t *)ctl_, sizeof(Ctl), 0);
         sync();
@@ -755,49 +729,92 @@ public:
     }
 
     void ensure_heavy(uint64_t edges) {
-        const size_t need = (size_t)(edges / kChunk + edges / kHeavyDeg + 4096);
+        // every list past a wave's budget becomes >= 1 descriptor: at most one per
+        // vertex plus one per kChunk edges, per pass
+        const size_t need = (size_t)(edges / kChunk + nv_ + 4096);
         if (need > heavy_cap_) {
-            dfree(heavy_);
-            heavy_ = nullptr;
-            heavy_ = dalloc<HeavyDesc>(need);
+            for (int i = 0; i < 3; ++i) {
+                dfree(heavy_[i]);
+                heavy_[i] = nullptr;
+            }
+            for (int i = 0; i < 3; ++i) {
+                heavy_[i] = dalloc<HeavyDesc>(need);
+                // zeroed: a slot never written holds an empty (start 0, len 1) piece
+                fill<uint32_t>((uint32_t *)heavy_[i], need * (sizeof(HeavyDesc) / 4), 0u);
+            }
             heavy_cap_ = need;
-            // zeroed: a descriptor slot never read before it is written has len 0
-            fill<uint32_t>((uint32_t *)heavy_, need * (sizeof(HeavyDesc) / 4), 0u);
         }
     }
 
     GraphView gview() const { return GraphView{nv_, off_, deg_, col_}; }
+    uint32_t budget() const { return cfg_.wave_budget ? cfg_.wave_budget : kWaveBudget; }
+
+    // dpre_[v] = sum of deg[0..v) for v in [0, nv], accumulated in 64 bits.
+    struct DegAt {
+        const uint32_t *d;
+        uint32_t nv;
+        __host__ __device__ unsigned long long operator()(uint32_t i) const { return i < nv ? d[i] : 0ull; }
+    };
+    void degree_prefix() {
+        hipcub::CountingInputIterator<uint32_t> cnt(0);
+        hipcub::TransformInputIterator<unsigned long long, DegAt, hipcub::CountingInputIterator<uint32_t>> in(cnt,
+                                                                                                            DegAt{deg_, nv_});
+        size_t b = 0;
+        SG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, in, dpre_, (int)nv_ + 1, stream_));
+        ensure(cub_tmp_, cub_cap_, b + 256);
+        b = cub_cap_;
+        SG_HIP(hipcub::DeviceScan::ExclusiveSum(cub_tmp_, b, in, dpre_, (int)nv_ + 1, stream_));
+    }
+    uint32_t heavy_deg() const { return std::max<uint32_t>(kChunk, cfg_.heavy_degree ? cfg_.heavy_degree : kHeavyDeg); }
 
     // ---- passes and rounds ---------------------------------------------------------
-    // Zero the counters of round R and R+1.
-    void frontier_reset() {
-        fill<uint32_t>(cnt_, 3, 0);
-    }
+    // Round R reads frontier q[R & 1] (count cnt[R % 3]) and the descriptors
+    // spilled by round R - 1 (hq[(R + 2) % 3]); it appends to q[(R + 1) & 1]
+    // and spills into hq[R % 3]; it clears cnt[(R + 2) % 3] and hc[(R + 1) % 3],
+    // whose readers have finished. Seed passes before round R append to
+    // round R's frontier and spill into hq[(R + 2) % 3].
+    void frontier_reset() { fill<uint32_t>(cnt_, 8, 0); }  // word 7 stays zero: an empty list count
 
-    PassIO pass_io(const uint32_t *in_list, const uint32_t *in_cnt, uint32_t range_lo, uint32_t range_n,
-                   uint32_t *out_list, uint32_t *out_cnt, uint32_t *zero_cnt) {
-        PassIO io;
+    PassIO base_io() const {
+        PassIO io{};
+        io.in_cap = nv_;
+        io.out_cap = nv_;
+        io.heavy_cap = (uint32_t)std::min<size_t>(heavy_cap_, 0xFFFFFFFFu);
+        io.budget = budget();
+        io.heavy_deg = heavy_deg();
+        io.stats = stats_;
+        return io;
+    }
+    PassIO round_io(uint32_t R) const {
+        PassIO io = base_io();
+        io.in_list = q_[R & 1];
+        io.in_cnt = cnt_ + (R % 3);
+        io.out_list = q_[(R + 1) & 1];
+        io.out_cnt = cnt_ + ((R + 1) % 3);
+        io.zero_cnt = cnt_ + ((R + 2) % 3);
+        io.heavy = heavy_[R % 3];
+        io.heavy_cnt = hcnt_ + (R % 3);
+        io.hin = heavy_[(R + 2) % 3];
+        io.hin_cnt = hcnt_ + ((R + 2) % 3);
+        io.heavy_zero = hcnt_ + ((R + 1) % 3);
+        return io;
+    }
+    // A seed pass over a list (in_list, in_cnt) or the range [0, range_n).
+    PassIO seed_io(uint32_t R, const uint32_t *in_list, const uint32_t *in_cnt, uint32_t range_n) const {
+        PassIO io = base_io();
         io.in_list = in_list;
         io.in_cnt = in_cnt;
-        io.in_cap = nv_;
-        io.range_lo = range_lo;
         io.range_n = range_n;
-        io.out_list = out_list;
-        io.out_cnt = out_cnt;
-        io.out_cap = nv_;
-        io.zero_cnt = zero_cnt;
-        io.heavy = heavy_;
-        io.heavy_cnt = hcnt_ + (P_ & 1);
-        io.heavy_zero = hcnt_ + ((P_ + 1) & 1);
-        io.heavy_cap = (uint32_t)std::min<size_t>(heavy_cap_, 0xFFFFFFFFu);
-        io.stats = stats_;
+        io.out_list = q_[R & 1];
+        io.out_cnt = cnt_ + (R % 3);
+        io.heavy = heavy_[(R + 2) % 3];
+        io.heavy_cnt = hcnt_ + ((R + 2) % 3);
         return io;
     }
 
     template <class VFn, class EFn>
     void launch_pass(const PassIO &io, VFn vfn, EFn efn, bool timed) {
-        const uint32_t g1 = io.in_list ? kGridList : grid_for(io.range_n, kWave * kWavesPerBlock, 16384);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const uint32_t g1 = io.in_list ? kGridList : grid_for(io.range_n, kWave * kWavesPerBlock, kGridList);
         if (timed) {
             if (ev_used_ == events_.size()) {
                 hipEvent_t a, b;
@@ -805,17 +822,27 @@ public:
                 SG_HIP(hipEventCreate(&b));
                 events_.push_back({a, b});
             }
-            e0 = events_[ev_used_].first;
-            e1 = events_[ev_used_].second;
-            ++ev_used_;
-            SG_HIP(hipEventRecord(e0, stream_));
+            SG_HIP(hipEventRecord(events_[ev_used_].first, stream_));
         }
-        hipLaunchKernelGGL((k_expand<VFn, EFn>), dim3(g1), dim3(kBlock), 0, stream_, gview(), io, vfn, efn);
+        hipLaunchKernelGGL((k_round<VFn, EFn>), dim3(g1), dim3(kBlock), 0, stream_, gview(), io, vfn, efn);
         SG_LAUNCHED();
-        if (timed) SG_HIP(hipEventRecord(e1, stream_));
-        hipLaunchKernelGGL((k_expand_heavy<EFn>), dim3(kGridHeavy), dim3(kBlock), 0, stream_, gview(), io, efn);
-        SG_LAUNCHED();
-        ++P_;
+        if (timed) SG_HIP(hipEventRecord(events_[ev_used_++].second, stream_));
+    }
+
+    // Walk, with efn, the descriptors that a seed pass before round R spilled
+    // (hq[(R + 2) % 3]), then empty that queue. For a pass whose chunks must not
+    // outlive the slot layout they index (PR cancel, before the PMA commit).
+    template <class EFn>
+    void drain_seed_spill(uint32_t R, EFn efn) {
+        PassIO io = base_io();
+        io.in_list = q_[R & 1];
+        io.in_cnt = cnt_ + 7;  // zero: no light items
+        io.out_list = q_[R & 1];
+        io.out_cnt = cnt_ + (R % 3);
+        io.hin = heavy_[(R + 2) % 3];
+        io.hin_cnt = hcnt_ + ((R + 2) % 3);
+        launch_pass(io, VAll{}, efn, false);
+        fill<uint32_t>(hcnt_ + ((R + 2) % 3), 1, 0u);
     }
 
     // Harvest kernel timings of completed events (call after a sync).
@@ -835,8 +862,10 @@ public:
         std::memcpy(out, h_u64_, 8 * sizeof(unsigned long long));
     }
 
-    // Run rounds from the frontier of round R_ until one produces nothing.
-    // make_fns(stamp_for_next) returns {vfn, efn}. Returns rounds executed.
+    // Run rounds from the frontier of round R_ until a round activates nothing
+    // and spills nothing; the host reads the counters every rounds_per_sync
+    // rounds (rounds past the last busy one find empty inputs and return at
+    // once). make_fns(stamp_for_next) returns {vfn, efn}. Returns the rounds run.
     template <class MakeFns>
     uint32_t run_rounds(MakeFns make_fns, bool timed) {
         const uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(cfg_.rounds_per_sync ? cfg_.rounds_per_sync : 4, 256));
@@ -848,18 +877,18 @@ public:
         for (;;) {
             for (uint32_t i = 0; i < k; ++i) {
                 const uint32_t R = R_;
-                PassIO io = pass_io(q_[R & 1], cnt_ + (R % 3), 0, 0, q_[(R + 1) & 1], cnt_ + ((R + 1) % 3),
-                                    cnt_ + ((R + 2) % 3));
                 auto fns = make_fns(R + 1);
-                launch_pass(io, fns.first, fns.second, do_time);
-                SG_HIP(hipMemcpyAsync(h_cnt_ + i, cnt_ + ((R + 1) % 3), sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+                launch_pass(round_io(R), fns.first, fns.second, do_time);
+                // counters after round R: its frontier and its spill
+                SG_HIP(hipMemcpyAsync(h_cnt_ + 8 * i, cnt_, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
                 ++R_;
             }
             sync();
             if (do_time) harvest_events();
             for (uint32_t i = 0; i < k; ++i) {
                 ++rounds;
-                if (h_cnt_[i] == 0) {
+                const uint32_t R = R_ - k + i;
+                if (h_cnt_[8 * i + ((R + 1) % 3)] == 0 && h_cnt_[8 * i + 3 + (R % 3)] == 0) {
                     finish_rounds(do_time, s0);
                     return rounds;
                 }
@@ -895,7 +924,8 @@ public:
         d_del = raw_del_;
     }
 
-    // Sort the batch, list touched sources, count new degrees. Leaves the graph unchanged.
+    // Sort the batch, list touched sources, cut their lists into chunks, count
+    // survivors and new degrees. Leaves the graph unchanged.
     bool prepare(const uint32_t *d_ins, uint64_t nins64, const uint32_t *d_del, uint64_t ndel64) {
         if (nins64 > 0x7FFFFFFFull || ndel64 > 0x7FFFFFFFull) fail(SG_EINVAL, "batch too large (max 2^31-1 pairs)");
         const uint32_t nins = (uint32_t)nins64, ndel = (uint32_t)ndel64;
@@ -908,16 +938,26 @@ public:
         ensure(del_k_, del_cap_, ndel + 1);
         ensure(del_k2_, del_cap2_, ndel + 1);
         ensure_heavy(ne_ + nins);
+        // chunks: sum over touched vertices of ceil(deg/CH) (with deletes) + ceil(ins/CH)
+        const uint64_t chunk_bound = (ne_ + nins) / kMergeChunk + (uint64_t)nins + ndel + 64;
+        if (chunk_bound > 0xFFFFFFF0ull) fail(SG_EINVAL, "batch too large for the merge chunk list");
+        chunk_bound_ = chunk_bound;
+        ensure(tinfo_, tinfo_cap_, (size_t)nins + ndel + 1);
+        ensure(chunks_, chunks_cap_, chunk_bound);
+        ensure(ccount_, ccount_cap_, chunk_bound + 1);
+        ensure(cpos_, cpos_cap_, chunk_bound + 1);
         fill<uint8_t>((uint8_t *)ctl_, sizeof(Ctl), 0);
+        fill<unsigned long long>(ccount_, chunk_bound + 1, 0ull);
         if (nins + ndel) {
             hipLaunchKernelGGL(k_split, dim3(grid_for((uint64_t)nins + ndel, 256, 8192)), dim3(256), 0, stream_, nv_, d_ins, nins,
-                               d_del, ndel, ins_k_, ins_v_, del_k_, mark_, ctl_);
+                               d_del, ndel, ins_k_, ins_v_, del_k_, ctl_);
             SG_LAUNCHED();
         }
-        size_t b1 = 0, b2 = 0;
+        size_t b1 = 0, b2 = 0, b3 = 0;
         SG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, ins_k_, ins_k2_, ins_v_, ins_v2_, (int)nins, 0, 32, stream_));
         SG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, b2, del_k_, del_k2_, (int)ndel, 0, 64, stream_));
-        ensure(cub_tmp_, cub_cap_, std::max(b1, b2) + 256);
+        SG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, ccount_, cpos_, (int)(chunk_bound + 1), stream_));
+        ensure(cub_tmp_, cub_cap_, std::max({b1, b2, b3}) + 256);
         size_t b = cub_cap_;
         if (nins) SG_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp_, b, ins_k_, ins_k2_, ins_v_, ins_v2_, (int)nins, 0, 32, stream_));
         b = cub_cap_;
@@ -927,16 +967,30 @@ public:
         bv_.nins = nins;
         bv_.del_key = (const uint64_t *)del_k2_;
         bv_.ndel = ndel;
-        hipLaunchKernelGGL(k_compact_marks, dim3(grid_for(nv_, kBlock, 8192)), dim3(kBlock), 0, stream_, nv_, mark_, touched_,
-                           &ctl_->tcount, &ctl_->pad[0]);
-        SG_LAUNCHED();
-        hipLaunchKernelGGL(k_merge_count, dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_, touched_, ctl_, newdeg_,
-                           L_, leaf_cnt_);
-        SG_LAUNCHED();
-        hipLaunchKernelGGL(k_overflow, dim3(1024), dim3(256), 0, stream_, gview(), touched_, ctl_, newdeg_, leaf_dirty_, L_);
-        SG_LAUNCHED();
+        if (nins + ndel) {
+            hipLaunchKernelGGL(k_touched, dim3(grid_for((uint64_t)nins + ndel, kBlock, 8192)), dim3(kBlock), 0, stream_, bv_,
+                               br_, mark_, touched_, &ctl_->tcount, nv_, &ctl_->pad[0]);
+            SG_LAUNCHED();
+            const uint32_t gt = grid_for((uint64_t)nins + ndel, kBlock, 8192);
+            hipLaunchKernelGGL(k_touch_info, dim3(gt), dim3(kBlock), 0, stream_, gview(), br_, touched_, &ctl_->tcount, tinfo_,
+                               tidx_, chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, &ctl_->pad[1]);
+            SG_LAUNCHED();
+            hipLaunchKernelGGL((k_chunk_filter<0>), dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_, tinfo_, chunks_,
+                               &ctl_->nchunks, (uint32_t)chunk_bound, ccount_, cpos_, tmp_);
+            SG_LAUNCHED();
+        }
+        b = cub_cap_;
+        SG_HIP(hipcub::DeviceScan::ExclusiveSum(cub_tmp_, b, ccount_, cpos_, (int)(chunk_bound + 1), stream_));
+        if (nins + ndel) {
+            hipLaunchKernelGGL(k_newdeg, dim3(grid_for((uint64_t)nins + ndel, kBlock, 8192)), dim3(kBlock), 0, stream_, gview(),
+                               tinfo_, &ctl_->tcount, cpos_, newdeg_, leaf_dirty_, L_, leaf_cnt_, ctl_);
+            SG_LAUNCHED();
+        }
         SG_HIP(hipMemcpyAsync(h_ctl_, ctl_, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+        SG_HIP(hipMemcpyAsync(h_u64_, cpos_ + chunk_bound, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
         sync();
+        if (h_ctl_->pad[0] || h_ctl_->pad[1]) fail(SG_EDEVICE, "internal touched/chunk list overflow");
+        merged_total_ = h_u64_[0];
         st_.dropped_updates += h_ctl_->dropped;
         return h_ctl_->overflow != 0;
     }
@@ -944,18 +998,36 @@ public:
     void commit(bool overflow) {
         const Ctl c = *h_ctl_;
         const uint64_t ne_new = ne_ + (uint64_t)(long long)c.ne_delta;
-        if (!overflow) {
-            hipLaunchKernelGGL(k_merge_inplace, dim3(kGridList), dim3(kBlock), 0, stream_, gview(), col_, deg_, bv_, touched_,
-                               ctl_, newdeg_, maximal_, nleaves_, L_, 0);
+        if (nins_ + ndel_) {
+            ensure(tmp_, tmp_cap_, std::max<uint64_t>(merged_total_, 1));
+            hipLaunchKernelGGL((k_chunk_filter<1>), dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_, tinfo_, chunks_,
+                               &ctl_->nchunks, (uint32_t)chunk_bound_, ccount_, cpos_, tmp_);
             SG_LAUNCHED();
-        } else {
-            rebalance(ne_new);
+            if (overflow) rebalance(ne_new);  // moves the touched vertices of its windows
+            hipLaunchKernelGGL(k_chunk_place, dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_, col_, tinfo_, chunks_,
+                               &ctl_->nchunks, (uint32_t)chunk_bound_, cpos_, tmp_, newdeg_, maximal_, nleaves_, L_,
+                               overflow ? 1 : 0);
+            SG_LAUNCHED();
+            hipLaunchKernelGGL(k_lane_place, dim3(grid_for(c.tcount, 256, 4096)), dim3(256), 0, stream_, gview(), bv_, col_,
+                               tinfo_, &ctl_->tcount, maximal_, nleaves_, L_, overflow ? 1 : 0);
+            SG_LAUNCHED();
+            hipLaunchKernelGGL(k_set_deg, dim3(1024), dim3(256), 0, stream_, tinfo_, &ctl_->tcount, newdeg_, deg_, maximal_,
+                               nleaves_, L_, overflow ? 1 : 0);
+            SG_LAUNCHED();
+            if (overflow) fill<uint8_t>(maximal_, 2 * nleaves_, 0);
         }
-        hipLaunchKernelGGL(k_clear_marks, dim3(1024), dim3(256), 0, stream_, touched_, ctl_, mark_);
-        SG_LAUNCHED();
         ne_ = ne_new;
         st_.ne = ne_;
         st_.capacity = cap_;
+    }
+
+    // The batch's per-vertex ranges and marks are kept until the snapshot's
+    // recompute has used them; then they are cleared for the next batch.
+    void finish_batch() {
+        if (nins_ + ndel_ == 0) return;
+        hipLaunchKernelGGL(k_clear_ranges, dim3(1024), dim3(256), 0, stream_, tinfo_, &ctl_->tcount, br_);
+        SG_LAUNCHED();
+        fill<uint32_t>(mark_, ((uint64_t)nv_ + 31) / 32, 0u);
     }
 
     void rebalance(uint64_t ne_new) {
@@ -1024,19 +1096,14 @@ public:
         hipLaunchKernelGGL(k_win_params, dim3(grid_for(nwin, 256, 8192)), dim3(256), 0, stream_, win_, nwin, wvoff_, P_arr_,
                            off_, nv_, cap_, new_cap);
         SG_LAUNCHED();
-        if (!resize) {
-            hipLaunchKernelGGL(k_merge_inplace, dim3(kGridList), dim3(kBlock), 0, stream_, gview(), col_, deg_, bv_, touched_,
-                               ctl_, newdeg_, maximal_, nleaves_, L_, 1);
-            SG_LAUNCHED();
-        }
-        ensure(tmp_, tmp_cap_, std::max<uint64_t>(ne_new, 1));
+        ensure(tmp2_, tmp2_cap_, std::max<uint64_t>(ne_new, 1));
         const uint32_t gw = grid_for(Wv, kWavesPerBlock, 16384);
-        hipLaunchKernelGGL(k_win_gather, dim3(gw), dim3(kBlock), 0, stream_, gview(), bv_, win_, nwin, wvoff_, Wv, P_arr_,
-                           mark_, tmp_);
+        hipLaunchKernelGGL(k_win_gather, dim3(gw), dim3(kBlock), 0, stream_, gview(), bv_, win_, nwin, wvoff_, Wv, P_arr_, mark_,
+                           tidx_, tinfo_, cpos_, tmp_, tmp2_);
         SG_LAUNCHED();
         uint32_t *dst = col_;
         if (resize) dst = dalloc<uint32_t>(new_cap);
-        hipLaunchKernelGGL(k_win_scatter, dim3(gw), dim3(kBlock), 0, stream_, win_, nwin, wvoff_, Wv, P_arr_, tmp_, dst, off_,
+        hipLaunchKernelGGL(k_win_scatter, dim3(gw), dim3(kBlock), 0, stream_, win_, nwin, wvoff_, Wv, P_arr_, tmp2_, dst, off_,
                            deg_);
         SG_LAUNCHED();
         if (resize) {
@@ -1047,7 +1114,6 @@ public:
             col_ = dst;
             cap_ = new_cap;
         }
-        fill<uint8_t>(maximal_, 2 * nleaves_, 0);
     }
 
     // ---- interface ------------------------------------------------------------------------
@@ -1122,15 +1188,16 @@ public:
         // 1. invalidate the subtrees hanging from deleted tree edges (fixed point)
         const uint32_t nwords = (nv_ + 31) / 32;
         fill<uint32_t>(inval_, nwords, 0u);
+        fill<uint32_t>(coarse_, ((uint64_t)nv_ + 2047) / 2048, 0u);
         fill<unsigned long long>(stats_ + 4, 1, 0ull);
         frontier_reset();
         if (ndel_) {
             hipLaunchKernelGGL(k_inval_seed<Op>, dim3(grid_for(ndel_, kBlock, 8192)), dim3(kBlock), 0, stream_, op_,
-                               (const unsigned long long *)bv_.del_key, ndel_, value_, buffer_, inval_, q_[R_ & 1],
-                               cnt_ + (R_ % 3), nv_, stats_, stats_ + 4);
+                               (const unsigned long long *)bv_.del_key, ndel_, value_, buffer_, inval_, coarse_,
+                               q_[R_ & 1], cnt_ + (R_ % 3), nv_, stats_, stats_ + 4);
             SG_LAUNCHED();
             run_rounds([this](uint32_t) {
-                return std::make_pair(VAll{}, EInvalidate<Op>{op_, value_, buffer_, inval_, stats_ + 4});
+                return std::make_pair(VAll{}, EInvalidate<Op>{op_, value_, buffer_, inval_, coarse_, stats_ + 4});
             }, false);
         }
         unsigned long long s[8];
@@ -1140,19 +1207,32 @@ public:
         frontier_reset();
         const uint32_t R = R_;
         if (s[4]) {
-            PassIO io = pass_io(nullptr, nullptr, 0, nv_, q_[R & 1], cnt_ + (R % 3), nullptr);
-            VSweep<Op> vf{op_, value_, buffer_, inval_, stamp_, R};
-            ERelaxInval<Op> ef{ERelax<Op>{op_, buffer_, stamp_, R, cfg_.weight_mode, cfg_.weight_seed}, inval_};
-            launch_pass(io, vf, ef, false);
+            // edges into invalidated vertices: edge-centric sweep over all live edges
+            degree_prefix();
+            const uint64_t ntiles = (ne_ + kSweepTile - 1) / kSweepTile;
+            ensure(tile_owner_, tile_owner_cap_, ntiles + 1);
+            hipLaunchKernelGGL(k_tile_owners, dim3(grid_for(nv_, 256, 16384)), dim3(256), 0, stream_, nv_, dpre_, tile_owner_);
+            SG_LAUNCHED();
+            PassIO io = seed_io(R, nullptr, nullptr, nv_);
+            ERelaxInval<Op> ef{ERelax<Op>{op_, buffer_, stamp_, R, cfg_.weight_mode, cfg_.weight_seed}, inval_, coarse_};
+            const uint32_t tiles = (uint32_t)std::min<uint64_t>(ntiles, 16384);
+            hipLaunchKernelGGL((k_sweep_edges<Op, ERelaxInval<Op>>), dim3(std::max<uint32_t>(tiles, 1)), dim3(kBlock), 0,
+                               stream_, gview(), dpre_, tile_owner_, op_, buffer_, ef, io);
+            SG_LAUNCHED();
+            // invalidated vertices active on their own (CC: own label)
+            hipLaunchKernelGGL(k_inval_activate<Op>, dim3(grid_for((nv_ + 31) / 32, 256, 4096)), dim3(256), 0, stream_, nv_,
+                               op_, inval_, value_, buffer_, stamp_, R, q_[R & 1], cnt_ + (R % 3), nv_, stats_ + 3);
+            SG_LAUNCHED();
         }
         if (nins_) {
             hipLaunchKernelGGL(k_insert_relax<Op>, dim3(grid_for(nins_, kBlock, 8192)), dim3(kBlock), 0, stream_, op_, bv_,
-                               buffer_, stamp_, R, cfg_.weight_mode, cfg_.weight_seed, q_[R & 1], cnt_ + (R % 3), nv_,
+                               br_, buffer_, stamp_, R, cfg_.weight_mode, cfg_.weight_seed, q_[R & 1], cnt_ + (R % 3), nv_,
                                stats_);
             SG_LAUNCHED();
         }
         // 3. converge
         st_.rounds = run_rounds([this](uint32_t sv) { return relax_fns(sv); }, true);
+        finish_batch();
         st_.snapshots += 1;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -1239,17 +1319,20 @@ public:
         const uint32_t R = R_;
         const uint32_t *tc = &ctl_->tcount;
         {
-            PassIO io = pass_io(touched_, tc, 0, 0, q_[R & 1], cnt_ + (R % 3), nullptr);
+            PassIO io = seed_io(R, touched_, tc, 0);
             launch_pass(io, VAmend<api::PageRank>{op_, -1, value_, buffer_},
                         ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0}, false);
+            // long old lists were cut into chunks: walk them before the commit moves them
+            drain_seed_spill(R, ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0});
         }
         commit(overflow);
         {
-            PassIO io = pass_io(touched_, tc, 0, 0, q_[R & 1], cnt_ + (R % 3), nullptr);
+            PassIO io = seed_io(R, touched_, tc, 0);
             launch_pass(io, VAmend<api::PageRank>{op_, +1, value_, buffer_},
                         ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0}, false);
         }
         st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, true);
+        finish_batch();
         st_.snapshots += 1;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         residual();

@@ -30,12 +30,13 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kHeavyDeg = 2048;         // lists longer than this go to the heavy tier
 constexpr uint32_t kChunk = 1024;            // edges per heavy descriptor
 
+// A kChunk-edge piece of an adjacency list: (start << kLenBits | len - 1),
+// source, push value bits.
+constexpr uint32_t kLenBits = 11;
 struct HeavyDesc {
-    uint64_t start;
-    uint32_t len;
+    uint64_t start_len;
     uint32_t u;
-    uint32_t push;  // push value bits
-    uint32_t pad;
+    uint32_t push;
 };
 
 // Graph as seen by kernels: PMA CSR with per-vertex gaps.
@@ -55,11 +56,15 @@ struct PassIO {
     uint32_t *out_list;       // next frontier
     uint32_t *out_cnt;
     uint32_t out_cap;
-    uint32_t *zero_cnt;       // a counter this pass clears for the pass after next (or null)
-    HeavyDesc *heavy;
-    uint32_t *heavy_cnt;      // appended by the light kernel, read by the heavy kernel
-    uint32_t *heavy_zero;     // cleared by the heavy kernel (the next pass's heavy_cnt)
+    uint32_t *zero_cnt;       // frontier counter this pass clears for a later round (or null)
+    HeavyDesc *heavy;         // descriptors this pass spills (processed by the next pass)
+    uint32_t *heavy_cnt;
+    const HeavyDesc *hin;     // descriptors spilled by the previous pass (or null)
+    const uint32_t *hin_cnt;
+    uint32_t *heavy_zero;     // descriptor counter this pass clears for a later round (or null)
     uint32_t heavy_cap;
+    uint32_t budget;          // light-tier edges per 64-item batch of a wave
+    uint32_t heavy_deg;       // lists longer than this always go to the heavy tier
     unsigned long long *stats;  // [0] edges (light), [1] vertices, [2] edges (heavy), [3] overflow
 };
 
@@ -110,27 +115,112 @@ template <> __device__ __forceinline__ float from_bits<float>(uint32_t x) { retu
 __device__ __forceinline__ uint64_t ld_relaxed(const unsigned long long *p) {
     return __hip_atomic_load(const_cast<unsigned long long *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Plain (cacheable) load for pre-tests of values that only decrease during a
+// phase: a stale copy is never below the current value, so a pre-test that
+// passes on the current value also passes on the copy.
+__device__ __forceinline__ uint64_t ld_peek(const unsigned long long *p) { return *p; }
 
-// ---- light tier ------------------------------------------------------------
-// Each wave takes 64 work items, runs the vertex functor on them, stages the
-// scheduled adjacency lists (start, prefix, source, push) in LDS and walks
-// their concatenation 64 edges at a time. Lists longer than kHeavyDeg are cut
-// into kChunk-edge descriptors for k_expand_heavy.
+// ---- per-wave output staging -------------------------------------------------
+// Activated vertices are collected in a per-wave LDS buffer and appended to
+// the global frontier kOutStage at a time: one atomicAdd per flush instead of
+// one per wave-iteration, and coalesced stores.
+constexpr uint32_t kOutStage = 512;
+constexpr uint32_t kWaveBudget = 1024;  // default light-tier edges per 64-item batch of a wave
+constexpr int kIlp = 4;                 // edges per lane per iteration
+
+struct OutStage {
+    uint32_t *buf;  // this wave's LDS slice [kOutStage]
+    uint32_t n;     // wave-uniform fill
+};
+
+__device__ __forceinline__ void stage_flush(OutStage &st, const PassIO &io) {
+    if (st.n == 0) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(io.out_cnt, st.n);
+    base = __shfl(base, 0, kWave);
+    for (uint32_t i = lane_id(); i < st.n; i += kWave) {
+        const uint32_t pos = base + i;
+        if (pos < io.out_cap) io.out_list[pos] = st.buf[i];
+        else atomicOr(io.stats + 3, 1ull);
+    }
+    st.n = 0;
+}
+
+__device__ __forceinline__ void stage_push(bool pred, uint32_t item, OutStage &st, const PassIO &io) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const uint32_t c = (uint32_t)__popcll(m);
+    if (st.n + c > kOutStage) stage_flush(st, io);
+    if (pred) st.buf[st.n + (uint32_t)__popcll(m & lanemask_lt())] = item;
+    st.n += c;
+}
+
+// Per-wave counters, reduced per block at the end of a kernel.
+struct WaveStats {
+    unsigned long long edges = 0, verts = 0, hedges = 0;
+};
+
+// LDS of the light tier (per block).
+struct LightShared {
+    uint64_t start[kWavesPerBlock][kWave];
+    uint32_t excl[kWavesPerBlock][kWave];
+    uint32_t src[kWavesPerBlock][kWave];
+    uint32_t push[kWavesPerBlock][kWave];
+    uint32_t out[kWavesPerBlock][kOutStage];
+    unsigned long long red[kWavesPerBlock][3];
+};
+
+// Owner of edge k among the 64 staged lists: the largest j with excl[j] <= k
+// (six unrolled LDS probes, so the kIlp searches of a lane interleave).
+__device__ __forceinline__ uint32_t staged_owner(const uint32_t *excl, uint32_t k) {
+    uint32_t lo = 0;
+#pragma unroll
+    for (uint32_t step = kWave / 2; step > 0; step >>= 1)
+        if (excl[lo + step] <= k) lo += step;
+    return lo;
+}
+
+// Edge loop of one wave over `total` edges whose owners are staged in LDS,
+// kIlp edges per lane per step: all target ids are loaded, then all functor
+// peeks, then the updates.
+template <class EFn>
+__device__ __forceinline__ void walk_staged(const GraphView &g, const PassIO &io, const EFn &efn, uint32_t total,
+                                            const uint32_t *s_excl, const uint64_t *s_start, const uint32_t *s_src,
+                                            const uint32_t *s_push, OutStage &st) {
+    const uint32_t lane = lane_id();
+    for (uint32_t k0 = 0; k0 < total; k0 += kIlp * kWave) {
+        uint32_t v[kIlp], src[kIlp], push[kIlp];
+        uint64_t peek[kIlp];
+        bool act[kIlp];
+#pragma unroll
+        for (int j = 0; j < kIlp; ++j) {
+            const uint32_t k = k0 + j * kWave + lane;
+            act[j] = k < total;
+            const uint32_t o = staged_owner(s_excl, act[j] ? k : 0);
+            src[j] = s_src[o];
+            push[j] = s_push[o];
+            v[j] = act[j] ? g.col[s_start[o] + (k - s_excl[o])] : kGap;
+        }
+#pragma unroll
+        for (int j = 0; j < kIlp; ++j) peek[j] = (v[j] != kGap) ? efn.load(v[j]) : 0;
+        bool enq[kIlp];
+#pragma unroll
+        for (int j = 0; j < kIlp; ++j) enq[j] = v[j] != kGap && efn.apply(src[j], v[j], push[j], peek[j]);
+#pragma unroll
+        for (int j = 0; j < kIlp; ++j) stage_push(enq[j], v[j], st, io);
+    }
+}
+
+// Light tier body: waves [gwave, +nwaves) take 64 work items at a time, run
+// the vertex functor, and walk the scheduled lists that fit the wave budget;
+// longer lists, and lists past the budget, are cut into kChunk-edge
+// descriptors for the heavy tier, so no wave carries more than
+// budget + heavy_deg edges per batch.
 template <class VFn, class EFn>
-__global__ void __launch_bounds__(kBlock) k_expand(GraphView g, PassIO io, VFn vfn, EFn efn) {
-    __shared__ uint64_t s_start[kWavesPerBlock][kWave];
-    __shared__ uint32_t s_excl[kWavesPerBlock][kWave];
-    __shared__ uint32_t s_src[kWavesPerBlock][kWave];
-    __shared__ uint32_t s_push[kWavesPerBlock][kWave];
-
-    if (blockIdx.x == 0 && threadIdx.x == 0 && io.zero_cnt) *io.zero_cnt = 0;
-    // a list never holds more than its capacity (appends past it are dropped and flagged)
-    const uint32_t n = io.in_list ? min(*io.in_cnt, io.in_cap) : io.range_n;
+__device__ __forceinline__ void light_body(const GraphView &g, const PassIO &io, const VFn &vfn, const EFn &efn,
+                                           uint32_t n, uint64_t gwave, uint64_t nwaves, LightShared &sh, OutStage &st,
+                                           WaveStats &ws) {
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
-    const uint64_t gwave = (uint64_t)blockIdx.x * kWavesPerBlock + w;
-    const uint64_t nwaves = (uint64_t)gridDim.x * kWavesPerBlock;
-    unsigned long long n_edges = 0, n_verts = 0;
-
     for (uint64_t base = gwave * kWave; base < n; base += nwaves * kWave) {
         const uint64_t idx = base + lane;
         uint32_t u = 0, d = 0, push = 0;
@@ -142,84 +232,116 @@ __global__ void __launch_bounds__(kBlock) k_expand(GraphView g, PassIO io, VFn v
             sched = vfn(u, du, push, self);
             if (sched && du) { d = du; start = g.off[u]; }
         }
-        wave_append(self, u, io.out_list, io.out_cnt, io.out_cap, io.stats + 3);
-        n_verts += sched ? 1 : 0;
-        if (d > kHeavyDeg) {
-            const uint32_t nch = (d + kChunk - 1) / kChunk;
-            const uint32_t hb = atomicAdd(io.heavy_cnt, nch);
-            if (hb + nch <= io.heavy_cap) {
-                for (uint32_t c = 0; c < nch; ++c) {
-                    HeavyDesc hd;
-                    hd.start = start + (uint64_t)c * kChunk;
-                    hd.len = (c + 1 < nch) ? kChunk : d - c * kChunk;
-                    hd.u = u;
-                    hd.push = push;
-                    hd.pad = 0;
-                    io.heavy[hb + c] = hd;
+        stage_push(self, u, st, io);
+        ws.verts += sched ? 1 : 0;
+        const uint32_t incl0 = wave_incl_scan(d);
+        const bool heavy = d > io.heavy_deg || (d > 0 && incl0 > io.budget);
+        const uint32_t nch = heavy ? (d + kChunk - 1) / kChunk : 0;
+        const uint32_t nch_incl = wave_incl_scan(nch);
+        const uint32_t nch_tot = __shfl(nch_incl, kWave - 1, kWave);
+        if (nch_tot) {
+            uint32_t hb = 0;
+            if (lane == 0) hb = atomicAdd(io.heavy_cnt, nch_tot);
+            hb = __shfl(hb, 0, kWave) + nch_incl - nch;
+            if (heavy) {
+                if (hb + nch <= io.heavy_cap) {
+                    for (uint32_t c = 0; c < nch; ++c) {
+                        const uint64_t len = (c + 1 < nch) ? kChunk : d - c * kChunk;
+                        HeavyDesc hd;
+                        hd.start_len = ((start + (uint64_t)c * kChunk) << kLenBits) | (len - 1);
+                        hd.u = u;
+                        hd.push = push;
+                        io.heavy[hb + c] = hd;
+                    }
+                } else {
+                    atomicOr(io.stats + 3, 2ull);
                 }
-            } else {
-                atomicOr(io.stats + 3, 2ull);
             }
-            d = 0;
         }
+        if (heavy) d = 0;
         const uint32_t incl = wave_incl_scan(d);
         const uint32_t total = __shfl(incl, kWave - 1, kWave);
-        s_start[w][lane] = start;
-        s_excl[w][lane] = incl - d;
-        s_src[w][lane] = u;
-        s_push[w][lane] = push;
+        sh.start[w][lane] = start;
+        sh.excl[w][lane] = incl - d;
+        sh.src[w][lane] = u;
+        sh.push[w][lane] = push;
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (uint32_t k0 = 0; k0 < total; k0 += kWave) {
-            const uint32_t k = k0 + lane;
-            bool enq = false;
-            uint32_t v = 0;
-            if (k < total) {
-                uint32_t lo = 0, hi = kWave;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (s_excl[w][mid] <= k) lo = mid; else hi = mid;
-                }
-                v = g.col[s_start[w][lo] + (k - s_excl[w][lo])];
-                enq = efn(s_src[w][lo], v, s_push[w][lo]);
-            }
-            wave_append(enq, v, io.out_list, io.out_cnt, io.out_cap, io.stats + 3);
-        }
-        n_edges += total;
+        walk_staged(g, io, efn, total, sh.excl[w], sh.start[w], sh.src[w], sh.push[w], st);
+        ws.edges += total;
         __builtin_amdgcn_wave_barrier();
-    }
-    n_verts = wave_sum_u64(n_verts);
-    if (lane == 0) {
-        if (n_edges) atomicAdd(io.stats + 0, n_edges);
-        if (n_verts) atomicAdd(io.stats + 1, n_verts);
     }
 }
 
-// ---- heavy tier: one wave per kChunk-edge descriptor -------------------------
+// Heavy tier body: one wave per kChunk-edge descriptor.
 template <class EFn>
-__global__ void __launch_bounds__(kBlock) k_expand_heavy(GraphView g, PassIO io, EFn efn) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && io.heavy_zero) *io.heavy_zero = 0;
-    const uint32_t nh = min(*io.heavy_cnt, io.heavy_cap);
-    const uint32_t w = threadIdx.x / kWave, lane = lane_id();
-    const uint64_t gwave = (uint64_t)blockIdx.x * kWavesPerBlock + w;
-    const uint64_t nwaves = (uint64_t)gridDim.x * kWavesPerBlock;
-    unsigned long long n_edges = 0;
+__device__ __forceinline__ void heavy_body(const GraphView &g, const PassIO &io, const EFn &efn, uint32_t nh,
+                                           uint64_t gwave, uint64_t nwaves, OutStage &st, WaveStats &ws) {
+    const uint32_t lane = lane_id();
     for (uint64_t i = gwave; i < nh; i += nwaves) {
-        const HeavyDesc hd = io.heavy[i];
-        for (uint32_t k0 = 0; k0 < hd.len; k0 += kWave) {
-            const uint32_t k = k0 + lane;
-            bool enq = false;
-            uint32_t v = 0;
-            if (k < hd.len) {
-                v = g.col[hd.start + k];
-                enq = efn(hd.u, v, hd.push);
+        const HeavyDesc hd = io.hin[i];
+        const uint64_t hstart = hd.start_len >> kLenBits;
+        const uint32_t hlen = (uint32_t)(hd.start_len & ((1u << kLenBits) - 1)) + 1;
+        for (uint32_t k0 = 0; k0 < hlen; k0 += kIlp * kWave) {
+            uint32_t v[kIlp];
+            uint64_t peek[kIlp];
+            bool enq[kIlp];
+#pragma unroll
+            for (int j = 0; j < kIlp; ++j) {
+                const uint32_t k = k0 + j * kWave + lane;
+                v[j] = k < hlen ? g.col[hstart + k] : kGap;
             }
-            wave_append(enq, v, io.out_list, io.out_cnt, io.out_cap, io.stats + 3);
+#pragma unroll
+            for (int j = 0; j < kIlp; ++j) peek[j] = v[j] != kGap ? efn.load(v[j]) : 0;
+#pragma unroll
+            for (int j = 0; j < kIlp; ++j) enq[j] = v[j] != kGap && efn.apply(hd.u, v[j], hd.push, peek[j]);
+#pragma unroll
+            for (int j = 0; j < kIlp; ++j) stage_push(enq[j], v[j], st, io);
         }
-        n_edges += hd.len;
+        ws.hedges += hlen;
     }
-    if (lane == 0 && n_edges) atomicAdd(io.stats + 2, n_edges);
+}
+
+// One atomic per block for the pass counters.
+__device__ __forceinline__ void block_stats(const WaveStats &ws, LightShared &sh, unsigned long long *stats) {
+    const uint32_t w = threadIdx.x / kWave;
+    const unsigned long long v = wave_sum_u64(ws.verts);
+    if (lane_id() == 0) {
+        sh.red[w][0] = ws.edges;
+        sh.red[w][1] = v;
+        sh.red[w][2] = ws.hedges;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kWavesPerBlock; ++i) t += sh.red[i][threadIdx.x];
+        const int slot = threadIdx.x == 2 ? 2 : (int)threadIdx.x;
+        if (t) atomicAdd(stats + slot, t);
+    }
+}
+
+// ---- one pass ---------------------------------------------------------------------
+// Light items of this pass, then the descriptors the previous pass spilled;
+// the edge updates of every operator are order-insensitive (min-relaxations,
+// residual pushes), so the spilled lists of round R-1 are walked in round R
+// and whatever they activate joins round R+1.
+template <class VFn, class EFn>
+__global__ void __launch_bounds__(kBlock) k_round(GraphView g, PassIO io, VFn vfn, EFn efn) {
+    __shared__ LightShared sh;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (io.zero_cnt) *io.zero_cnt = 0;
+        if (io.heavy_zero) *io.heavy_zero = 0;
+    }
+    // a list never holds more than its capacity (appends past it are dropped and flagged)
+    const uint32_t n = io.in_list ? min(*io.in_cnt, io.in_cap) : io.range_n;
+    const uint32_t nh = io.hin ? min(*io.hin_cnt, io.heavy_cap) : 0;
+    const uint32_t w = threadIdx.x / kWave;
+    const uint64_t gwave = (uint64_t)blockIdx.x * kWavesPerBlock + w, nwaves = (uint64_t)gridDim.x * kWavesPerBlock;
+    OutStage st{sh.out[w], 0};
+    WaveStats ws;
+    light_body(g, io, vfn, efn, n, gwave, nwaves, sh, st, ws);
+    heavy_body(g, io, efn, nh, gwave, nwaves, st, ws);
+    stage_flush(st, io);
+    block_stats(ws, sh, io.stats);
 }
 
 // ---- vertex functors ---------------------------------------------------------
@@ -290,7 +412,8 @@ struct VAll {
 };
 
 // ---- edge functors -------------------------------------------------------------
-// Signature: bool (uint32_t u, uint32_t v, uint32_t push_bits) -> append v.
+// load(v) peeks the destination state (issued for kIlp edges at once);
+// apply(u, v, push, peek) updates it and returns whether v joins the output.
 
 template <class Op>
 struct ERelax {
@@ -300,22 +423,46 @@ struct ERelax {
     uint32_t stamp_val;
     uint32_t wmode;
     uint64_t wseed;
-    __device__ bool operator()(uint32_t u, uint32_t v, uint32_t push) const {
-        if (v == kGap) return false;
+    __device__ ERelax with_round(uint32_t stamp_next) const {
+        ERelax e = *this;
+        e.stamp_val = stamp_next;
+        return e;
+    }
+    __device__ uint64_t load(uint32_t v) const {
+        if constexpr (Op::kTraversal) return ld_peek(&buffer[v]);
+        else return 0;
+    }
+    __device__ bool apply(uint32_t u, uint32_t v, uint32_t push, uint64_t peek) const {
         const uint32_t w = Op::kWeighted ? sg_weight(wmode, wseed, u, v) : 0u;
-        if (!op.AccumulateBuffer(u, v, w, &buffer[v], from_bits<typename Op::push_t>(push))) return false;
+        const auto p = from_bits<typename Op::push_t>(push);
+        if constexpr (Op::kTraversal) {
+            if (!op.MayImprove(u, w, p, peek)) return false;
+        }
+        if (!op.AccumulateBuffer(u, v, w, &buffer[v], p)) return false;
         return atomicExch(&stamp[v], stamp_val) != stamp_val;
     }
 };
 
-// Relax restricted to invalidated targets (boundary sweep).
+// Relax restricted to invalidated targets (boundary sweep). The probe reads a
+// coarse bitmap (one bit per 64 vertices, small enough to stay in the vector
+// L1) before the fine one, so most edges never touch L2 for the test.
 template <class Op>
 struct ERelaxInval {
     ERelax<Op> r;
     const uint32_t *inval;
-    __device__ bool operator()(uint32_t u, uint32_t v, uint32_t push) const {
-        if (v == kGap || !((inval[v >> 5] >> (v & 31)) & 1u)) return false;
-        return r(u, v, push);
+    const uint32_t *coarse;
+    __device__ ERelaxInval with_round(uint32_t stamp_next) const {
+        ERelaxInval e = *this;
+        e.r.stamp_val = stamp_next;
+        return e;
+    }
+    __device__ uint64_t load(uint32_t v) const {
+        if (!((coarse[v >> 11] >> ((v >> 6) & 31)) & 1u)) return 0;
+        return inval[v >> 5];
+    }
+    __device__ bool apply(uint32_t u, uint32_t v, uint32_t push, uint64_t peek) const {
+        if (!((peek >> (v & 31)) & 1u)) return false;
+        return r.apply(u, v, push, r.load(v));
     }
 };
 
@@ -327,13 +474,15 @@ struct EInvalidate {
     typename Op::value_t *value;
     typename Op::buffer_t *buffer;
     uint32_t *inval;
+    uint32_t *coarse;
     unsigned long long *count;
-    __device__ bool operator()(uint32_t u, uint32_t v, uint32_t) const {
-        if (v == kGap) return false;
-        const uint64_t b = ld_relaxed(&buffer[v]);
+    __device__ EInvalidate with_round(uint32_t) const { return *this; }
+    __device__ uint64_t load(uint32_t v) const { return ld_relaxed(&buffer[v]); }
+    __device__ bool apply(uint32_t u, uint32_t v, uint32_t, uint64_t b) const {
         if (Op::Parent(b) != u) return false;
         const uint32_t bit = 1u << (v & 31);
         if (atomicOr(&inval[v >> 5], bit) & bit) return false;
+        atomicOr(&coarse[v >> 11], 1u << ((v >> 6) & 31));
         typename Op::buffer_t bb = b;
         op.AccumulateBuffer_del(u, v, &bb, &value[v]);
         buffer[v] = bb;
@@ -341,6 +490,7 @@ struct EInvalidate {
         return true;
     }
 };
+
 
 }  // namespace sg
 

@@ -113,6 +113,10 @@ struct SSSP : TraversalBase {
         const uint32_t b = hi32(*p_buffer);
         return {b, b != kInf};
     }
+    // Cheap pre-test against a peeked buffer word: can (pushed + w, src) win?
+    __device__ bool MayImprove(uint32_t src, uint32_t weight, push_t pushed, uint64_t peek) const {
+        return pushed != kInf && pack(pushed + weight, src) < peek;
+    }
     // hybrid_sssp.cu:97-118: atomicMin(dst, src_dist + w) and parent = src
     __device__ bool AccumulateBuffer(uint32_t src, uint32_t, uint32_t weight, buffer_t *p_buffer, push_t pushed) const {
         if (pushed == kInf) return false;
@@ -147,6 +151,9 @@ struct BFS : TraversalBase {
         const uint32_t b = hi32(*p_buffer);
         return {b == kInf ? kInf : b + 1, b != kInf};
     }
+    __device__ bool MayImprove(uint32_t src, uint32_t, push_t pushed, uint64_t peek) const {
+        return pushed != kInf && pack(pushed, src) < peek;
+    }
     // hybrid_bfs.cu:102-109 (atomicMin of the pushed level), plus the parent word
     __device__ bool AccumulateBuffer(uint32_t src, uint32_t, uint32_t, buffer_t *p_buffer, push_t pushed) const {
         if (pushed == kInf) return false;
@@ -175,6 +182,9 @@ struct CC : TraversalBase {
     }
     __device__ Push<push_t> CombineValueBufferAmend(uint32_t, int, value_t *, buffer_t *p_buffer, uint32_t) const {
         return {hi32(*p_buffer), true};
+    }
+    __device__ bool MayImprove(uint32_t, uint32_t, push_t pushed, uint64_t peek) const {
+        return pushed < hi32(peek);
     }
     // hybrid_cc.cu:96-110: atomicMin of the label; the parent is recorded only
     // on a strict label decrease (a CAS on the packed word), so parent edges

@@ -37,7 +37,8 @@ class Config(C.Structure):
         ("pr_alpha", C.c_double), ("pr_epsilon", C.c_double), ("pma_density", C.c_double),
         ("segments", C.c_uint32), ("n_stream", C.c_uint32), ("block_size", C.c_uint32),
         ("max_rounds", C.c_uint32), ("rounds_per_sync", C.c_uint32), ("flags", C.c_uint32),
-        ("device_budget_gb", C.c_double), ("device", C.c_int32), ("reserved", C.c_uint32 * 7),
+        ("device_budget_gb", C.c_double), ("device", C.c_int32),
+        ("wave_budget", C.c_uint32), ("heavy_degree", C.c_uint32), ("reserved", C.c_uint32 * 5),
     ]
 
 

@@ -64,7 +64,9 @@ typedef struct sg_config {
     uint32_t flags;           /* SG_FLAG_*                                                */
     double device_budget_gb;  /* cap on engine device memory in GiB (0 = no cap)          */
     int32_t device;           /* HIP device ordinal                                       */
-    uint32_t reserved[7];
+    uint32_t wave_budget;     /* light-tier edges per 64-vertex batch of a wave (0 = 1024) */
+    uint32_t heavy_degree;    /* lists longer than this always become chunks (0 = 2048)  */
+    uint32_t reserved[5];
 } sg_config;
 
 /* Time the dominant relax kernel with HIP events recorded on its stream. */
