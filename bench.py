@@ -57,6 +57,40 @@ def cpu_baseline(row, col, seed, budget_s):
     }
 
 
+def run_batched(args, row, col, ins, dele, wseed, algos, plain_value):
+    """The same queries answered together on one shared graph (one PMA update,
+    one boundary sweep and one launch per round for all of them)."""
+    nsnap = args.warmup + args.steps
+    e = sgraph.Engine("multi", weight_mode="seeded", weight_seed=wseed)
+    t = time.perf_counter()
+    e.load_csr(row, col)
+    e.init_queries([0] * len(algos), algos)
+    log(f"[bench] batched {'+'.join(algos)}: load+static solve {time.perf_counter() - t:.2f}s")
+    dev = [(e.to_device(ins[s]), e.to_device(dele[s])) for s in range(nsnap)]
+    for s in range(args.warmup):
+        e.apply_device(dev[s][0], args.batch, dev[s][1], args.batch)
+    e.synchronize()
+    lat, num = [], 0
+    for s in range(args.warmup, nsnap):
+        t0 = time.perf_counter()
+        e.apply_device(dev[s][0], args.batch, dev[s][1], args.batch)
+        lat.append(time.perf_counter() - t0)
+        num += e.stats()["ne"] * len(algos)
+    value = num / sum(lat) / 1e9
+    for di, dd in dev:
+        e.device_free(di)
+        e.device_free(dd)
+    e.close()
+    lat_ms = np.array(lat) * 1e3
+    return {
+        "value": round(value, 4), "unit": "GTEPS", "vs_plain": round(value / plain_value, 3),
+        "latency_p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+        "latency_p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "what": f"{len(algos)} queries ({'+'.join(algos)} from source 0) on one shared graph: one PMA update, one "
+                f"sweep and one launch per round per snapshot (sg_init_queries)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +105,8 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="do not time the relax kernel with HIP events")
     ap.add_argument("--wave-budget", type=int, default=0)
     ap.add_argument("--heavy-degree", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched (shared-graph) measurement")
+    ap.add_argument("--batched-only", action="store_true", help="measure only the batched path (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU "
@@ -84,6 +120,11 @@ def main():
     ins, dele = sgraph.gen_updates(row, col, args.scale, nsnap, args.batch, args.batch, seed=args.seed + 1)
     log(f"[bench] graph nv={nv} ne={len(col)} snapshots={nsnap} batch={args.batch} gen {time.perf_counter() - t:.1f}s")
     wseed = args.seed + 2
+
+    if args.batched_only:
+        b = run_batched(args, row, col, ins, dele, wseed, algos, 1.0)
+        print(json.dumps({"batched": b}), flush=True)
+        return 0
 
     engines = {}
     dev_batches = {}
@@ -172,6 +213,8 @@ def main():
         "engine": {a: {k: stats[a][k] for k in ("rounds", "invalidated", "rebalances", "resizes", "edges_relaxed",
                                                "vertices_processed", "device_bytes")} for a in algos},
     }
+    if not args.no_batched:
+        out["batched"] = run_batched(args, row, col, ins, dele, wseed, algos, value)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(row, col, wseed, 30)

@@ -36,6 +36,7 @@
 #include "sg_kernels.h"
 #include "sg_operators.h"
 #include "sg_pma.h"
+#include "sg_multi.h"
 
 namespace sg {
 
@@ -555,8 +556,16 @@ public:
 
     // --- frontier -----------------------------------------------------------
     uint32_t *q_[2] = {nullptr, nullptr};
+    // Control block (device, one allocation, read back by one copy):
+    //   u32 [0..2] frontier counters, [3..5] spill counters, [7] always zero,
+    //   u32 [8, 8 + 2 * kRoundLog) round log, then u64 stats[16]:
+    //   [0] edges (light), [1] vertices, [2] edges (spilled chunks), [3] overflow
+    //   flags, [4] invalidated vertices, [8..11] the same four for timed rounds.
+    static constexpr size_t kCtlBlockBytes = (8 + 2 * kRoundLog) * 4 + 16 * 8;
     uint32_t *cnt_ = nullptr;    // [3] rotating round counters
-    uint32_t *hcnt_ = nullptr;   // [2] rotating heavy counters
+    uint32_t *hcnt_ = nullptr;   // [3] rotating spill counters
+    uint32_t *rlog_ = nullptr;
+    uint64_t timed_edges_ = 0, timed_verts_ = 0;  // host totals of the timed-round stats
     HeavyDesc *heavy_[3] = {nullptr, nullptr, nullptr};  // rotating spill queues
     size_t heavy_cap_ = 0;
     unsigned long long *stats_ = nullptr;  // [0..3] pass stats, [4] inval count
@@ -578,8 +587,11 @@ public:
     uint32_t *mark_ = nullptr;    // bitmap of touched sources
     BatchRanges br_{};             // per-vertex [begin, end) of its inserts / deletes in the sorted batch
     unsigned long long *dpre_ = nullptr;  // exclusive prefix of deg (edge-centric sweep)
-    uint32_t *tile_owner_ = nullptr;      // source of the first live edge of each sweep tile
-    size_t tile_owner_cap_ = 0;
+    unsigned long long *bsum_ = nullptr;  // per-block degree sums of the scan
+    size_t bsum_cap_ = 0;
+    uint32_t *tile_v_ = nullptr;          // merge-path sweep tiles: first source ...
+    unsigned long long *tile_e_ = nullptr;  // ... and first live edge of each tile
+    size_t tile_v_cap_ = 0, tile_e_cap_ = 0;
     uint32_t *touched_ = nullptr, *newdeg_ = nullptr, *tidx_ = nullptr;
     TInfo *tinfo_ = nullptr;
     size_t tinfo_cap_ = 0;
@@ -603,8 +615,9 @@ public:
     // --- pinned host scratch --------------------------------------------------
     void *pinned_ = nullptr;
     Ctl *h_ctl_ = nullptr;
-    uint32_t *h_cnt_ = nullptr;   // per-round counts [256]
+    uint32_t *h_blk_ = nullptr;   // host copy of the control block
     unsigned long long *h_u64_ = nullptr;  // [16]
+    const unsigned long long *h_stats() const { return (const unsigned long long *)(h_blk_ + 8 + 2 * kRoundLog); }
 
     // --- kernel timing --------------------------------------------------------
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
@@ -625,7 +638,7 @@ public:
         SG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         SG_HIP(hipHostMalloc(&pinned_, 65536, hipHostMallocDefault));
         h_ctl_ = (Ctl *)pinned_;
-        h_cnt_ = (uint32_t *)((char *)pinned_ + 4096);             // 256 rounds x 8 counters
+        h_blk_ = (uint32_t *)((char *)pinned_ + 4096);
         h_u64_ = (unsigned long long *)((char *)pinned_ + 16384);
     }
 
@@ -676,9 +689,11 @@ public:
         maximal_ = dalloc<uint8_t>(2 * nleaves_);
         q_[0] = dalloc<uint32_t>(nv);
         q_[1] = dalloc<uint32_t>(nv);
-        cnt_ = dalloc<uint32_t>(8);  // [0..2] frontier counters, [3..5] spill counters
+        // control block: counters, round log and stats, read back by one copy
+        cnt_ = (uint32_t *)dalloc<unsigned long long>(kCtlBlockBytes / 8);
         hcnt_ = cnt_ + 3;
-        stats_ = dalloc<unsigned long long>(8);
+        rlog_ = cnt_ + 8;
+        stats_ = (unsigned long long *)(cnt_ + 8 + 2 * kRoundLog);
         stamp_ = dalloc<uint32_t>(nv);
         inval_ = dalloc<uint32_t>(((size_t)nv + 31) / 32);
         coarse_ = dalloc<uint32_t>(((size_t)nv + 2047) / 2048);
@@ -717,8 +732,7 @@ public:
         fill<uint32_t>(br_.del_b, nv, 0u);
         fill<uint32_t>(br_.del_e, nv, 0u);
         fill<uint32_t>(stamp_, nv, 0);
-        fill<uint32_t>(cnt_, 8, 0);
-        fill<unsigned long long>(stats_, 8, 0ull);
+        fill<uint32_t>(cnt_, kCtlBlockBytes / 4, 0u);
         fill<uint8_t>((uint8_t *)ctl_, sizeof(Ctl), 0);
         sync();
         dfree(d_row);
@@ -749,21 +763,22 @@ public:
     GraphView gview() const { return GraphView{nv_, off_, deg_, col_}; }
     uint32_t budget() const { return cfg_.wave_budget ? cfg_.wave_budget : kWaveBudget; }
 
-    // dpre_[v] = sum of deg[0..v) for v in [0, nv], accumulated in 64 bits.
-    struct DegAt {
-        const uint32_t *d;
-        uint32_t nv;
-        __host__ __device__ unsigned long long operator()(uint32_t i) const { return i < nv ? d[i] : 0ull; }
-    };
-    void degree_prefix() {
-        hipcub::CountingInputIterator<uint32_t> cnt(0);
-        hipcub::TransformInputIterator<unsigned long long, DegAt, hipcub::CountingInputIterator<uint32_t>> in(cnt,
-                                                                                                            DegAt{deg_, nv_});
-        size_t b = 0;
-        SG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b, in, dpre_, (int)nv_ + 1, stream_));
-        ensure(cub_tmp_, cub_cap_, b + 256);
-        b = cub_cap_;
-        SG_HIP(hipcub::DeviceScan::ExclusiveSum(cub_tmp_, b, in, dpre_, (int)nv_ + 1, stream_));
+    // dpre_[v] = sum of deg[0..v) for v in [0, nv] (64-bit) and the merge-path
+    // tiles of the edge-centric sweep, in three kernels.
+    SweepTiles sweep_tiles() {
+        const uint32_t nb = (uint32_t)(((uint64_t)nv_ + kScanSpan - 1) / kScanSpan);
+        ensure(bsum_, bsum_cap_, (size_t)nb + 1);
+        ensure(tile_v_, tile_v_cap_, ((uint64_t)nv_ + ne_) / kSweepTile + 2);
+        ensure(tile_e_, tile_e_cap_, tile_v_cap_);
+        SweepTiles t{tile_v_, tile_e_, std::min<uint64_t>(tile_v_cap_, tile_e_cap_)};
+        hipLaunchKernelGGL(k_deg_block_sums, dim3(nb), dim3(kScanBlock), 0, stream_, nv_, deg_, bsum_);
+        SG_LAUNCHED();
+        hipLaunchKernelGGL(k_deg_scan_sums, dim3(1), dim3(1024), 0, stream_, nb, bsum_);
+        SG_LAUNCHED();
+        hipLaunchKernelGGL(k_deg_scan_write, dim3(nb), dim3(kScanBlock), 0, stream_, nv_, deg_, bsum_, dpre_, t,
+                           stats_ + 3);
+        SG_LAUNCHED();
+        return t;
     }
     uint32_t heavy_deg() const { return std::max<uint32_t>(kChunk, cfg_.heavy_degree ? cfg_.heavy_degree : kHeavyDeg); }
 
@@ -812,21 +827,28 @@ public:
         return io;
     }
 
+    // HIP events around one launch on the engine's stream.
+    void begin_timed(bool timed) {
+        if (!timed) return;
+        if (ev_used_ == events_.size()) {
+            hipEvent_t a, b;
+            SG_HIP(hipEventCreate(&a));
+            SG_HIP(hipEventCreate(&b));
+            events_.push_back({a, b});
+        }
+        SG_HIP(hipEventRecord(events_[ev_used_].first, stream_));
+    }
+    void end_timed(bool timed) {
+        if (timed) SG_HIP(hipEventRecord(events_[ev_used_++].second, stream_));
+    }
+
     template <class VFn, class EFn>
     void launch_pass(const PassIO &io, VFn vfn, EFn efn, bool timed) {
         const uint32_t g1 = io.in_list ? kGridList : grid_for(io.range_n, kWave * kWavesPerBlock, kGridList);
-        if (timed) {
-            if (ev_used_ == events_.size()) {
-                hipEvent_t a, b;
-                SG_HIP(hipEventCreate(&a));
-                SG_HIP(hipEventCreate(&b));
-                events_.push_back({a, b});
-            }
-            SG_HIP(hipEventRecord(events_[ev_used_].first, stream_));
-        }
+        begin_timed(timed);
         hipLaunchKernelGGL((k_round<VFn, EFn>), dim3(g1), dim3(kBlock), 0, stream_, gview(), io, vfn, efn);
         SG_LAUNCHED();
-        if (timed) SG_HIP(hipEventRecord(events_[ev_used_++].second, stream_));
+        end_timed(timed);
     }
 
     // Walk, with efn, the descriptors that a seed pass before round R spilled
@@ -856,57 +878,83 @@ public:
         ev_used_ = 0;
     }
 
-    void read_stats(unsigned long long out[8]) {
-        SG_HIP(hipMemcpyAsync(h_u64_, stats_, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
+    // Copy the control block to the host and wait.
+    void read_block() {
+        SG_HIP(hipMemcpyAsync(h_blk_, cnt_, kCtlBlockBytes, hipMemcpyDeviceToHost, stream_));
         sync();
-        std::memcpy(out, h_u64_, 8 * sizeof(unsigned long long));
     }
 
     // Run rounds from the frontier of round R_ until a round activates nothing
-    // and spills nothing; the host reads the counters every rounds_per_sync
-    // rounds (rounds past the last busy one find empty inputs and return at
-    // once). make_fns(stamp_for_next) returns {vfn, efn}. Returns the rounds run.
+    // and spills nothing. Rounds are enqueued in batches (rounds_per_sync, then
+    // doubling up to 64); each round logs what the previous one produced, so a
+    // batch costs one copy of the control block and one wait. Rounds past the
+    // last busy one find empty inputs and return at once. make_fns(stamp_for_next)
+    // returns {vfn, efn}. Timed runs count into stats [8..11]. Returns the rounds run.
     template <class MakeFns>
     uint32_t run_rounds(MakeFns make_fns, bool timed) {
-        const uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(cfg_.rounds_per_sync ? cfg_.rounds_per_sync : 4, 256));
+        const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
+        return drive_rounds([&](const PassIO &io) {
+            auto fns = make_fns(io.round + 1);
+            launch_pass(io, fns.first, fns.second, do_time);
+        }, timed);
+    }
+
+    // launch(io) enqueues round io.round (timed with begin_timed / end_timed
+    // when the run is timed and SG_FLAG_KERNEL_TIMING is set).
+    template <class Launch>
+    uint32_t drive_rounds(Launch launch, bool timed) {
+        uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(cfg_.rounds_per_sync ? cfg_.rounds_per_sync : 4, kRoundLog));
         const uint32_t maxr = cfg_.max_rounds ? cfg_.max_rounds : 1000000u;
         const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
-        unsigned long long s0[8];
-        if (do_time) read_stats(s0);
+        if (timed) fill<unsigned long long>(stats_ + 8, 4, 0ull);
         uint32_t rounds = 0;
         for (;;) {
+            const uint32_t R0 = R_;
             for (uint32_t i = 0; i < k; ++i) {
-                const uint32_t R = R_;
-                auto fns = make_fns(R + 1);
-                launch_pass(round_io(R), fns.first, fns.second, do_time);
-                // counters after round R: its frontier and its spill
-                SG_HIP(hipMemcpyAsync(h_cnt_ + 8 * i, cnt_, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+                PassIO io = round_io(R_);
+                io.log = rlog_;
+                io.round = R_;
+                if (timed) io.stats = stats_ + 8;
+                launch(io);
                 ++R_;
             }
-            sync();
+            read_block();
             if (do_time) harvest_events();
             for (uint32_t i = 0; i < k; ++i) {
                 ++rounds;
-                const uint32_t R = R_ - k + i;
-                if (h_cnt_[8 * i + ((R + 1) % 3)] == 0 && h_cnt_[8 * i + 3 + (R % 3)] == 0) {
-                    finish_rounds(do_time, s0);
+                const uint32_t R = R0 + i;
+                uint32_t front, spill;
+                if (i + 1 < k) {
+                    front = h_blk_[8 + 2 * (R % kRoundLog)];
+                    spill = h_blk_[8 + 2 * (R % kRoundLog) + 1];
+                } else {
+                    front = h_blk_[(R + 1) % 3];
+                    spill = h_blk_[3 + R % 3];
+                }
+                if (front == 0 && spill == 0) {
+                    finish_rounds(timed);
                     return rounds;
                 }
             }
             if (rounds >= maxr) {
-                finish_rounds(do_time, s0);
+                finish_rounds(timed);
                 fail(SG_EDEVICE, "no convergence within " + std::to_string(maxr) + " rounds");
             }
+            k = std::min<uint32_t>(2 * k, 64);
         }
     }
 
-    void finish_rounds(bool do_time, const unsigned long long s0[8]) {
-        unsigned long long s1[8];
-        read_stats(s1);
-        if (s1[3]) fail(SG_EDEVICE, "internal queue overflow (flags " + std::to_string(s1[3]) + ")");
-        if (do_time) st_.relax_bytes += 12ull * (s1[0] - s0[0]) + 32ull * (s1[1] - s0[1]);
-        st_.edges_relaxed = s1[0] + s1[2];
-        st_.vertices_processed = s1[1];
+    // Uses the control block read by the last batch of rounds.
+    void finish_rounds(bool timed) {
+        const unsigned long long *s = h_stats();
+        if (s[3] | s[11]) fail(SG_EDEVICE, "internal queue overflow (flags " + std::to_string(s[3] | s[11]) + ")");
+        if (timed) {
+            timed_edges_ += s[8] + s[10];
+            timed_verts_ += s[9];
+            if (cfg_.flags & SG_FLAG_KERNEL_TIMING) st_.relax_bytes += 12ull * (s[8] + s[10]) + 32ull * s[9];
+        }
+        st_.edges_relaxed = s[0] + s[2] + timed_edges_;
+        st_.vertices_processed = s[1] + timed_verts_;
     }
 
     // ---- batch prepare / commit -------------------------------------------------------
@@ -1120,6 +1168,12 @@ public:
     virtual void init(const uint32_t *sources, uint32_t nsrc) = 0;
     virtual void apply(const uint32_t *d_ins, uint64_t nins, const uint32_t *d_del, uint64_t ndel) = 0;
     virtual void gather(uint32_t q, void *out, size_t bytes) = 0;
+    virtual void init_queries(const uint32_t *, const uint32_t *, uint32_t) {
+        fail(SG_EINVAL, "sg_init_queries needs an engine created with SG_ALGO_MULTI");
+    }
+    virtual void gather_query(uint32_t, uint32_t, void *, size_t) {
+        fail(SG_EINVAL, "sg_gather_query needs an engine created with SG_ALGO_MULTI");
+    }
 
     void gather_graph(uint32_t q, void *out, size_t bytes) {
         if (q == SG_Q_DEGREE) {
@@ -1200,24 +1254,21 @@ public:
                 return std::make_pair(VAll{}, EInvalidate<Op>{op_, value_, buffer_, inval_, coarse_, stats_ + 4});
             }, false);
         }
-        unsigned long long s[8];
-        read_stats(s);
-        st_.invalidated = s[4];
+        // the invalidation rounds' last read of the control block holds the count
+        const uint64_t ninval = ndel_ ? h_stats()[4] : 0;
+        st_.invalidated = ninval;
         // 2. seed the recompute: edges into invalidated vertices + inserted edges
         frontier_reset();
         const uint32_t R = R_;
-        if (s[4]) {
+        if (ninval) {
             // edges into invalidated vertices: edge-centric sweep over all live edges
-            degree_prefix();
-            const uint64_t ntiles = (ne_ + kSweepTile - 1) / kSweepTile;
-            ensure(tile_owner_, tile_owner_cap_, ntiles + 1);
-            hipLaunchKernelGGL(k_tile_owners, dim3(grid_for(nv_, 256, 16384)), dim3(256), 0, stream_, nv_, dpre_, tile_owner_);
-            SG_LAUNCHED();
+            const SweepTiles tl = sweep_tiles();
             PassIO io = seed_io(R, nullptr, nullptr, nv_);
             ERelaxInval<Op> ef{ERelax<Op>{op_, buffer_, stamp_, R, cfg_.weight_mode, cfg_.weight_seed}, inval_, coarse_};
-            const uint32_t tiles = (uint32_t)std::min<uint64_t>(ntiles, 16384);
-            hipLaunchKernelGGL((k_sweep_edges<Op, ERelaxInval<Op>>), dim3(std::max<uint32_t>(tiles, 1)), dim3(kBlock), 0,
-                               stream_, gview(), dpre_, tile_owner_, op_, buffer_, ef, io);
+            const uint64_t ntiles = ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile;
+            const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, 16384));
+            hipLaunchKernelGGL((k_sweep_edges<Op, ERelaxInval<Op>>), dim3(grid), dim3(kBlock), 0, stream_, gview(), dpre_, tl,
+                               op_, buffer_, ef, io);
             SG_LAUNCHED();
             // invalidated vertices active on their own (CC: own label)
             hipLaunchKernelGGL(k_inval_activate<Op>, dim3(grid_for((nv_ + 31) / 32, 256, 4096)), dim3(256), 0, stream_, nv_,
@@ -1355,6 +1406,200 @@ public:
     }
 };
 
+// ---------------------------------------------------------------------------
+// Batched BFS/SSSP queries on one graph (no reference counterpart: the
+// reference answers one source per process). The PMA update, the boundary
+// sweep and every round are shared by all queries.
+// ---------------------------------------------------------------------------
+class MultiEngine final : public Engine {
+public:
+    using Engine::Engine;
+    uint32_t nq_ = 0, G_ = 0, KP_ = 0, QS_ = 0, qs_log_ = 0;
+    unsigned long long *best_ = nullptr, *act_ = nullptr, *minv_ = nullptr, *ninv_ = nullptr;
+    uint8_t *kind_ = nullptr;
+    uint32_t *src_ = nullptr;
+    MDesc *mspill_[3] = {nullptr, nullptr, nullptr};
+    size_t mspill_cap_ = 0;
+    unsigned long long *hits_ = nullptr;
+    size_t hits_cap_ = 0;
+    size_t state_nv_ = 0, state_kp_ = 0;
+
+    MultiState mstate() const {
+        MultiState m;
+        m.best = best_;
+        m.act = act_;
+        m.inv = minv_;
+        m.ninv = ninv_;
+        m.invbit = inval_;
+        m.coarse = coarse_;
+        m.kind = kind_;
+        m.G = G_;
+        m.KP = KP_;
+        m.QS = QS_;
+        m.qs_log = qs_log_;
+        m.wmode = cfg_.weight_mode;
+        m.wseed = cfg_.weight_seed;
+        return m;
+    }
+
+    void ensure_mspill() {
+        const size_t need = (size_t)((ne_ + nins_) / kMEdgeChunk + 1) * G_ + 4096;
+        if (need <= mspill_cap_) return;
+        for (auto &p : mspill_) { dfree(p); p = nullptr; }
+        for (auto &p : mspill_) p = dalloc<MDesc>(need);
+        mspill_cap_ = need;
+    }
+
+    template <int kMode>
+    void mround(const PassIO &io, bool timed) {
+        begin_timed(timed);
+        hipLaunchKernelGGL((k_mround<kMode>), dim3(kGridList), dim3(kBlock), 0, stream_, gview(), mstate(), io,
+                           mspill_[io.round % 3], (const MDesc *)mspill_[(io.round + 2) % 3],
+                           (uint32_t)std::min<size_t>(mspill_cap_, 0xFFFFFFFFu), stamp_);
+        SG_LAUNCHED();
+        end_timed(timed);
+    }
+
+    template <int kMode>
+    uint32_t mrounds(bool timed) {
+        const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
+        return drive_rounds([&](const PassIO &io) { mround<kMode>(io, do_time); }, timed);
+    }
+
+    template <int kMode>
+    void mpairs(uint64_t n, uint32_t R, const uint32_t *nhits) {
+        if (n == 0) return;
+        PassIO io = seed_io(R, nullptr, nullptr, 0);
+        const uint64_t per = kWave / QS_;
+        const uint32_t grid = grid_for((n + per - 1) / per, kWavesPerBlock, 8192);
+        hipLaunchKernelGGL((k_mpairs<kMode>), dim3(grid), dim3(kBlock), 0, stream_, mstate(), bv_, br_,
+                           (const unsigned long long *)hits_, nhits, (uint64_t)hits_cap_, io, stamp_, R);
+        SG_LAUNCHED();
+    }
+
+    void init(const uint32_t *sources, uint32_t nsrc) override {
+        std::vector<uint32_t> kinds(nsrc, cfg_.weight_mode == SG_WEIGHT_ONE ? (uint32_t)SG_ALGO_BFS : (uint32_t)SG_ALGO_SSSP);
+        init_queries(sources, kinds.data(), nsrc);
+    }
+
+    void init_queries(const uint32_t *sources, const uint32_t *kinds, uint32_t nq) override {
+        if (!loaded_) fail(SG_EINVAL, "sg_init before sg_load_csr");
+        if (nq == 0 || !sources || !kinds) fail(SG_EINVAL, "need at least one query");
+        if (nq > 4096) fail(SG_EINVAL, "at most 4096 queries per engine");
+        for (uint32_t i = 0; i < nq; ++i) {
+            if (sources[i] >= nv_) fail(SG_EINVAL, "query " + std::to_string(i) + ": source id >= nv");
+            if (kinds[i] != SG_ALGO_SSSP && kinds[i] != SG_ALGO_BFS)
+                fail(SG_EINVAL, "query " + std::to_string(i) + ": kind must be SG_ALGO_SSSP or SG_ALGO_BFS");
+        }
+        nq_ = nq;
+        QS_ = 1;
+        while (QS_ < std::min<uint32_t>(nq, 64)) QS_ <<= 1;
+        qs_log_ = 0;
+        while ((1u << qs_log_) < QS_) ++qs_log_;
+        G_ = (nq + QS_ - 1) / QS_;
+        KP_ = G_ * QS_;
+        if (state_nv_ != nv_ || state_kp_ != KP_) {
+            dfree(best_); dfree(act_); dfree(minv_); dfree(ninv_); dfree(kind_); dfree(src_);
+            best_ = act_ = minv_ = ninv_ = nullptr;
+            kind_ = nullptr;
+            src_ = nullptr;
+            best_ = dalloc<unsigned long long>((size_t)nv_ * KP_);
+            act_ = dalloc<unsigned long long>((size_t)nv_ * G_);
+            minv_ = dalloc<unsigned long long>((size_t)nv_ * G_);
+            ninv_ = dalloc<unsigned long long>((size_t)nv_ * G_);
+            kind_ = dalloc<uint8_t>(KP_);
+            src_ = dalloc<uint32_t>(KP_);
+            state_nv_ = nv_;
+            state_kp_ = KP_;
+        }
+        std::vector<uint8_t> hk(KP_, kQNone);
+        for (uint32_t i = 0; i < nq; ++i) hk[i] = kinds[i] == SG_ALGO_BFS ? kQUnit : kQWeighted;
+        auto t0 = std::chrono::steady_clock::now();
+        SG_HIP(hipMemcpyAsync(kind_, hk.data(), KP_, hipMemcpyHostToDevice, stream_));
+        SG_HIP(hipMemcpyAsync(src_, sources, (size_t)nq * 4, hipMemcpyHostToDevice, stream_));
+        fill<unsigned long long>(best_, (uint64_t)nv_ * KP_, ~0ull);
+        fill<unsigned long long>(act_, (uint64_t)nv_ * G_, 0ull);
+        fill<unsigned long long>(minv_, (uint64_t)nv_ * G_, 0ull);
+        fill<unsigned long long>(ninv_, (uint64_t)nv_ * G_, 0ull);
+        fill<uint32_t>(inval_, ((uint64_t)nv_ + 31) / 32, 0u);
+        fill<uint32_t>(coarse_, ((uint64_t)nv_ + 2047) / 2048, 0u);
+        nins_ = 0;
+        ensure_mspill();
+        frontier_reset();
+        hipLaunchKernelGGL(k_minit_sources, dim3(grid_for(nq, 256, 64)), dim3(256), 0, stream_, mstate(), src_, nq, stamp_,
+                           R_, q_[R_ & 1], cnt_ + (R_ % 3), nv_, stats_ + 3);
+        SG_LAUNCHED();
+        st_.rounds = mrounds<0>(false);
+        sync();  // host vectors above were read by the async copies
+        inited_ = true;
+        st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    void apply(const uint32_t *d_ins, uint64_t nins, const uint32_t *d_del, uint64_t ndel) override {
+        if (!inited_) fail(SG_EINVAL, "sg_apply before sg_init");
+        auto t0 = std::chrono::steady_clock::now();
+        const bool overflow = prepare(d_ins, nins, d_del, ndel);
+        commit(overflow);
+        ensure_mspill();
+        // 1. reset the slots whose tree edge was deleted, then their tree children (fixed point)
+        fill<unsigned long long>(stats_ + 4, 1, 0ull);
+        frontier_reset();
+        uint64_t ninval = 0;
+        if (ndel_) {
+            mpairs<1>(ndel_, R_, nullptr);
+            mrounds<1>(false);
+            ninval = h_stats()[4];
+        }
+        st_.invalidated = ninval;
+        // 2. seed the recompute: live edges into reset slots (one sweep for all queries) + inserted edges
+        frontier_reset();  // also zeroes the hit counter (word 6)
+        const uint32_t R = R_;
+        uint32_t *nhits = cnt_ + 6;
+        if (ninval) {
+            const SweepTiles tl = sweep_tiles();
+            ensure(hits_, hits_cap_, ne_ + 1);
+            const uint64_t ntiles = ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile;
+            const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, 16384));
+            hipLaunchKernelGGL(k_msweep, dim3(grid), dim3(kBlock), 0, stream_, gview(), dpre_, tl, (const uint32_t *)inval_,
+                               (const uint32_t *)coarse_, hits_, nhits, (uint32_t)std::min<size_t>(hits_cap_, 0xFFFFFFFFu),
+                               stats_ + 3);
+            SG_LAUNCHED();
+            // every live edge is at most one hit: the list length is bounded by ne
+            mpairs<2>(std::min<uint64_t>(ne_, hits_cap_), R, nhits);
+            hipLaunchKernelGGL(k_mclear_inv, dim3(grid_for((nv_ + 31) / 32, 256, 4096)), dim3(256), 0, stream_, nv_, G_,
+                               (const uint32_t *)inval_, minv_);
+            SG_LAUNCHED();
+            fill<uint32_t>(inval_, ((uint64_t)nv_ + 31) / 32, 0u);
+            fill<uint32_t>(coarse_, ((uint64_t)nv_ + 2047) / 2048, 0u);
+        }
+        mpairs<0>(nins_, R, nullptr);
+        // 3. converge
+        st_.rounds = mrounds<0>(true);
+        finish_batch();
+        st_.snapshots += 1;
+        st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    void gather_query(uint32_t qi, uint32_t what, void *out, size_t bytes) override {
+        if (!inited_) fail(SG_EINVAL, "no queries before sg_init_queries");
+        if (qi >= nq_) fail(SG_EINVAL, "query index out of range");
+        if (what != SG_Q_VALUE && what != SG_Q_PARENT) fail(SG_EINVAL, "sg_gather_query returns SG_Q_VALUE or SG_Q_PARENT");
+        if (bytes < (size_t)nv_ * 4) fail(SG_EINVAL, "output too small");
+        ensure(tmp_, tmp_cap_, nv_);
+        hipLaunchKernelGGL(k_mextract, dim3(grid_for(nv_, 256, 16384)), dim3(256), 0, stream_, nv_,
+                           (const unsigned long long *)best_, KP_, qi, tmp_, what == SG_Q_PARENT ? 1 : 0);
+        SG_LAUNCHED();
+        SG_HIP(hipMemcpyAsync(out, tmp_, (size_t)nv_ * 4, hipMemcpyDeviceToHost, stream_));
+        sync();
+    }
+
+    void gather(uint32_t q, void *out, size_t bytes) override {
+        if (q == SG_Q_VALUE || q == SG_Q_PARENT) return gather_query(0, q, out, bytes);
+        if (q == SG_Q_BUFFER) fail(SG_EINVAL, "batched queries keep no separate buffer");
+        gather_graph(q, out, bytes);
+    }
+};
+
 }  // namespace sg
 
 // ---------------------------------------------------------------------------
@@ -1414,6 +1659,7 @@ sg_status sg_create(const sg_config *cfg, sg_engine **out) {
             case SG_ALGO_SSSP: impl.reset(new sg::TraversalEngine<sg::api::SSSP>(*cfg)); break;
             case SG_ALGO_BFS: impl.reset(new sg::TraversalEngine<sg::api::BFS>(*cfg)); break;
             case SG_ALGO_CC: impl.reset(new sg::TraversalEngine<sg::api::CC>(*cfg)); break;
+            case SG_ALGO_MULTI: impl.reset(new sg::MultiEngine(*cfg)); break;
             default: sg::g_create_error = "unknown algo"; return SG_EINVAL;
         }
         if (cfg->weight_mode > SG_WEIGHT_ONE) { sg::g_create_error = "unknown weight mode"; return SG_EINVAL; }
@@ -1447,6 +1693,17 @@ sg_status sg_load_csr(sg_engine *eng, uint32_t nv, uint64_t ne, const uint64_t *
 
 sg_status sg_init(sg_engine *eng, const uint32_t *sources, uint32_t nsrc) {
     return guarded(eng, [&](sg::Engine &e) { e.init(sources, nsrc); });
+}
+
+sg_status sg_init_queries(sg_engine *eng, const uint32_t *sources, const uint32_t *kinds, uint32_t nq) {
+    return guarded(eng, [&](sg::Engine &e) { e.init_queries(sources, kinds, nq); });
+}
+
+sg_status sg_gather_query(sg_engine *eng, uint32_t qi, uint32_t what, void *out, size_t bytes) {
+    return guarded(eng, [&](sg::Engine &e) {
+        if (!out) sg::fail(SG_EINVAL, "null output");
+        e.gather_query(qi, what, out, bytes);
+    });
 }
 
 sg_status sg_apply_snapshot(sg_engine *eng, const uint32_t *ins_uv, uint64_t nins, const uint32_t *del_uv, uint64_t ndel) {
@@ -1490,8 +1747,9 @@ sg_status sg_reset_stats(sg_engine *eng) {
         e.st_.relax_launches = 0;
         e.st_.relax_ms = 0;
         e.st_.relax_bytes = 0;
+        e.timed_edges_ = e.timed_verts_ = 0;
         if (e.stats_) {
-            e.fill<unsigned long long>(e.stats_, 8, 0ull);
+            e.fill<unsigned long long>(e.stats_, 16, 0ull);
             e.sync();
         }
     });

@@ -66,7 +66,10 @@ struct PassIO {
     uint32_t budget;          // light-tier edges per 64-item batch of a wave
     uint32_t heavy_deg;       // lists longer than this always go to the heavy tier
     unsigned long long *stats;  // [0] edges (light), [1] vertices, [2] edges (heavy), [3] overflow
+    uint32_t *log;            // round log (or null): round R records the counts its inputs had
+    uint32_t round;           // R, for the log
 };
+constexpr uint32_t kRoundLog = 256;  // entries of the round log ring
 
 // ---- wave64 primitives -----------------------------------------------------
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -129,8 +132,9 @@ constexpr uint32_t kWaveBudget = 1024;  // default light-tier edges per 64-item 
 constexpr int kIlp = 4;                 // edges per lane per iteration
 
 struct OutStage {
-    uint32_t *buf;  // this wave's LDS slice [kOutStage]
-    uint32_t n;     // wave-uniform fill
+    uint32_t *buf;              // this wave's LDS slice [cap]
+    uint32_t n;                 // wave-uniform fill
+    uint32_t cap = kOutStage;   // slice size (>= kWave)
 };
 
 __device__ __forceinline__ void stage_flush(OutStage &st, const PassIO &io) {
@@ -150,7 +154,7 @@ __device__ __forceinline__ void stage_push(bool pred, uint32_t item, OutStage &s
     const uint64_t m = __ballot(pred);
     if (m == 0) return;
     const uint32_t c = (uint32_t)__popcll(m);
-    if (st.n + c > kOutStage) stage_flush(st, io);
+    if (st.n + c > st.cap) stage_flush(st, io);
     if (pred) st.buf[st.n + (uint32_t)__popcll(m & lanemask_lt())] = item;
     st.n += c;
 }
@@ -330,6 +334,11 @@ __global__ void __launch_bounds__(kBlock) k_round(GraphView g, PassIO io, VFn vf
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (io.zero_cnt) *io.zero_cnt = 0;
         if (io.heavy_zero) *io.heavy_zero = 0;
+        if (io.log) {  // the inputs of round R are what round R - 1 produced
+            const uint32_t i = 2 * ((io.round + kRoundLog - 1) % kRoundLog);
+            io.log[i] = *io.in_cnt;
+            io.log[i + 1] = io.hin_cnt ? *io.hin_cnt : 0u;
+        }
     }
     // a list never holds more than its capacity (appends past it are dropped and flagged)
     const uint32_t n = io.in_list ? min(*io.in_cnt, io.in_cap) : io.range_n;

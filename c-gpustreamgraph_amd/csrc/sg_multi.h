@@ -1,0 +1,574 @@
+// Batched BFS/SSSP queries over one shared PMA graph: many sources answered by
+// one launch per round, one PMA update per batch and one boundary sweep for
+// all queries. No reference counterpart (the reference runs one query per
+// process); each query follows the single-source semantics of the SSSP/BFS
+// operators (hybrid_sssp.cu:28-194, hybrid_bfs.cu:26-167): a BFS query is an
+// SSSP with unit weights, and the pair (distance, parent) is kept in one
+// 64-bit word updated by a lexicographic atomicMin, so a query gives the same
+// distances and parents alone and in a batch.
+//
+// Layout: queries are cut into G groups of QS slots (QS = 64, or the next
+// power of two >= the query count when there are fewer than 64); a vertex
+// holds G * QS words of state and one 64-bit mask per group for each of
+// act / inv / ninv. A wave step covers 64 / QS items (edges, pairs) times QS
+// queries: lane = item * QS + slot.
+#ifndef SG_MULTI_H
+#define SG_MULTI_H
+
+#include "sg_operators.h"
+#include "sg_pma.h"
+
+namespace sg {
+
+using api::hi32;
+using api::kInf;
+using api::kNone;
+using api::lo32;
+using api::pack;
+
+constexpr uint32_t kMEdgeChunk = 512;  // edges a wave walks for one (vertex, group) item; the rest is spilled
+constexpr int kMUnroll = 8;            // wave steps whose loads are in flight together
+
+enum : uint8_t { kQWeighted = 0, kQUnit = 1, kQNone = 2 };
+
+struct MultiState {
+    unsigned long long *best;  // [nv][KP]: distance << 32 | parent; ~0 = unreached
+    unsigned long long *act;   // [nv][G]: slots whose distance dropped (to push)
+    unsigned long long *inv;   // [nv][G]: slots reset by this batch's deletions
+    unsigned long long *ninv;  // [nv][G]: slots reset and not yet propagated to tree children
+    uint32_t *invbit;          // [nv / 32]: v has a reset slot in some group
+    uint32_t *coarse;          // [nv / 2048]: one bit per 64 vertices of invbit
+    const uint8_t *kind;       // [KP]: kQWeighted, kQUnit or kQNone (padding)
+    uint32_t G, KP, QS, qs_log;
+    uint32_t wmode;
+    uint64_t wseed;
+};
+
+// Spilled piece of an adjacency list: walk [start, start + len) of u for the
+// slots in mask of group grp.
+struct MDesc {
+    uint64_t start;
+    uint64_t mask;
+    uint32_t u, len, grp, pad;
+};
+
+struct QLane {
+    uint32_t sub;    // item of the wave step
+    uint32_t slot;   // query slot in the group
+    uint32_t per;    // items per wave step (64 / QS)
+    uint64_t smask;  // the QS bits of this lane's item in a ballot
+};
+__device__ __forceinline__ QLane qlane(const MultiState &ms) {
+    QLane q;
+    const uint32_t l = lane_id();
+    q.sub = l >> ms.qs_log;
+    q.slot = l & (ms.QS - 1);
+    q.per = kWave >> ms.qs_log;
+    const uint64_t ones = ms.QS == 64 ? ~0ull : ((1ull << ms.QS) - 1);
+    q.smask = ones << (q.sub * ms.QS);
+    return q;
+}
+
+__device__ __forceinline__ uint32_t qweight(const MultiState &ms, uint8_t kind, uint32_t u, uint32_t v) {
+    return kind == kQUnit ? 1u : sg_weight(ms.wmode, ms.wseed, u, v);
+}
+
+// Relax slot (grp, slot) of v from u at distance du; true when v's distance
+// dropped.
+__device__ __forceinline__ bool mrelax(const MultiState &ms, uint32_t u, uint32_t v, uint32_t grp, uint32_t slot,
+                                       uint32_t du, uint8_t kind) {
+    const uint32_t cand = du + qweight(ms, kind, u, v);
+    unsigned long long *p = &ms.best[(uint64_t)v * ms.KP + grp * ms.QS + slot];
+    const unsigned long long nw = pack(cand, u);
+    if (nw >= ld_peek(p)) return false;
+    const unsigned long long old = atomicMin(p, nw);
+    return cand < hi32(old);
+}
+
+// After a wave step: for every item whose ballot bits are set, OR them into
+// the item's mask word and put its vertex on the frontier once per round.
+__device__ __forceinline__ void mark_items(const MultiState &ms, const PassIO &io, OutStage &st, const QLane &q,
+                                           unsigned long long *masks, uint32_t grp, uint32_t v, bool hit, uint32_t *stamp,
+                                           uint32_t stamp_val) {
+    const uint64_t b = __ballot(hit) & q.smask;
+    bool app = false;
+    if (q.slot == 0 && b) {
+        atomicOr(&masks[(uint64_t)v * ms.G + grp], b >> (q.sub * ms.QS));
+        app = atomicExch(&stamp[v], stamp_val) != stamp_val;
+    }
+    stage_push(app, v, st, io);
+}
+
+// Reset the slots of v whose parent is the walking source (child = true in
+// the slot's lane): claim them in inv, queue them in ninv, flag v in the
+// vertex bitmaps, reset the state words and put v on the frontier.
+__device__ __forceinline__ void reset_children(const MultiState &ms, const PassIO &io, OutStage &st, const QLane &q,
+                                               uint32_t grp, uint32_t v, bool child, uint32_t *stamp, uint32_t stamp_val) {
+    const uint64_t b = __ballot(child) & q.smask;
+    uint64_t fresh = 0;
+    if (q.slot == 0 && b) {
+        const uint64_t m = b >> (q.sub * ms.QS);
+        fresh = m & ~atomicOr(&ms.inv[(uint64_t)v * ms.G + grp], m);
+    }
+    fresh = __shfl(fresh, (int)(q.sub * ms.QS), kWave);
+    bool app = false;
+    if (q.slot == 0 && fresh) {
+        atomicAdd(io.stats + 4, (unsigned long long)__popcll(fresh));
+        atomicOr(&ms.ninv[(uint64_t)v * ms.G + grp], fresh);
+        atomicOr(&ms.invbit[v >> 5], 1u << (v & 31));
+        atomicOr(&ms.coarse[v >> 11], 1u << ((v >> 6) & 31));
+        app = atomicExch(&stamp[v], stamp_val) != stamp_val;
+    }
+    if ((fresh >> q.slot) & 1u) ms.best[(uint64_t)v * ms.KP + grp * ms.QS + q.slot] = ~0ull;
+    stage_push(app, v, st, io);
+}
+
+// Walk len edges of u from slot position start for the slots in mask.
+// kMode 0: relax (activations go to act); kMode 1: reset tree children whose
+// parent is u (resets go to inv / ninv and the vertex bitmaps).
+template <int kMode>
+__device__ void mwalk(const GraphView &g, const MultiState &ms, const PassIO &io, OutStage &st, uint32_t *stamp,
+                      uint32_t stamp_val, uint32_t u, uint32_t grp, uint64_t mask, uint64_t start, uint32_t len,
+                      unsigned long long &edges) {
+    const QLane q = qlane(ms);
+    const bool on = (mask >> q.slot) & 1u;
+    const uint32_t qi = grp * ms.QS + q.slot;
+    const uint8_t kind = ms.kind[qi];
+    uint32_t du = kInf;
+    if (kMode == 0 && on) du = hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + qi]));
+    const bool live = on && kind != kQNone && (kMode == 1 || du != kInf);
+    if (!__any(live)) return;
+    edges += len;
+    for (uint32_t e0 = 0; e0 < len; e0 += q.per * kMUnroll) {
+        uint32_t vv[kMUnroll];
+#pragma unroll
+        for (int j = 0; j < kMUnroll; ++j) {
+            const uint32_t e = e0 + j * q.per + q.sub;
+            vv[j] = e < len ? g.col[start + e] : kGap;
+        }
+        if (kMode == 0) {
+            // all peeks in flight before the first atomic
+            uint64_t pk[kMUnroll];
+            uint32_t cand[kMUnroll];
+#pragma unroll
+            for (int j = 0; j < kMUnroll; ++j) {
+                const bool ok = live && vv[j] != kGap;
+                cand[j] = ok ? du + qweight(ms, kind, u, vv[j]) : kInf;
+                pk[j] = ok ? ld_peek(&ms.best[(uint64_t)vv[j] * ms.KP + qi]) : 0ull;
+            }
+#pragma unroll
+            for (int j = 0; j < kMUnroll; ++j) {
+                bool hit = false;
+                const unsigned long long nw = pack(cand[j], u);
+                if (cand[j] != kInf && nw < pk[j]) {
+                    const unsigned long long old = atomicMin(&ms.best[(uint64_t)vv[j] * ms.KP + qi], nw);
+                    hit = cand[j] < hi32(old);
+                }
+                mark_items(ms, io, st, q, ms.act, grp, vv[j], hit, stamp, stamp_val);
+            }
+        } else {
+            uint64_t peek[kMUnroll];
+#pragma unroll
+            for (int j = 0; j < kMUnroll; ++j)
+                peek[j] = (live && vv[j] != kGap) ? ld_relaxed(&ms.best[(uint64_t)vv[j] * ms.KP + qi]) : 0ull;
+#pragma unroll
+            for (int j = 0; j < kMUnroll; ++j) {
+                const uint32_t v = vv[j];
+                const bool child = live && v != kGap && lo32(peek[j]) == u && hi32(peek[j]) != kInf;
+                reset_children(ms, io, st, q, grp, v, child, stamp, stamp_val);
+            }
+        }
+    }
+}
+
+// LDS of the light path (per wave): 64 items of a wave batch.
+constexpr uint32_t kMLightQS = 16;  // light path for one group of at most this many slots
+struct MLight {
+    uint32_t excl[kWave];
+    uint32_t u[kWave];
+    uint64_t start[kWave];
+    uint32_t mask[kWave];
+    uint32_t du[kWave * kMLightQS];
+};
+
+// Light path (one group, QS <= kMLightQS): a wave takes 64 frontier vertices
+// at once, stages their lists (at most kMEdgeChunk edges each, the rest is
+// spilled) and walks the concatenation, 64 / QS edges per wave step.
+template <int kMode>
+__device__ void mlight(const GraphView &g, const MultiState &ms, const PassIO &io, OutStage &st, MLight &sh,
+                       MDesc *spill, uint32_t spill_cap, uint32_t *stamp, uint32_t n, uint64_t gw, uint64_t nw,
+                       unsigned long long &edges, unsigned long long &verts) {
+    unsigned long long *masks = kMode == 0 ? ms.act : ms.ninv;
+    const QLane q = qlane(ms);
+    const uint32_t lane = lane_id();
+    const uint32_t stamp_val = io.round + 1;
+    const uint8_t kind = ms.kind[q.slot];
+    for (uint64_t b = gw * kWave; b < n; b += nw * kWave) {
+        const uint64_t idx = b + lane;
+        uint32_t u = 0, m = 0, len = 0, deg = 0;
+        uint64_t start = 0;
+        if (idx < n) {
+            u = io.in_list[idx];
+            m = (uint32_t)atomicExch(&masks[u], 0ull);
+            if (m) {
+                deg = g.deg[u];
+                start = g.off[u];
+                len = min(deg, kMEdgeChunk);
+            }
+        }
+        verts += (uint32_t)__popcll(__ballot(m != 0));
+        // spill the tails of long lists
+        const uint32_t nch = deg > len ? (deg - len + kMEdgeChunk - 1) / kMEdgeChunk : 0u;
+        const uint32_t incl = wave_incl_scan(nch);
+        const uint32_t tot = __shfl(incl, kWave - 1, kWave);
+        if (tot) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(io.heavy_cnt, tot);
+            base = __shfl(base, 0, kWave) + incl - nch;
+            for (uint32_t c = 0; c < nch; ++c) {
+                const uint32_t lo = len + c * kMEdgeChunk;
+                if (base + c < spill_cap) spill[base + c] = MDesc{start + lo, m, u, min(kMEdgeChunk, deg - lo), 0, 0};
+                else atomicOr(io.stats + 3, 2ull);
+            }
+        }
+        const uint32_t lincl = wave_incl_scan(len);
+        const uint32_t total = __shfl(lincl, kWave - 1, kWave);
+        sh.excl[lane] = lincl - len;
+        sh.u[lane] = u;
+        sh.start[lane] = start;
+        sh.mask[lane] = m;
+        if (kMode == 0) {
+            for (uint32_t s2 = 0; s2 < ms.QS; ++s2)
+                sh.du[lane * ms.QS + s2] = ((m >> s2) & 1u) ? hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + s2])) : kInf;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (total == 0) continue;
+        edges += total;
+        for (uint32_t k0 = 0; k0 < total; k0 += q.per * kMUnroll) {
+            uint32_t vv[kMUnroll], src[kMUnroll], dv[kMUnroll];
+            bool on[kMUnroll];
+#pragma unroll
+            for (int j = 0; j < kMUnroll; ++j) {
+                const uint32_t k = k0 + j * q.per + q.sub;
+                const bool ok = k < total;
+                const uint32_t o = staged_owner(sh.excl, ok ? k : 0u);
+                src[j] = sh.u[o];
+                on[j] = ok && ((sh.mask[o] >> q.slot) & 1u) && kind != kQNone;
+                dv[j] = kMode == 0 ? sh.du[o * ms.QS + q.slot] : 0u;
+                vv[j] = ok ? g.col[sh.start[o] + (k - sh.excl[o])] : kGap;
+            }
+            if (kMode == 0) {
+                uint64_t pk[kMUnroll];
+                uint32_t cand[kMUnroll];
+#pragma unroll
+                for (int j = 0; j < kMUnroll; ++j) {
+                    const bool ok = on[j] && vv[j] != kGap && dv[j] != kInf;
+                    cand[j] = ok ? dv[j] + qweight(ms, kind, src[j], vv[j]) : kInf;
+                    pk[j] = ok ? ld_peek(&ms.best[(uint64_t)vv[j] * ms.KP + q.slot]) : 0ull;
+                }
+#pragma unroll
+                for (int j = 0; j < kMUnroll; ++j) {
+                    bool hit = false;
+                    const unsigned long long nwv = pack(cand[j], src[j]);
+                    if (cand[j] != kInf && nwv < pk[j]) {
+                        const unsigned long long old = atomicMin(&ms.best[(uint64_t)vv[j] * ms.KP + q.slot], nwv);
+                        hit = cand[j] < hi32(old);
+                    }
+                    mark_items(ms, io, st, q, ms.act, 0, vv[j], hit, stamp, stamp_val);
+                }
+            } else {
+                uint64_t pk[kMUnroll];
+#pragma unroll
+                for (int j = 0; j < kMUnroll; ++j)
+                    pk[j] = (on[j] && vv[j] != kGap) ? ld_relaxed(&ms.best[(uint64_t)vv[j] * ms.KP + q.slot]) : 0ull;
+#pragma unroll
+                for (int j = 0; j < kMUnroll; ++j) {
+                    const uint32_t v = vv[j];
+                    const bool child = on[j] && v != kGap && lo32(pk[j]) == src[j] && hi32(pk[j]) != kInf;
+                    reset_children(ms, io, st, q, 0, v, child, stamp, stamp_val);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// One convergence (kMode 0) or invalidation (kMode 1) round: items are
+// (frontier vertex, group); the mask of the item is taken (and cleared) from
+// act or ninv; lists longer than kMEdgeChunk leave their tail to the next
+// round as descriptors; then the descriptors the previous round spilled.
+template <int kMode>
+__global__ void __launch_bounds__(kBlock) k_mround(GraphView g, MultiState ms, PassIO io, MDesc *spill, const MDesc *sin,
+                                                   uint32_t spill_cap, uint32_t *stamp) {
+    __shared__ uint32_t s_out[kWavesPerBlock][kOutStage];
+    __shared__ unsigned long long s_red[kWavesPerBlock];
+    __shared__ MLight s_light[kWavesPerBlock];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (io.zero_cnt) *io.zero_cnt = 0;
+        if (io.heavy_zero) *io.heavy_zero = 0;
+        if (io.log) {
+            const uint32_t i = 2 * ((io.round + kRoundLog - 1) % kRoundLog);
+            io.log[i] = *io.in_cnt;
+            io.log[i + 1] = io.hin_cnt ? *io.hin_cnt : 0u;
+        }
+    }
+    unsigned long long *masks = kMode == 0 ? ms.act : ms.ninv;
+    const uint32_t n = min(*io.in_cnt, io.in_cap);
+    const uint32_t nh = sin ? min(*io.hin_cnt, spill_cap) : 0u;
+    const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+    const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerBlock + w, nw = (uint64_t)gridDim.x * kWavesPerBlock;
+    OutStage st{s_out[w], 0};
+    unsigned long long edges = 0, verts = 0;
+    const bool light = ms.G == 1 && ms.QS <= kMLightQS;
+    if (light) mlight<kMode>(g, ms, io, st, s_light[w], spill, spill_cap, stamp, n, gw, nw, edges, verts);
+    const uint64_t items = light ? 0 : (uint64_t)n * ms.G;
+    for (uint64_t it = gw; it < items; it += nw) {
+        const uint32_t u = io.in_list[it / ms.G], grp = (uint32_t)(it % ms.G);
+        unsigned long long m = 0;
+        if (lane == 0) m = atomicExch(&masks[(uint64_t)u * ms.G + grp], 0ull);
+        m = __shfl(m, 0, kWave);
+        if (m == 0) continue;
+        ++verts;
+        const uint64_t start = g.off[u];
+        const uint32_t deg = g.deg[u];
+        const uint32_t first = min(deg, kMEdgeChunk);
+        if (deg > first) {
+            const uint32_t nch = (deg - first + kMEdgeChunk - 1) / kMEdgeChunk;
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(io.heavy_cnt, nch);
+            base = __shfl(base, 0, kWave);
+            for (uint32_t c = lane; c < nch; c += kWave) {
+                const uint32_t lo = first + c * kMEdgeChunk;
+                if (base + c < spill_cap) spill[base + c] = MDesc{start + lo, m, u, min(kMEdgeChunk, deg - lo), grp, 0};
+                else atomicOr(io.stats + 3, 2ull);
+            }
+        }
+        mwalk<kMode>(g, ms, io, st, stamp, io.round + 1, u, grp, m, start, first, edges);
+    }
+    for (uint64_t h = gw; h < nh; h += nw) {
+        const MDesc d = sin[h];
+        mwalk<kMode>(g, ms, io, st, stamp, io.round + 1, d.u, d.grp, d.mask, d.start, d.len, edges);
+    }
+    stage_flush(st, io);
+    if (lane == 0) s_red[w] = edges;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0;
+        for (int i = 0; i < kWavesPerBlock; ++i) a += s_red[i];
+        if (a) atomicAdd(io.stats + 0, a);
+    }
+    if (lane == 0 && verts) atomicAdd(io.stats + 1, verts);
+}
+
+// Pairs (u, v) processed per wave step (one per item, QS slots each):
+// kMode 0 relaxes all slots of inserted edges that survive the batch;
+// kMode 1 resets the slots of v whose parent is u for deleted edges;
+// kMode 2 relaxes the reset slots of v (sweep hits, pairs packed u << 32 | v).
+template <int kMode>
+__global__ void __launch_bounds__(kBlock) k_mpairs(MultiState ms, BatchView bv, BatchRanges br,
+                                                   const unsigned long long *hits, const uint32_t *nhits, uint64_t hit_cap,
+                                                   PassIO io, uint32_t *stamp, uint32_t stamp_val) {
+    __shared__ uint32_t s_out[kWavesPerBlock][kOutStage];
+    const uint32_t w = threadIdx.x / kWave;
+    const QLane q = qlane(ms);
+    OutStage st{s_out[w], 0};
+    const uint64_t n = kMode == 0 ? bv.nins : kMode == 1 ? bv.ndel : min((uint64_t)*nhits, hit_cap);
+    const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerBlock + w, nw = (uint64_t)gridDim.x * kWavesPerBlock;
+    for (uint64_t b0 = gw * q.per; b0 < n; b0 += nw * q.per) {
+        const uint64_t i = b0 + q.sub;
+        bool ok = i < n;
+        uint32_t u = 0, v = 0;
+        if (ok) {
+            if (kMode == 0) {
+                u = bv.ins_key[i];
+                v = bv.ins_val[i];
+                ok = u != kGap;
+                if (ok) {
+                    const uint32_t db = br.del_b[u], de = br.del_e[u];
+                    ok = db == de || !in_sorted_u64(bv.del_key, db, de, ((uint64_t)u << 32) | v);
+                }
+            } else if (kMode == 1) {
+                const uint64_t k = bv.del_key[i];
+                ok = k != ~0ull;
+                u = (uint32_t)(k >> 32);
+                v = (uint32_t)k;
+            } else {
+                const unsigned long long h = hits[i];
+                u = (uint32_t)(h >> 32);
+                v = (uint32_t)h;
+            }
+        }
+        for (uint32_t grp = 0; grp < ms.G; ++grp) {
+            const uint32_t qi = grp * ms.QS + q.slot;
+            const uint8_t kind = ms.kind[qi];
+            const bool on = ok && kind != kQNone;
+            if (kMode == 1) {
+                const uint64_t cur = on ? ld_relaxed(&ms.best[(uint64_t)v * ms.KP + qi]) : 0ull;
+                const bool child = on && lo32(cur) == u && hi32(cur) != kInf;
+                reset_children(ms, io, st, q, grp, v, child, stamp, stamp_val);
+            } else {
+                bool want = on;
+                if (kMode == 2 && on) want = (ms.inv[(uint64_t)v * ms.G + grp] >> q.slot) & 1u;
+                const uint32_t du = want ? hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + qi])) : kInf;
+                const bool hit = want && du != kInf && mrelax(ms, u, v, grp, q.slot, du, kind);
+                mark_items(ms, io, st, q, ms.act, grp, v, hit, stamp, stamp_val);
+            }
+        }
+    }
+    stage_flush(st, io);
+}
+
+// Edge-centric sweep for the batch: every live edge (u, v) whose target has a
+// reset slot becomes a hit (u << 32 | v), appended in wave-sized groups.
+// Same merge-path tiles as k_sweep_edges.
+__global__ void __launch_bounds__(kBlock) k_msweep(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
+                                                   const uint32_t *invbit, const uint32_t *coarse,
+                                                   unsigned long long *hits, uint32_t *nhits, uint32_t hit_cap,
+                                                   unsigned long long *overflow) {
+    __shared__ uint16_t s_own[kSweepTile];
+    __shared__ int32_t s_base[kSweepTile + 1];
+    __shared__ uint32_t s_w[kWavesPerBlock];
+    __shared__ uint32_t s_wide;
+    __shared__ unsigned long long s_hits[kSweepTile];
+    __shared__ uint32_t s_nh, s_gbase;
+    const unsigned long long E = dpre[g.nv];
+    const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
+    const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+    if (threadIdx.x == 0) s_nh = 0;
+    __syncthreads();
+    for (unsigned long long t = blockIdx.x; t < ntiles && t + 1 < tiles.cap; t += gridDim.x) {
+        const uint32_t v0 = tiles.v[t], v1 = tiles.v[t + 1];
+        const unsigned long long e0 = tiles.e[t], e1 = tiles.e[t + 1];
+        const uint32_t ne = (uint32_t)min<unsigned long long>(e1 - e0, kSweepTile);
+        const uint32_t span = min<uint32_t>(v1 - v0 + 1, kSweepTile + 1);
+        const uint64_t off0 = g.off[v0];
+        if (threadIdx.x == 0) s_wide = (g.off[v0 + span] - off0) >= 0x40000000ull;
+#pragma unroll
+        for (int j = 0; j < kSweepPerThread; ++j) s_own[j * kBlock + threadIdx.x] = 0;
+        __syncthreads();
+        const bool wide = s_wide;
+        for (uint32_t i = threadIdx.x; i < span; i += kBlock) {
+            const uint32_t v = v0 + i;
+            const unsigned long long a = dpre[v], b = dpre[v + 1];
+            if (!wide) s_base[i] = (int32_t)((long long)(g.off[v] - off0) - (long long)(a - e0));
+            if (b > a && a >= e0 && a < e0 + ne) s_own[a - e0] = (uint16_t)i;
+        }
+        __syncthreads();
+        uint32_t own[kSweepPerThread];
+        uint32_t m = 0;
+#pragma unroll
+        for (int j = 0; j < kSweepPerThread; ++j) {
+            own[j] = s_own[threadIdx.x * kSweepPerThread + j];
+            m = max(m, own[j]);
+        }
+        uint32_t x = m;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, kWave);
+            if (lane >= (uint32_t)d) x = max(x, y);
+        }
+        if (lane == kWave - 1) s_w[w] = x;
+        uint32_t prev = __shfl_up(x, 1, kWave);
+        if (lane == 0) prev = 0;
+        __syncthreads();
+        for (uint32_t i = 0; i < w; ++i) prev = max(prev, s_w[i]);
+#pragma unroll
+        for (int j = 0; j < kSweepPerThread; ++j) {
+            prev = max(prev, own[j]);
+            s_own[threadIdx.x * kSweepPerThread + j] = (uint16_t)prev;
+        }
+        __syncthreads();
+        uint32_t vv[kSweepPerThread];
+#pragma unroll
+        for (int j = 0; j < kSweepPerThread; ++j) {
+            const uint32_t p = j * kBlock + threadIdx.x;
+            vv[j] = kGap;
+            if (p < ne) {
+                const uint32_t o = s_own[p];
+                uint64_t slot;
+                if (!wide) {
+                    slot = off0 + (uint64_t)((long long)s_base[o] + (long long)p);
+                } else {
+                    const uint32_t v = v0 + o;
+                    slot = g.off[v] + (e0 + p - dpre[v]);
+                }
+                vv[j] = g.col[slot];
+            }
+        }
+        // hits are gathered in LDS (a tile has at most kSweepTile) and
+        // appended with one global atomic per tile
+#pragma unroll
+        for (int j = 0; j < kSweepPerThread; ++j) {
+            const uint32_t v = vv[j];
+            bool hit = false;
+            if (v != kGap && ((coarse[v >> 11] >> ((v >> 6) & 31)) & 1u)) hit = (invbit[v >> 5] >> (v & 31)) & 1u;
+            const uint64_t bm = __ballot(hit);
+            if (bm) {
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)bm) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&s_nh, (uint32_t)__popcll(bm));
+                base = __shfl(base, (int)leader, kWave);
+                if (hit) {
+                    const uint32_t src = v0 + s_own[j * kBlock + threadIdx.x];
+                    s_hits[base + (uint32_t)__popcll(bm & lanemask_lt())] = ((unsigned long long)src << 32) | v;
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t nh = s_nh;
+        if (nh) {
+            if (threadIdx.x == 0) s_gbase = atomicAdd(nhits, nh);
+            __syncthreads();
+            const uint32_t gb = s_gbase;
+            for (uint32_t i = threadIdx.x; i < nh; i += kBlock) {
+                if (gb + i < hit_cap) hits[gb + i] = s_hits[i];
+                else atomicOr(overflow, 8ull);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_nh = 0;
+        __syncthreads();
+    }
+}
+
+// Clear the masks of the vertices with a reset slot (one thread per bitmap word).
+__global__ void k_mclear_inv(uint32_t nv, uint32_t G, const uint32_t *invbit, unsigned long long *inv) {
+    const uint32_t nw = (nv + 31) / 32;
+    for (uint64_t wd = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; wd < nw; wd += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t bits = invbit[wd];
+        while (bits) {
+            const uint32_t b = __builtin_ctz(bits);
+            bits &= bits - 1;
+            const uint64_t v = wd * 32 + b;
+            for (uint32_t grp = 0; grp < G; ++grp) inv[v * G + grp] = 0ull;
+        }
+    }
+}
+
+// Initial state: every slot unreached, each query's source at distance 0 and
+// active; the sources form the first frontier (each vertex once).
+__global__ void k_minit_sources(MultiState ms, const uint32_t *src, uint32_t nq, uint32_t *stamp, uint32_t stamp_val,
+                                uint32_t *list, uint32_t *cnt, uint32_t cap, unsigned long long *overflow) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+        const uint32_t s = src[i], grp = i / ms.QS, slot = i % ms.QS;
+        ms.best[(uint64_t)s * ms.KP + i] = pack(0, kNone);
+        atomicOr(&ms.act[(uint64_t)s * ms.G + grp], 1ull << slot);
+        if (atomicExch(&stamp[s], stamp_val) != stamp_val) {
+            const uint32_t pos = atomicAdd(cnt, 1u);
+            if (pos < cap) list[pos] = s;
+            else atomicOr(overflow, 1ull);
+        }
+    }
+}
+
+// Column q of the state: distance (what = 0) or parent (what = 1).
+__global__ void k_mextract(uint32_t nv, const unsigned long long *best, uint32_t KP, uint32_t q, uint32_t *out, int what) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long b = best[v * KP + q];
+        out[v] = what ? lo32(b) : hi32(b);
+    }
+}
+
+}  // namespace sg
+
+#endif

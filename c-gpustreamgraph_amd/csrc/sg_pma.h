@@ -31,6 +31,7 @@
 namespace sg {
 
 constexpr uint32_t kMergeChunk = 2048;  // entries per merge chunk
+constexpr int kMergePerLane = kMergeChunk / kWave;
 constexpr uint32_t kDelStage = 1024;    // delete targets of a vertex staged in LDS
 constexpr uint32_t kLaneIns = 128;      // insert-only vertices with at most this many inserts: one lane each
 
@@ -53,10 +54,19 @@ struct TInfo {
 __device__ __forceinline__ bool has_del(const TInfo &t) { return t.de > t.db; }
 
 struct MChunk {
-    uint32_t t;       // touched index
+    uint64_t base;    // off[u] when the batch was prepared
+    uint32_t u;
     uint32_t lo, hi;  // positions in the vertex's sequence (old list ++ inserts, or inserts only)
-    uint32_t pad;
+    uint32_t d0, ib, db, de;  // as TInfo
+    uint32_t first;   // the vertex's first chunk (TInfo::cbase)
+    uint32_t pad[2];
 };
+__device__ __forceinline__ TInfo chunk_info(const MChunk &m) { return TInfo{m.u, m.d0, 0, m.ib, m.db, m.de, m.first, 0}; }
+__device__ __forceinline__ MChunk load_chunk(const MChunk *chunks, uint64_t c, uint64_t n) {
+    if (c < n) return chunks[c];
+    MChunk z{};
+    return z;
+}
 
 __device__ __forceinline__ bool in_sorted_u64(const uint64_t *a, uint32_t b, uint32_t e, uint64_t key) {
     uint32_t lo = b, hi = e;
@@ -153,12 +163,17 @@ __global__ void __launch_bounds__(kBlock) k_touch_info(GraphView g, BatchRanges 
             tidx[ti.u] = (uint32_t)t;
             const uint32_t len = seq_len(ti);
             for (uint32_t c = 0; c < ti.nch; ++c) {
-                MChunk m;
-                m.t = (uint32_t)t;
+                MChunk m{};
+                m.base = g.off[ti.u];
+                m.u = ti.u;
                 m.lo = c * kMergeChunk;
                 m.hi = min(len, m.lo + kMergeChunk);
-                m.pad = 0;
-                chunks[cb + c] = m;
+                m.d0 = ti.d0;
+                m.ib = ti.ib;
+                m.db = ti.db;
+                m.de = ti.de;
+                m.first = ti.cbase;
+                chunks[ti.cbase + c] = m;
             }
         }
     }
@@ -221,37 +236,34 @@ __global__ void __launch_bounds__(kBlock) k_chunk_filter(GraphView g, BatchView 
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
     const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerBlock + w;
     const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
+    MChunk mn = load_chunk(chunks, gw, n);  // descriptors are prefetched one iteration ahead
     for (uint64_t c = gw; c < n; c += nw) {
-        const MChunk m = chunks[c];
-        const TInfo ti = tinfo[m.t];
+        const MChunk m = mn;
+        mn = load_chunk(chunks, c + nw, n);
+        const TInfo ti = chunk_info(m);
         if (!has_del(ti)) {
             if (!kPass && lane == 0) ccount[c] = m.hi - m.lo;
             continue;
         }
-        const uint32_t *old = g.col + g.off[ti.u];
+        const uint32_t *old = g.col + m.base;
         const uint32_t *ins = bv.ins_val + ti.ib;
         const uint32_t ns = stage_deletes(bv, ti, s_del[w], s_mask[w]);
+        // the whole chunk is loaded before the first probe: one memory latency per chunk
+        uint32_t x[kMergePerLane];
+#pragma unroll
+        for (int j = 0; j < kMergePerLane; ++j) {
+            const uint32_t i = m.lo + j * kWave + lane;
+            x[j] = i < m.hi ? (i < ti.d0 ? old[i] : ins[i - ti.d0]) : 0;
+        }
         uint32_t cnt = 0;
         uint32_t *dst = kPass ? out + cpos[c] : nullptr;
-        for (uint32_t k0 = m.lo; k0 < m.hi; k0 += kIlp * kWave) {
-            uint32_t x[kIlp];
-            bool keep[kIlp];
 #pragma unroll
-            for (int j = 0; j < kIlp; ++j) {  // kIlp independent loads in flight
-                const uint32_t i = k0 + j * kWave + lane;
-                x[j] = i < m.hi ? (i < ti.d0 ? old[i] : ins[i - ti.d0]) : 0;
-            }
-#pragma unroll
-            for (int j = 0; j < kIlp; ++j) {
-                const uint32_t i = k0 + j * kWave + lane;
-                keep[j] = i < m.hi && survives(x[j], bv, ti, s_del[w], s_mask[w], ns);
-            }
-#pragma unroll
-            for (int j = 0; j < kIlp; ++j) {  // survivors stay in sequence order
-                const uint64_t b = __ballot(keep[j]);
-                if (kPass && keep[j]) dst[cnt + __popcll(b & lanemask_lt())] = x[j];
-                cnt += (uint32_t)__popcll(b);
-            }
+        for (int j = 0; j < kMergePerLane; ++j) {  // survivors stay in sequence order
+            const uint32_t i = m.lo + j * kWave + lane;
+            const bool keep = i < m.hi && survives(x[j], bv, ti, s_del[w], s_mask[w], ns);
+            const uint64_t b = __ballot(keep);
+            if (kPass && keep) dst[cnt + __popcll(b & lanemask_lt())] = x[j];
+            cnt += (uint32_t)__popcll(b);
         }
         if (!kPass && lane == 0) ccount[c] = cnt;
         __builtin_amdgcn_wave_barrier();
@@ -278,27 +290,27 @@ __global__ void __launch_bounds__(kBlock) k_chunk_place(GraphView g, BatchView b
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
     const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerBlock + w;
     const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
+    MChunk mn = load_chunk(chunks, gw, n);
     for (uint64_t c = gw; c < n; c += nw) {
-        const MChunk m = chunks[c];
-        const TInfo ti = tinfo[m.t];
+        const MChunk m = mn;
+        mn = load_chunk(chunks, c + nw, n);
+        const TInfo ti = chunk_info(m);
         if (skip_covered && covered(maximal, nleaves, L, ti.u)) continue;
-        uint32_t *lst = col + g.off[ti.u];
+        uint32_t *lst = col + m.base;
         const bool hd = has_del(ti);
         const uint64_t p0 = (hd ? 0 : ti.d0) + (cpos[c] - cpos[ti.cbase]);
         const uint64_t len = cpos[c + 1] - cpos[c];
         const uint32_t *srcp = hd ? merged + cpos[c] : bv.ins_val + ti.ib + m.lo;
-        for (uint64_t k0 = 0; k0 < len; k0 += kIlp * kWave) {
-            uint32_t x[kIlp];
+        uint32_t x[kMergePerLane];  // len <= kMergeChunk: one load per lane and slot, all in flight
 #pragma unroll
-            for (int j = 0; j < kIlp; ++j) {
-                const uint64_t k = k0 + j * kWave + lane;
-                x[j] = k < len ? srcp[k] : 0;
-            }
+        for (int j = 0; j < kMergePerLane; ++j) {
+            const uint64_t k = (uint64_t)j * kWave + lane;
+            x[j] = k < len ? srcp[k] : 0;
+        }
 #pragma unroll
-            for (int j = 0; j < kIlp; ++j) {
-                const uint64_t k = k0 + j * kWave + lane;
-                if (k < len) lst[p0 + k] = x[j];
-            }
+        for (int j = 0; j < kMergePerLane; ++j) {
+            const uint64_t k = (uint64_t)j * kWave + lane;
+            if (k < len) lst[p0 + k] = x[j];
         }
         if (hd) {
             const uint32_t nd = newdeg[ti.u];
@@ -322,95 +334,234 @@ __global__ void k_lane_place(GraphView g, BatchView bv, uint32_t *col, const TIn
 }
 
 // ---- edge-centric sweep ----------------------------------------------------------
-// A block takes a tile of kSweepTile live edges (positions in the concatenation
-// of all adjacency lists, dpre = exclusive prefix of the degrees), finds the
-// tile's source range once, stages that slice of dpre in LDS and maps every
-// edge to its source with an LDS binary search. Every block does the same
-// amount of work whatever the degree distribution.
-constexpr uint32_t kSweepTile = 1024;
-constexpr uint32_t kSweepStage = 2048;
+// Live edges are walked along a merge path over vertices and edges: path
+// position v + dpre[v] (dpre = exclusive prefix of the degrees) is the first
+// edge of vertex v, the position after its last edge is v's end mark. A tile
+// is kSweepTile consecutive path positions, so it holds at most kSweepTile
+// edges and kSweepTile + 1 sources whatever the degree distribution.
+constexpr uint32_t kSweepTile = 2048;
+constexpr uint32_t kScanBlock = 256;
+constexpr uint32_t kScanIters = 16;
+constexpr uint32_t kScanSpan = kScanBlock * kScanIters;  // vertices per block of the degree scan
 
-__device__ __forceinline__ uint32_t upper_bound_u64(const unsigned long long *a, uint32_t n, unsigned long long key) {
-    uint32_t lo = 0, hi = n;
-    while (lo < hi) { uint32_t m = (lo + hi) >> 1; if (a[m] <= key) lo = m + 1; else hi = m; }
-    return lo;
+struct SweepTiles {
+    uint32_t *v;             // [cap] first source of tile t (tile cap - 1 is the sentinel)
+    unsigned long long *e;   // [cap] first live-edge index of tile t
+    uint64_t cap;            // entries of v and e
+};
+
+__device__ __forceinline__ unsigned long long wave_incl_scan(unsigned long long x) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned long long y = __shfl_up(x, d, kWave);
+        if (lane >= d) x += y;
+    }
+    return x;
 }
 
-// tile_owner[t] = source of live edge t * kSweepTile, for t in [0, ntiles];
-// tile_owner[ntiles] = nv - 1. One thread per vertex writes the boundaries
-// that fall in its range.
-__global__ void k_tile_owners(uint32_t nv, const unsigned long long *dpre, uint32_t *tile_owner) {
-    const unsigned long long E = dpre[nv];
-    const unsigned long long ntiles = (E + kSweepTile - 1) / kSweepTile;
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {
-        const unsigned long long a = dpre[v], b = dpre[v + 1];
-        for (unsigned long long t = (a + kSweepTile - 1) / kSweepTile; t * kSweepTile < b; ++t) tile_owner[t] = (uint32_t)v;
-        if (v == 0) tile_owner[ntiles] = nv - 1;
+// Block-wide exclusive scan of one value per thread; *total gets the sum.
+// s_w holds kScanBlock / kWave words; callers alternate two of them.
+__device__ __forceinline__ unsigned long long block_excl_scan(unsigned long long x, unsigned long long *s_w,
+                                                              unsigned long long &total) {
+    const unsigned long long inc = wave_incl_scan(x);
+    const uint32_t w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == kWave - 1) s_w[w] = inc;
+    __syncthreads();
+    unsigned long long before = 0;
+    total = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kScanBlock / kWave; ++i) {
+        if (i < w) before += s_w[i];
+        total += s_w[i];
+    }
+    return before + inc - x;
+}
+
+// Degree scan, phase 1: the degree sum of each kScanSpan-vertex block.
+__global__ void __launch_bounds__(kScanBlock) k_deg_block_sums(uint32_t nv, const uint32_t *deg, unsigned long long *bsum) {
+    __shared__ unsigned long long s_w[kScanBlock / kWave];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanSpan;
+    unsigned long long x = 0;
+#pragma unroll 4
+    for (uint32_t i = 0; i < kScanIters; ++i) {
+        const uint64_t v = base + (uint64_t)i * kScanBlock + threadIdx.x;
+        if (v < nv) x += deg[v];
+    }
+    unsigned long long total;
+    block_excl_scan(x, s_w, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// Phase 2 (one block): exclusive scan of the nb block sums in place.
+__global__ void __launch_bounds__(1024) k_deg_scan_sums(uint32_t nb, unsigned long long *bsum) {
+    __shared__ unsigned long long s_w[1024 / kWave];
+    __shared__ unsigned long long s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+        const uint32_t i = b0 + threadIdx.x;
+        const unsigned long long x = i < nb ? bsum[i] : 0;
+        const unsigned long long inc = wave_incl_scan(x);
+        const uint32_t w = threadIdx.x / kWave;
+        if ((threadIdx.x & (kWave - 1)) == kWave - 1) s_w[w] = inc;
+        __syncthreads();
+        unsigned long long before = s_carry, total = 0;
+        for (uint32_t j = 0; j < 1024 / kWave; ++j) {
+            if (j < w) before += s_w[j];
+            total += s_w[j];
+        }
+        if (i < nb) bsum[i] = before + inc - x;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry += total;
+        __syncthreads();
     }
 }
 
-template <class Op, class EFn>
-__global__ void __launch_bounds__(kBlock) k_sweep_edges(GraphView g, const unsigned long long *dpre,
-                                                        const uint32_t *tile_owner, Op op,
-                                                        const typename Op::buffer_t *buffer, EFn efn, PassIO io) {
-    __shared__ long long s_pre[kSweepStage];
-    __shared__ uint32_t s_out[kWavesPerBlock][kOutStage];
-    __shared__ unsigned long long s_red[kWavesPerBlock];
-    const unsigned long long E = dpre[g.nv];
-    const uint32_t w = threadIdx.x / kWave;
-    OutStage st{s_out[w], 0};
-    unsigned long long n_edges = 0;
-    for (unsigned long long t0 = (unsigned long long)blockIdx.x * kSweepTile; t0 < E;
-         t0 += (unsigned long long)gridDim.x * kSweepTile) {
-        const unsigned long long t1 = min(t0 + kSweepTile, E);
-        const unsigned long long tile = t0 / kSweepTile;
-        // sources of the tile: from the owner of its first edge to the owner
-        // of the next tile's first edge (a superset; extra vertices start past t1)
-        const uint32_t v0 = tile_owner[tile], span = tile_owner[tile + 1] - v0 + 1;
-        const bool staged = span <= kSweepStage;
-        if (staged)
-            for (uint32_t i = threadIdx.x; i < span; i += blockDim.x) s_pre[i] = (long long)dpre[v0 + i] - (long long)t0;
-        __syncthreads();
-        uint32_t v[kIlp], src[kIlp];
-        uint64_t peek[kIlp];
-#pragma unroll
-        for (int j = 0; j < kIlp; ++j) {
-            const unsigned long long e = t0 + (unsigned long long)j * kBlock + threadIdx.x;
-            v[j] = kGap;
-            src[j] = 0;
-            if (e < t1) {
-                uint32_t o;
-                if (staged) {
-                    const long long rel = (long long)(e - t0);
-                    uint32_t lo = 0, hi = span;  // largest o with s_pre[o] <= rel
-                    while (hi - lo > 1) { const uint32_t m = (lo + hi) >> 1; if (s_pre[m] <= rel) lo = m; else hi = m; }
-                    o = lo;
-                    src[j] = v0 + o;
-                    v[j] = g.col[g.off[v0 + o] + (uint64_t)(rel - s_pre[o])];
+// Phase 3: dpre[v] for v in [0, nv] and the merge-path tile boundaries:
+// tile t starts at path position d = t * kSweepTile, inside vertex v with
+// v + dpre[v] <= d < v + 1 + dpre[v + 1], at edge min(d - v, dpre[v + 1]).
+// Tile ntiles (= ceil((nv + E) / kSweepTile)) is the sentinel {nv - 1, E}.
+__global__ void __launch_bounds__(kScanBlock) k_deg_scan_write(uint32_t nv, const uint32_t *deg,
+                                                               const unsigned long long *bsum, unsigned long long *dpre,
+                                                               SweepTiles tiles, unsigned long long *overflow) {
+    __shared__ unsigned long long s_w[2][kScanBlock / kWave];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanSpan;
+    unsigned long long carry = bsum[blockIdx.x];
+    for (uint32_t i = 0; i < kScanIters; ++i) {
+        const uint64_t v = base + (uint64_t)i * kScanBlock + threadIdx.x;
+        const uint32_t d = v < nv ? deg[v] : 0u;
+        unsigned long long total;
+        const unsigned long long a = carry + block_excl_scan(d, s_w[i & 1], total);
+        carry += total;
+        if (v < nv) {
+            dpre[v] = a;
+            const unsigned long long b = a + d;
+            const unsigned long long p0 = v + a, p1 = v + 1 + b;  // path range of v
+            for (unsigned long long t = (p0 + kSweepTile - 1) / kSweepTile; t * kSweepTile < p1; ++t) {
+                if (t >= tiles.cap - 1) { atomicOr(overflow, 4ull); break; }
+                tiles.v[t] = (uint32_t)v;
+                tiles.e[t] = min(t * kSweepTile - v, b);
+            }
+            if (v + 1 == nv) {
+                dpre[nv] = b;
+                const unsigned long long nt = (p1 + kSweepTile - 1) / kSweepTile;
+                if (nt < tiles.cap) {
+                    tiles.v[nt] = (uint32_t)v;
+                    tiles.e[nt] = b;
                 } else {
-                    o = upper_bound_u64(dpre + v0, span, e) - 1;
-                    src[j] = v0 + o;
-                    v[j] = g.col[g.off[v0 + o] + (e - dpre[v0 + o])];
+                    atomicOr(overflow, 4ull);
                 }
             }
         }
+    }
+}
+
+// One block per tile (grid-stride). Each source of the tile gets its slot base
+// relative to the tile's first source (off[v] - off[v0] - (dpre[v] - e0), 32
+// bits) in LDS, the first edge of every source with edges is marked with its
+// 16-bit index, a block max-scan fills the owners forward, and the tile's
+// edges are loaded coalesced, kSweepPerThread per thread, all in flight. A
+// tile whose slot range does not fit 32 bits takes the slot of each edge from
+// off and dpre in global memory instead.
+constexpr int kSweepPerThread = kSweepTile / kBlock;
+constexpr int kSweepGroup = 8;         // bitmap probes in flight per thread
+constexpr uint32_t kSweepOutStage = 128;
+
+template <class Op, class EFn>
+__global__ void __launch_bounds__(kBlock) k_sweep_edges(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
+                                                        Op op, const typename Op::buffer_t *buffer, EFn efn, PassIO io) {
+    __shared__ uint16_t s_own[kSweepTile];
+    __shared__ int32_t s_base[kSweepTile + 1];
+    __shared__ uint32_t s_out[kWavesPerBlock][kSweepOutStage];
+    __shared__ uint32_t s_w[kWavesPerBlock];
+    __shared__ uint32_t s_wide;
+    const unsigned long long E = dpre[g.nv];
+    const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
+    const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+    OutStage st{s_out[w], 0, kSweepOutStage};
+    unsigned long long n_edges = 0;
+    for (unsigned long long t = blockIdx.x; t < ntiles && t + 1 < tiles.cap; t += gridDim.x) {
+        const uint32_t v0 = tiles.v[t], v1 = tiles.v[t + 1];
+        const unsigned long long e0 = tiles.e[t], e1 = tiles.e[t + 1];
+        const uint32_t ne = (uint32_t)min<unsigned long long>(e1 - e0, kSweepTile);
+        const uint32_t span = min<uint32_t>(v1 - v0 + 1, kSweepTile + 1);
+        const uint64_t off0 = g.off[v0];
+        if (threadIdx.x == 0) s_wide = (g.off[v0 + span] - off0) >= 0x40000000ull;
 #pragma unroll
-        for (int j = 0; j < kIlp; ++j) peek[j] = v[j] != kGap ? efn.load(v[j]) : 0;
-        bool enq[kIlp];
+        for (int j = 0; j < kSweepPerThread; ++j) s_own[j * kBlock + threadIdx.x] = 0;
+        __syncthreads();
+        const bool wide = s_wide;
+        for (uint32_t i = threadIdx.x; i < span; i += kBlock) {
+            const uint32_t v = v0 + i;
+            const unsigned long long a = dpre[v], b = dpre[v + 1];
+            if (!wide) s_base[i] = (int32_t)((long long)(g.off[v] - off0) - (long long)(a - e0));
+            if (b > a && a >= e0 && a < e0 + ne) s_own[a - e0] = (uint16_t)i;
+        }
+        __syncthreads();
+        // forward fill of the owners: thread k holds positions [k * P, k * P + P)
+        uint32_t own[kSweepPerThread];
+        uint32_t m = 0;
 #pragma unroll
-        for (int j = 0; j < kIlp; ++j) {
-            enq[j] = false;
-            if (v[j] != kGap && ((peek[j] >> (v[j] & 31)) & 1u))
-                enq[j] = efn.apply(src[j], v[j], op.PushValue(ld_peek(&buffer[src[j]])), peek[j]);
+        for (int j = 0; j < kSweepPerThread; ++j) {
+            own[j] = s_own[threadIdx.x * kSweepPerThread + j];
+            m = max(m, own[j]);
+        }
+        uint32_t x = m;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t y = __shfl_up(x, d, kWave);
+            if (lane >= (uint32_t)d) x = max(x, y);
+        }
+        if (lane == kWave - 1) s_w[w] = x;
+        uint32_t prev = __shfl_up(x, 1, kWave);
+        if (lane == 0) prev = 0;
+        __syncthreads();
+        for (uint32_t i = 0; i < w; ++i) prev = max(prev, s_w[i]);
+#pragma unroll
+        for (int j = 0; j < kSweepPerThread; ++j) {
+            prev = max(prev, own[j]);
+            s_own[threadIdx.x * kSweepPerThread + j] = (uint16_t)prev;
+        }
+        __syncthreads();
+        uint32_t vv[kSweepPerThread];
+#pragma unroll
+        for (int j = 0; j < kSweepPerThread; ++j) {
+            const uint32_t p = j * kBlock + threadIdx.x;
+            vv[j] = kGap;
+            if (p < ne) {
+                const uint32_t o = s_own[p];
+                uint64_t slot;
+                if (!wide) {
+                    slot = off0 + (uint64_t)((long long)s_base[o] + (long long)p);
+                } else {
+                    const uint32_t v = v0 + o;
+                    slot = g.off[v] + (e0 + p - dpre[v]);
+                }
+                vv[j] = g.col[slot];
+            }
         }
 #pragma unroll
-        for (int j = 0; j < kIlp; ++j) stage_push(enq[j], v[j], st, io);
-        n_edges += t1 - t0;
+        for (int j0 = 0; j0 < kSweepPerThread; j0 += kSweepGroup) {
+            uint64_t peek[kSweepGroup];
+#pragma unroll
+            for (int j = 0; j < kSweepGroup; ++j) peek[j] = vv[j0 + j] != kGap ? efn.load(vv[j0 + j]) : 0;
+#pragma unroll
+            for (int j = 0; j < kSweepGroup; ++j) {
+                const uint32_t v = vv[j0 + j];
+                bool enq = false;
+                if (v != kGap && ((peek[j] >> (v & 31)) & 1u)) {
+                    const uint32_t src = v0 + s_own[(j0 + j) * kBlock + threadIdx.x];
+                    enq = efn.apply(src, v, op.PushValue(ld_peek(&buffer[src])), peek[j]);
+                }
+                stage_push(enq, v, st, io);
+            }
+        }
+        n_edges += ne;
         __syncthreads();
     }
     stage_flush(st, io);
     if (threadIdx.x == 0 && n_edges) atomicAdd(io.stats + 0, n_edges);
-    (void)s_red;
 }
 
 // Invalidated vertices that are active on their own (CC: own label) join the

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libsgraph.so")
 
 SG_OK, SG_EINVAL, SG_ENOMEM, SG_EBUDGET, SG_EDEVICE, SG_EIO = range(6)
 STATUS_NAMES = {0: "SG_OK", 1: "SG_EINVAL", 2: "SG_ENOMEM", 3: "SG_EBUDGET", 4: "SG_EDEVICE", 5: "SG_EIO"}
-ALGO = {"pr": 0, "sssp": 1, "bfs": 2, "cc": 3}
+ALGO = {"pr": 0, "sssp": 1, "bfs": 2, "cc": 3, "multi": 4}
 WEIGHT = {"hash128": 0, "seeded": 1, "one": 2}
 Q_VALUE, Q_BUFFER, Q_PARENT, Q_DEGREE, Q_OFFSETS, Q_SLOTS = range(6)
 FLAG_KERNEL_TIMING = 0x1
@@ -70,6 +70,8 @@ SIGNATURES = [
     ("sg_apply_snapshot", C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint64]),
     ("sg_apply_snapshot_device", C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint64]),
     ("sg_gather", C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
+    ("sg_init_queries", C.c_int, [_P, _P, _P, C.c_uint32]),
+    ("sg_gather_query", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_size_t]),
     ("sg_get_stats", C.c_int, [_P, C.POINTER(Stats)]),
     ("sg_reset_stats", C.c_int, [_P]),
     ("sg_device_alloc", C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
@@ -232,6 +234,20 @@ class Engine:
         else:
             s = np.array([source], dtype=np.uint32)
             _check(lib().sg_init(self.h, _ptr(s), 1), self.h)
+
+    def init_queries(self, sources, kinds):
+        """Batched queries (engine created with algo "multi"): kinds are "sssp" / "bfs"."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        k = np.array([ALGO[x] if isinstance(x, str) else int(x) for x in kinds], dtype=np.uint32)
+        if len(k) != len(src):
+            raise ValueError("one kind per source")
+        _check(lib().sg_init_queries(self.h, _ptr(src), _ptr(k), len(src)), self.h)
+
+    def query(self, qi, q=None):
+        """Distances (or parents with q=Q_PARENT) of query qi of a batched engine."""
+        out = np.empty(self.nv, dtype=np.uint32)
+        _check(lib().sg_gather_query(self.h, qi, Q_VALUE if q is None else q, _ptr(out), out.nbytes), self.h)
+        return out
 
     def apply(self, ins, dele):
         ins = np.ascontiguousarray(ins, dtype=np.uint32).reshape(-1)

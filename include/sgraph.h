@@ -38,7 +38,10 @@ typedef enum sg_algo {
     SG_ALGO_PR = 0,   /* operator PageRank, samples/hybrid_pr/hybrid_pr.cu:25-152   */
     SG_ALGO_SSSP = 1, /* operator SSSP, samples/hybrid_sssp/hybrid_sssp.cu:28-194   */
     SG_ALGO_BFS = 2,  /* operator BFS, samples/hybrid_bfs/hybrid_bfs.cu:26-167      */
-    SG_ALGO_CC = 3    /* operator CC, samples/hybrid_cc/hybrid_cc.cu:26-170         */
+    SG_ALGO_CC = 3,   /* operator CC, samples/hybrid_cc/hybrid_cc.cu:26-170         */
+    SG_ALGO_MULTI = 4 /* batched BFS/SSSP queries on one graph (sg_init_queries); no
+                         reference counterpart: the reference answers one source per
+                         process (samples/hybrid_sssp/main.cu, samples/hybrid_bfs/main.cu) */
 } sg_algo;
 
 /* Edge-weight function of SSSP. Weights are computed from (u, v) on the fly
@@ -137,6 +140,23 @@ sg_status sg_load_csr(sg_engine *eng, uint32_t nv, uint64_t ne,
  * sources: SSSP/BFS source vertex (nsrc must be 1); ignored by PR and CC.
  * Replaces Engine::InitGraph + Engine::Start (framework.cuh:389-439, 549-584). */
 sg_status sg_init(sg_engine *eng, const uint32_t *sources, uint32_t nsrc);
+
+/* SG_ALGO_MULTI engines: start nq queries on the loaded graph, query i from
+ * sources[i], of kind kinds[i] (SG_ALGO_SSSP: weights of cfg.weight_mode;
+ * SG_ALGO_BFS: unit weights, i.e. levels), and run them to convergence
+ * together. Every later sg_apply_snapshot* updates the graph once and brings
+ * all queries to their fixed points with one launch per round. Each query's
+ * distances and parents equal those of a single-source SSSP/BFS engine.
+ * sg_init on such an engine starts nsrc queries of one kind (BFS when
+ * cfg.weight_mode is SG_WEIGHT_ONE, else SSSP). At most 4096 queries.
+ * Batches the per-source runs of Engine::InitGraph + Engine::Start
+ * (framework.cuh:389-439, 549-584); no reference counterpart as one call. */
+sg_status sg_init_queries(sg_engine *eng, const uint32_t *sources, const uint32_t *kinds, uint32_t nq);
+
+/* Per-query result of an SG_ALGO_MULTI engine: what = SG_Q_VALUE (u32
+ * distance or level, UINT32_MAX = unreached) or SG_Q_PARENT; bytes >= 4 * nv.
+ * GatherValue / GatherParent (graph_datum.cuh:297-335) for query qi. */
+sg_status sg_gather_query(sg_engine *eng, uint32_t qi, uint32_t what, void *out, size_t bytes);
 
 /* Apply one snapshot from host memory: ins_uv/del_uv are (u, v) pairs, i.e.
  * 2*nins / 2*ndel uint32 values. Inserts are applied first, then deletes
