@@ -91,6 +91,140 @@ def run_batched(args, row, col, ins, dele, wseed, algos, plain_value):
     }
 
 
+def run_oom(args):
+    """Out-of-memory path (PageRank + CC with the edge array in pinned host
+    memory under an edge budget, the HBM edge cache and SEGMENT streaming):
+    updated-edge GTEPS, host-link GB/s of the streamed segments, cache hit rate."""
+    nsnap = args.warmup + args.steps
+    t = time.perf_counter()
+    nv, row, col = sgraph.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
+    ins, dele = sgraph.gen_updates(row, col, args.scale, nsnap, args.batch, args.batch, seed=args.seed + 1,
+                                   hub_frac=0.8, hub_top=0.01)
+    log(f"[bench] oom graph nv={nv} ne={len(col)} gen {time.perf_counter() - t:.1f}s")
+    res = {}
+    for a in args.algos.split(","):
+        e = sgraph.Engine(a, edge_budget_gb=args.edge_budget_gb, cache_level=args.cache, segments=args.segments)
+        t = time.perf_counter()
+        e.load_csr(row, col)
+        e.init()
+        st = e.stats()
+        log(f"[bench] oom {a}: host_edges={st['host_edges']} load+static solve {time.perf_counter() - t:.1f}s "
+            f"cache={st['cache_slots']} cached_vertices={st['cached_vertices']}")
+        for s in range(args.warmup):
+            e.apply(ins[s], dele[s])
+        e.reset_stats()
+        lat, num = [], 0
+        for s in range(args.warmup, nsnap):
+            t0 = time.perf_counter()
+            e.apply(ins[s], dele[s])
+            lat.append(time.perf_counter() - t0)
+            num += e.stats()["ne"]
+            log(f"[bench] oom {a} snapshot {s}: {lat[-1] * 1e3:.1f} ms")
+        st = e.stats()
+        e.close()
+        res[a] = {"gteps": num / sum(lat) / 1e9, "lat_ms": [x * 1e3 for x in lat], "stats": st}
+    num = sum(sum(x["lat_ms"]) for x in res.values())
+    edges = 0
+    for a, x in res.items():
+        edges += x["gteps"] * sum(x["lat_ms"]) * 1e6
+    value = edges / (num * 1e-3) / 1e9
+    sb = sum(x["stats"]["streamed_bytes"] for x in res.values())
+    sms = sum(x["stats"]["stream_ms"] for x in res.values())
+    rel = sum(x["stats"]["edges_relaxed"] for x in res.values())
+    hit = sum(x["stats"]["edges_from_cache"] for x in res.values())
+    all_lat = np.array([y for x in res.values() for y in x["lat_ms"]])
+    return {
+        "metric": "updated-edge GTEPS (incremental PR+CC, out-of-memory path: edges in pinned host memory)",
+        "value": round(value, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "dtype": "u32/f32", "data": "synthetic (seeded R-MAT, hub-biased update stream)",
+        "config": {"workload": f"PR+CC on R-MAT scale {args.scale} EF{args.edge_factor}, edge budget "
+                               f"{args.edge_budget_gb} GiB, -cache {args.cache}, SEGMENT {args.segments}, "
+                               f"{args.batch}+{args.batch} updates/snapshot (80% of inserts from the top-1% sources)",
+                   "nv": nv, "ne0": len(col)},
+        "latency_p50_ms": round(float(np.percentile(all_lat, 50)), 3),
+        "latency_p99_ms": round(float(np.percentile(all_lat, 99)), 3),
+        "host_link": {"streamed_bytes": sb, "stream_ms": round(sms, 3),
+                      "achieved_gbs": round(sb / (sms * 1e-3) / 1e9, 2) if sms else None, "peak_gbs": 64.0},
+        "cache_hit_rate": round(hit / rel, 4) if rel else None,
+        "per_algo": {a: {"gteps": round(x["gteps"], 4), "p50_ms": round(float(np.percentile(x["lat_ms"], 50)), 3),
+                         "host_edges": x["stats"]["host_edges"], "cached_vertices": x["stats"]["cached_vertices"],
+                         "host_bytes": x["stats"]["host_bytes"], "device_bytes": x["stats"]["device_bytes"]}
+                     for a, x in res.items()},
+    }
+
+
+def run_serve(args):
+    """Request loop: Poisson update batches and mixed queries against one
+    resident graph; throughput and p50/p99 request latency."""
+    import threading
+    rng = np.random.default_rng(args.seed)
+    t = time.perf_counter()
+    nv, row, col = sgraph.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
+    pool = 32
+    ins, dele = sgraph.gen_updates(row, col, args.scale, pool, args.max_update // 2, args.max_update // 2,
+                                   seed=args.seed + 1)
+    log(f"[bench] serve graph nv={nv} ne={len(col)} gen {time.perf_counter() - t:.1f}s")
+    srv = sgraph.Server(row, col, query_slots=args.slots, weight_mode="seeded", weight_seed=args.seed + 2)
+    log(f"[bench] serve ready {time.perf_counter() - t:.1f}s")
+    dur = args.duration
+    lock = threading.Lock()
+    done = {"updates": 0, "queries": 0, "lookups": 0}
+
+    def updates():
+        r = np.random.default_rng(args.seed + 3)
+        t_end = time.perf_counter() + dur
+        k = 0
+        while time.perf_counter() < t_end:
+            time.sleep(r.exponential(1.0 / args.update_rate))
+            size = int(np.exp(r.uniform(np.log(100), np.log(args.max_update))))
+            b = k % pool
+            srv.wait(srv.submit(sgraph.REQ_UPDATE, ins=ins[b][: size // 2], dele=dele[b][: size - size // 2]))
+            k += 1
+            with lock:
+                done["updates"] += 1
+
+    def queries(cid):
+        r = np.random.default_rng(args.seed + 10 + cid)
+        t_end = time.perf_counter() + dur
+        pend = []
+        while time.perf_counter() < t_end:
+            time.sleep(r.exponential(args.clients / args.query_rate))
+            x = r.random()
+            kind = sgraph.REQ_BFS if x < 0.4 else sgraph.REQ_SSSP if x < 0.7 else sgraph.REQ_CC if x < 0.9 else sgraph.REQ_PR_TOPK
+            arg = int(r.integers(0, nv)) if kind != sgraph.REQ_PR_TOPK else 10
+            tgt = int(r.integers(0, nv)) if kind in (sgraph.REQ_BFS, sgraph.REQ_SSSP) else sgraph.INF
+            pend.append((kind, arg, tgt, srv.submit(kind, arg, target=tgt)))
+            while len(pend) > 64:
+                kk, aa, tt, tk = pend.pop(0)
+                srv.wait(tk, kk, aa, tt)
+        for kk, aa, tt, tk in pend:
+            srv.wait(tk, kk, aa, tt)
+
+    th = [threading.Thread(target=updates)] + [threading.Thread(target=queries, args=(i,)) for i in range(args.clients)]
+    t0 = time.perf_counter()
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return {
+        "metric": "request loop: answered-query GTEPS and p50/p99 request latency (1 MI355X)",
+        "value": round(st["edges_answered"] / wall / 1e9, 4), "unit": "GTEPS", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "u32/f32", "data": "synthetic (seeded R-MAT, Poisson arrivals)",
+        "config": {"workload": f"serving on R-MAT scale {args.scale} EF{args.edge_factor}: updates {args.update_rate}/s "
+                               f"(log-uniform 100..{args.max_update} pairs), queries {args.query_rate}/s "
+                               f"(40% BFS, 30% SSSP, 20% CC label, 10% PR top-10), {args.slots} query slots, "
+                               f"{dur:.0f} s", "nv": nv, "ne0": len(col)},
+        "requests_per_s": round((st["queries"] + st["lookups"] + st["snapshots"]) / wall, 1),
+        "latency_p50_ms": round(st["p50_ms"], 3), "latency_p99_ms": round(st["p99_ms"], 3),
+        "query_p50_ms": round(st["query_p50_ms"], 3), "query_p99_ms": round(st["query_p99_ms"], 3),
+        "update_p50_ms": round(st["update_p50_ms"], 3), "update_p99_ms": round(st["update_p99_ms"], 3),
+        "server": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,7 +233,7 @@ def main():
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--batch", type=int, default=100000, help="inserts and deletes per snapshot")
-    ap.add_argument("--algos", default="sssp,bfs")
+    ap.add_argument("--algos", default=None, help="default sssp,bfs (incremental) or pr,cc (oom)")
     ap.add_argument("--seed", type=int, default=sgraph.DEFAULT_SEED)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="do not time the relax kernel with HIP events")
@@ -107,12 +241,31 @@ def main():
     ap.add_argument("--heavy-degree", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (shared-graph) measurement")
     ap.add_argument("--batched-only", action="store_true", help="measure only the batched path (profiling)")
+    ap.add_argument("--workload", default="incremental", choices=["incremental", "oom", "serve"],
+                    help="incremental: the headline line; oom: out-of-memory path; serve: request loop")
+    ap.add_argument("--edge-budget-gb", type=float, default=8.0, help="oom: device budget of the edge array")
+    ap.add_argument("--cache", type=int, default=4, help="oom: -cache level")
+    ap.add_argument("--segments", type=int, default=512, help="oom: SEGMENT count")
+    ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals")
+    ap.add_argument("--update-rate", type=float, default=200.0)
+    ap.add_argument("--query-rate", type=float, default=500.0)
+    ap.add_argument("--max-update", type=int, default=100000)
+    ap.add_argument("--slots", type=int, default=64)
+    ap.add_argument("--clients", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU "
               f"(every box has a single MI355X); run with --gpus 1", file=sys.stderr)
         return 2
 
+    if args.algos is None:
+        args.algos = "pr,cc" if args.workload == "oom" else "sssp,bfs"
+    if args.workload == "oom":
+        print(json.dumps(run_oom(args)), flush=True)
+        return 0
+    if args.workload == "serve":
+        print(json.dumps(run_serve(args)), flush=True)
+        return 0
     algos = args.algos.split(",")
     nsnap = args.warmup + args.steps
     t = time.perf_counter()

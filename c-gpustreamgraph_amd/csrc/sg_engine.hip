@@ -119,6 +119,103 @@ __global__ void k_leaf_init(uint32_t nv, uint32_t L, uint64_t nleaves, const uin
     }
 }
 
+// SEGMENT split of Engine::LoadGraph (framework.cuh:310-362): from the first
+// vertex a of a segment, take vertices while the accumulated slot span
+// (gaps included) is below ceil(capacity / nseg); the next segment starts at
+// the first b with off[b] - off[a] >= ceil(capacity / nseg). One thread,
+// one binary search per cut. Cuts past the last vertex are nv.
+__global__ void k_segment_cuts(uint32_t nv, const uint64_t *off, uint32_t nseg, uint32_t *seg_v,
+                               unsigned long long *seg_slot) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint64_t cap = off[nv];
+    const uint64_t per = (cap + nseg - 1) / nseg;
+    uint32_t a = 0;
+    for (uint32_t s = 0; s < nseg; ++s) {
+        seg_v[s] = a;
+        seg_slot[s] = off[a];
+        if (a < nv) {
+            uint32_t lo = a + 1, hi = nv;  // first b in [a + 1, nv] with off[b] - off[a] >= per (nv if none)
+            while (lo < hi) {
+                const uint32_t m = lo + (hi - lo) / 2;
+                if (off[m] - off[a] >= per) hi = m; else lo = m + 1;
+            }
+            a = lo;
+        }
+    }
+    seg_v[nseg] = nv;
+    seg_slot[nseg] = cap;
+}
+// Path position v + dpre[v] of every cut (merge-path tiles of the sweep).
+__global__ void k_segment_paths(uint32_t nseg, const uint32_t *seg_v, const unsigned long long *dpre,
+                                unsigned long long *seg_path) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s <= nseg; s += gridDim.x * blockDim.x)
+        seg_path[s] = (unsigned long long)seg_v[s] + dpre[seg_v[s]];
+}
+
+// HBM edge cache (compute_hot_vertices / LoadCache, framework.cuh:149-279):
+// rank vertices by the activations of the last 4 snapshots, cache the longest
+// prefix of the ranking whose lists fit, copy each list from the old cache
+// when it is there and unchanged, else from the edge array.
+__global__ void k_cache_keys(uint32_t nv, const uint32_t *deg, const uint8_t *h0, const uint8_t *h1, const uint8_t *h2,
+                             const uint8_t *h3, unsigned long long *keys) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t score = (uint32_t)h0[v] + h1[v] + h2[v] + h3[v];
+        keys[v] = (score && deg[v]) ? ((unsigned long long)(1023u - score) << 32) | v : ~0ull;
+    }
+}
+struct CacheDegAt {
+    const unsigned long long *keys;
+    const uint32_t *deg;
+    uint32_t n;
+    __host__ __device__ unsigned long long operator()(uint32_t i) const {
+        if (i >= n) return 0ull;
+        const unsigned long long k = keys[i];
+        return k == ~0ull ? 0ull : deg[(uint32_t)k];
+    }
+};
+__global__ void k_cache_assign(uint32_t nv, const unsigned long long *keys, const unsigned long long *pos,
+                               const uint32_t *deg, uint64_t slots, uint64_t *coff_new, unsigned long long *count) {
+    unsigned long long c = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long k = keys[i];
+        if (k == ~0ull) continue;
+        const uint32_t v = (uint32_t)k;
+        if (pos[i] + deg[v] <= slots) {
+            coff_new[v] = pos[i];
+            ++c;
+        }
+    }
+    if (c) atomicAdd(count, c);
+}
+// One wave per ranked vertex: fill its slot range of the new cache.
+__global__ void __launch_bounds__(256) k_cache_fill(uint32_t nv, const unsigned long long *keys, const uint64_t *coff_new,
+                                                    const uint64_t *coff_old, const uint32_t *ccol_old, const uint32_t *col,
+                                                    const uint64_t *off, const uint32_t *deg, uint32_t *ccol_new,
+                                                    unsigned long long *fetched) {
+    const uint64_t gw = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
+    unsigned long long f = 0;
+    for (uint64_t i = gw; i < nv; i += nw) {
+        const unsigned long long k = keys[i];
+        if (k == ~0ull) break;  // keys are sorted: the invalid ones come last
+        const uint32_t v = (uint32_t)k;
+        const uint64_t dst = coff_new[v];
+        if (dst == ~0ull) continue;
+        const uint32_t d = deg[v];
+        const uint64_t old = coff_old[v];
+        const uint32_t *src = old != ~0ull ? ccol_old + old : col + off[v];
+        if (old == ~0ull) f += d;
+        for (uint32_t e = lane_id(); e < d; e += kWave) ccol_new[dst + e] = src[e];
+    }
+    if (lane_id() == 0 && f) atomicAdd(fetched, f);
+}
+// Lists that the batch changed leave the cache.
+__global__ void k_cache_drop(const TInfo *tinfo, const uint32_t *tcount, uint64_t *coff) {
+    const uint32_t n = *tcount;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x)
+        coff[tinfo[t].u] = ~0ull;
+}
+
 // Split (u, v) pairs into sort keys; drop ids >= nv.
 __global__ void k_split(uint32_t nv, const uint32_t *ins_uv, uint32_t nins, const uint32_t *del_uv, uint32_t ndel,
                         uint32_t *ins_k, uint32_t *ins_v, unsigned long long *del_k, Ctl *ctl) {
@@ -464,6 +561,19 @@ __global__ void k_extract(uint32_t nv, const typename Op::buffer_t *buffer, uint
         out[v] = hi ? (uint32_t)(buffer[v] >> 32) : (uint32_t)buffer[v];
 }
 
+// Pointwise gather: out[i] = 32-bit word base[verts[i] * stride + word].
+__global__ void k_gather_words(const uint32_t *base, uint64_t stride, uint32_t word, uint32_t nv, const uint32_t *verts,
+                               uint64_t n, uint32_t *out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = verts[i];
+        out[i] = v < nv ? base[(uint64_t)v * stride + word] : 0xFFFFFFFFu;
+    }
+}
+__global__ void k_iota(uint32_t *p, uint32_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+
 // ---------------------------------------------------------------------------
 // engine
 // ---------------------------------------------------------------------------
@@ -528,11 +638,54 @@ public:
         p = dalloc<T>(nc);
         cap = nc;
     }
+    // Pinned host memory, mapped into the device address space (the edge array
+    // of the out-of-memory path and its load staging).
+    uint64_t host_bytes_ = 0;
+    std::vector<std::pair<void *, size_t>> hallocs_;
+    template <class T>
+    T *halloc(size_t count) {
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        void *p = nullptr;
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocMapped);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            fail(SG_ENOMEM, std::string("hipHostMalloc of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+        }
+        void *dp = nullptr;
+        SG_HIP(hipHostGetDevicePointer(&dp, p, 0));
+        if (dp != p) {
+            (void)hipHostFree(p);
+            fail(SG_EDEVICE, "pinned host memory is not addressable by the device at its host address");
+        }
+        host_bytes_ += bytes;
+        hallocs_.push_back({p, bytes});
+        return (T *)p;
+    }
+    void hfree(void *p) {
+        if (!p) return;
+        for (size_t i = 0; i < hallocs_.size(); ++i) {
+            if (hallocs_[i].first == p) {
+                host_bytes_ -= hallocs_[i].second;
+                hallocs_[i] = hallocs_.back();
+                hallocs_.pop_back();
+                (void)hipHostFree(p);
+                return;
+            }
+        }
+    }
+
     void release() {
         if (stream_) (void)hipStreamSynchronize(stream_);
+        if (cstream_) (void)hipStreamSynchronize(cstream_);
         for (auto &a : allocs_) (void)hipFree(a.first);
         allocs_.clear();
         dev_bytes_ = 0;
+        for (auto &a : hallocs_) (void)hipHostFree(a.first);
+        hallocs_.clear();
+        host_bytes_ = 0;
+        for (auto &ev : seg_ev_) if (ev) (void)hipEventDestroy(ev);
+        if (cstream_) (void)hipStreamDestroy(cstream_);
+        cstream_ = nullptr;
         for (auto &ev : events_) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
         events_.clear();
         if (pinned_) (void)hipHostFree(pinned_);
@@ -565,7 +718,7 @@ public:
     uint32_t *cnt_ = nullptr;    // [3] rotating round counters
     uint32_t *hcnt_ = nullptr;   // [3] rotating spill counters
     uint32_t *rlog_ = nullptr;
-    uint64_t timed_edges_ = 0, timed_verts_ = 0;  // host totals of the timed-round stats
+    uint64_t timed_edges_ = 0, timed_verts_ = 0, timed_cedges_ = 0;  // host totals of the timed-round stats
     HeavyDesc *heavy_[3] = {nullptr, nullptr, nullptr};  // rotating spill queues
     size_t heavy_cap_ = 0;
     unsigned long long *stats_ = nullptr;  // [0..3] pass stats, [4] inval count
@@ -611,6 +764,26 @@ public:
     size_t wsz_cap_ = 0, wvoff_cap_ = 0, ndarr_cap_ = 0, Parr_cap_ = 0;
     BatchView bv_{};
     uint32_t nins_ = 0, ndel_ = 0;
+
+    // --- out-of-memory path ------------------------------------------------------
+    bool host_edges_ = false;                 // col_ is pinned host memory
+    uint32_t nseg_ = 0;                       // SEGMENT count
+    uint32_t *seg_v_ = nullptr;               // [nseg + 1] first vertex of each segment
+    unsigned long long *seg_aux_ = nullptr;   // [2 * (nseg + 1)]: slot and path position of each cut
+    std::vector<uint32_t> h_seg_v_;
+    std::vector<unsigned long long> h_seg_slot_, h_seg_path_;
+    bool seg_dirty_ = true;
+    hipStream_t cstream_ = nullptr;           // copy stream of the streamed segments
+    hipEvent_t seg_ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // copied[2], used[2], start
+    uint32_t *stage_[2] = {nullptr, nullptr};
+    size_t stage_cap_[2] = {0, 0};
+    // HBM edge cache (host edges only)
+    uint64_t cache_slots_ = 0;
+    uint32_t *ccol_[2] = {nullptr, nullptr};
+    uint64_t *coff_ = nullptr, *coff2_ = nullptr;
+    uint8_t *hotw_[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t hot_cur_ = 0, ccur_ = 0;
+    unsigned long long *ckeys_ = nullptr, *ckeys2_ = nullptr, *cpos2_ = nullptr;
 
     // --- pinned host scratch --------------------------------------------------
     void *pinned_ = nullptr;
@@ -681,7 +854,29 @@ public:
 
         off_ = dalloc<uint64_t>((size_t)nv + 1);
         deg_ = dalloc<uint32_t>(nv);
-        col_ = dalloc<uint32_t>(cap_);
+        host_edges_ = (cfg_.flags & SG_FLAG_HOST_EDGES) ||
+                      (cfg_.edge_budget_gb > 0 && (double)cap_ * 4.0 > cfg_.edge_budget_gb * 1073741824.0);
+        col_ = host_edges_ ? halloc<uint32_t>(cap_) : dalloc<uint32_t>(cap_);
+        nseg_ = std::max<uint32_t>(1, std::min<uint32_t>(cfg_.segments ? cfg_.segments : 1, 65536));
+        seg_v_ = dalloc<uint32_t>(nseg_ + 1);
+        seg_aux_ = dalloc<unsigned long long>(2 * ((size_t)nseg_ + 1));
+        if (host_edges_) {
+            static const uint64_t kCacheSlots[5] = {0ull, 268435455ull, 536870921ull, 805306365ull, 1ull << 30};
+            // no larger than twice the edge array: the rest could never be filled
+            cache_slots_ = std::min<uint64_t>(kCacheSlots[std::min<uint32_t>(cfg_.cache_level, 4)], 2 * cap_);
+            SG_HIP(hipStreamCreateWithFlags(&cstream_, hipStreamNonBlocking));
+            for (auto &ev : seg_ev_) SG_HIP(hipEventCreate(&ev));
+        }
+        if (cache_slots_) {
+            ccol_[0] = dalloc<uint32_t>(cache_slots_);
+            ccol_[1] = dalloc<uint32_t>(cache_slots_);
+            coff_ = dalloc<uint64_t>(nv);
+            coff2_ = dalloc<uint64_t>(nv);
+            for (auto &h : hotw_) h = dalloc<uint8_t>(nv);
+            ckeys_ = dalloc<unsigned long long>(nv);
+            ckeys2_ = dalloc<unsigned long long>(nv);
+            cpos2_ = dalloc<unsigned long long>((size_t)nv + 1);
+        }
         leaf_cnt_ = dalloc<unsigned long long>(nleaves_ + 1);
         leaf_pre_ = dalloc<unsigned long long>(nleaves_ + 1);
         leaf_dirty_ = dalloc<uint8_t>(nleaves_);
@@ -710,15 +905,25 @@ public:
         ensure_heavy(ne);
 
         uint64_t *d_row = dalloc<uint64_t>((size_t)nv + 1);
-        ensure(tmp_, tmp_cap_, std::max<uint64_t>(ne, 1));
         SG_HIP(hipMemcpyAsync(d_row, row, ((size_t)nv + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
-        if (ne) SG_HIP(hipMemcpyAsync(tmp_, col, ne * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+        const uint32_t *csr_src = nullptr;
+        uint32_t *csr_pinned = nullptr;
+        if (host_edges_) {
+            // the CSR is staged in pinned host memory and laid out over the host link
+            csr_pinned = halloc<uint32_t>(std::max<uint64_t>(ne, 1));
+            if (ne) std::memcpy(csr_pinned, col, ne * sizeof(uint32_t));
+            csr_src = csr_pinned;
+        } else {
+            ensure(tmp_, tmp_cap_, std::max<uint64_t>(ne, 1));
+            if (ne) SG_HIP(hipMemcpyAsync(tmp_, col, ne * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+            csr_src = tmp_;
+        }
         fill<uint32_t>(col_, cap_, kGap);
         hipLaunchKernelGGL(k_layout_offsets, dim3(grid_for((uint64_t)nv + 1, 256, 16384)), dim3(256), 0, stream_, nv, d_row,
                            cap_, off_, deg_);
         SG_LAUNCHED();
         hipLaunchKernelGGL(k_layout_copy, dim3(grid_for(nv, kWavesPerBlock, 16384)), dim3(kBlock), 0, stream_, nv, d_row,
-                           tmp_, off_, col_);
+                           csr_src, off_, col_);
         SG_LAUNCHED();
         hipLaunchKernelGGL(k_leaf_init, dim3(grid_for(nleaves_ + 1, 256, 16384)), dim3(256), 0, stream_, nv, L_, nleaves_,
                            d_row, leaf_cnt_);
@@ -734,12 +939,20 @@ public:
         fill<uint32_t>(stamp_, nv, 0);
         fill<uint32_t>(cnt_, kCtlBlockBytes / 4, 0u);
         fill<uint8_t>((uint8_t *)ctl_, sizeof(Ctl), 0);
+        if (cache_slots_) {
+            fill<uint64_t>(coff_, nv, ~0ull);
+            for (auto &h : hotw_) fill<uint8_t>(h, nv, 0);
+        }
         sync();
         dfree(d_row);
+        hfree(csr_pinned);
+        seg_dirty_ = true;
         loaded_ = true;
         st_.nv = nv_;
         st_.ne = ne_;
         st_.capacity = cap_;
+        st_.host_edges = host_edges_ ? 1 : 0;
+        st_.cache_slots = cache_slots_;
     }
 
     void ensure_heavy(uint64_t edges) {
@@ -760,7 +973,15 @@ public:
         }
     }
 
-    GraphView gview() const { return GraphView{nv_, off_, deg_, col_}; }
+    GraphView gview() const {
+        GraphView g{nv_, off_, deg_, col_, nullptr, nullptr, cap_, nullptr};
+        if (cache_slots_) {
+            g.ccol = ccol_[ccur_];
+            g.coff = coff_;
+            g.hot = hotw_[hot_cur_];
+        }
+        return g;
+    }
     uint32_t budget() const { return cfg_.wave_budget ? cfg_.wave_budget : kWaveBudget; }
 
     // dpre_[v] = sum of deg[0..v) for v in [0, nv] (64-bit) and the merge-path
@@ -906,7 +1127,7 @@ public:
         uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(cfg_.rounds_per_sync ? cfg_.rounds_per_sync : 4, kRoundLog));
         const uint32_t maxr = cfg_.max_rounds ? cfg_.max_rounds : 1000000u;
         const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
-        if (timed) fill<unsigned long long>(stats_ + 8, 4, 0ull);
+        if (timed) fill<unsigned long long>(stats_ + 8, 8, 0ull);
         uint32_t rounds = 0;
         for (;;) {
             const uint32_t R0 = R_;
@@ -953,8 +1174,10 @@ public:
             timed_verts_ += s[9];
             if (cfg_.flags & SG_FLAG_KERNEL_TIMING) st_.relax_bytes += 12ull * (s[8] + s[10]) + 32ull * s[9];
         }
+        if (timed) timed_cedges_ += s[13];
         st_.edges_relaxed = s[0] + s[2] + timed_edges_;
         st_.vertices_processed = s[1] + timed_verts_;
+        st_.edges_from_cache = s[5] + timed_cedges_;
     }
 
     // ---- batch prepare / commit -------------------------------------------------------
@@ -1062,6 +1285,10 @@ public:
             hipLaunchKernelGGL(k_set_deg, dim3(1024), dim3(256), 0, stream_, tinfo_, &ctl_->tcount, newdeg_, deg_, maximal_,
                                nleaves_, L_, overflow ? 1 : 0);
             SG_LAUNCHED();
+            if (cache_slots_) {
+                hipLaunchKernelGGL(k_cache_drop, dim3(1024), dim3(256), 0, stream_, tinfo_, &ctl_->tcount, coff_);
+                SG_LAUNCHED();
+            }
             if (overflow) fill<uint8_t>(maximal_, 2 * nleaves_, 0);
         }
         ne_ = ne_new;
@@ -1072,6 +1299,7 @@ public:
     // The batch's per-vertex ranges and marks are kept until the snapshot's
     // recompute has used them; then they are cleared for the next batch.
     void finish_batch() {
+        cache_rebuild();
         if (nins_ + ndel_ == 0) return;
         hipLaunchKernelGGL(k_clear_ranges, dim3(1024), dim3(256), 0, stream_, tinfo_, &ctl_->tcount, br_);
         SG_LAUNCHED();
@@ -1150,7 +1378,9 @@ public:
                            tidx_, tinfo_, cpos_, tmp_, tmp2_);
         SG_LAUNCHED();
         uint32_t *dst = col_;
-        if (resize) dst = dalloc<uint32_t>(new_cap);
+        if (resize) {
+            dst = host_edges_ ? halloc<uint32_t>(new_cap) : dalloc<uint32_t>(new_cap);
+        }
         hipLaunchKernelGGL(k_win_scatter, dim3(gw), dim3(kBlock), 0, stream_, win_, nwin, wvoff_, Wv, P_arr_, tmp2_, dst, off_,
                            deg_);
         SG_LAUNCHED();
@@ -1158,10 +1388,126 @@ public:
             hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, stream_, off_ + nv_, new_cap);
             SG_LAUNCHED();
             sync();
-            dfree(col_);
+            if (host_edges_) hfree(col_);
+            else dfree(col_);
             col_ = dst;
             cap_ = new_cap;
         }
+        seg_dirty_ = true;
+    }
+
+    // ---- out-of-memory path: segments, streamed sweep, edge cache -----------------------
+    void ensure_segments() {
+        if (!seg_dirty_) return;
+        hipLaunchKernelGGL(k_segment_cuts, dim3(1), dim3(1), 0, stream_, nv_, off_, nseg_, seg_v_, seg_aux_);
+        SG_LAUNCHED();
+        h_seg_v_.resize((size_t)nseg_ + 1);
+        h_seg_slot_.resize((size_t)nseg_ + 1);
+        SG_HIP(hipMemcpyAsync(h_seg_v_.data(), seg_v_, ((size_t)nseg_ + 1) * 4, hipMemcpyDeviceToHost, stream_));
+        SG_HIP(hipMemcpyAsync(h_seg_slot_.data(), seg_aux_, ((size_t)nseg_ + 1) * 8, hipMemcpyDeviceToHost, stream_));
+        sync();
+        seg_dirty_ = false;
+    }
+
+    // Edge-centric sweep with the edge array in pinned host memory: segment by
+    // segment, the slot range is copied to one of two HBM staging buffers on
+    // the copy stream while the sweep kernel walks the previous segment
+    // (framework.cuh:310-384 SEGMENT, n_stream streams). launch(colp, slot_lo,
+    // v_lo, v_hi, t_lo, t_hi) enqueues the sweep of one segment on stream_.
+    template <class Launch>
+    void streamed_sweep(Launch launch) {
+        ensure_segments();
+        hipLaunchKernelGGL(k_segment_paths, dim3(grid_for((uint64_t)nseg_ + 1, 256, 64)), dim3(256), 0, stream_, nseg_,
+                           seg_v_, dpre_, seg_aux_ + nseg_ + 1);
+        SG_LAUNCHED();
+        h_seg_path_.resize((size_t)nseg_ + 1);
+        SG_HIP(hipMemcpyAsync(h_seg_path_.data(), seg_aux_ + nseg_ + 1, ((size_t)nseg_ + 1) * 8, hipMemcpyDeviceToHost,
+                              stream_));
+        sync();
+        size_t span = 0;
+        for (uint32_t k = 0; k < nseg_; ++k) span = std::max<size_t>(span, h_seg_slot_[k + 1] - h_seg_slot_[k]);
+        for (int b = 0; b < 2; ++b) ensure(stage_[b], stage_cap_[b], std::max<size_t>(span, 1));
+        auto t0 = std::chrono::steady_clock::now();
+        uint64_t bytes = 0;
+        for (uint32_t k = 0; k < nseg_; ++k) {
+            const uint32_t a = h_seg_v_[k], b2 = h_seg_v_[k + 1];
+            if (a >= b2) continue;
+            const int b = k & 1;
+            const uint64_t slo = h_seg_slot_[k], shi = h_seg_slot_[k + 1];
+            if (k >= 2) SG_HIP(hipStreamWaitEvent(cstream_, seg_ev_[2 + b], 0));  // segment k - 2 released buffer b
+            if (shi > slo)
+                SG_HIP(hipMemcpyAsync(stage_[b], col_ + slo, (shi - slo) * 4, hipMemcpyHostToDevice, cstream_));
+            SG_HIP(hipEventRecord(seg_ev_[b], cstream_));
+            SG_HIP(hipStreamWaitEvent(stream_, seg_ev_[b], 0));
+            const uint64_t tlo = h_seg_path_[k] / kSweepTile;
+            const uint64_t thi = (h_seg_path_[k + 1] + kSweepTile - 1) / kSweepTile;
+            launch((const uint32_t *)stage_[b], slo, a, b2, tlo, thi);
+            SG_HIP(hipEventRecord(seg_ev_[2 + b], stream_));
+            bytes += (shi - slo) * 4;
+        }
+        sync();
+        st_.streamed_bytes += bytes;
+        st_.stream_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    // Sweep over all live edges: resident, or streamed by segment.
+    template <class Op, class EFn>
+    void sweep_all(const SweepTiles &tl, Op op, const typename Op::buffer_t *buffer, EFn ef, const PassIO &io) {
+        const uint64_t ntiles = ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile;
+        if (!host_edges_) {
+            const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, 16384));
+            hipLaunchKernelGGL((k_sweep_edges<Op, EFn>), dim3(grid), dim3(kBlock), 0, stream_, gview(), dpre_, tl,
+                               (const uint32_t *)col_, (uint64_t)0, 0u, nv_, (uint64_t)0, ntiles, op, buffer, ef, io);
+            SG_LAUNCHED();
+            return;
+        }
+        streamed_sweep([&](const uint32_t *colp, uint64_t slo, uint32_t va, uint32_t vb, uint64_t tlo, uint64_t thi) {
+            const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(thi - tlo, 16384));
+            hipLaunchKernelGGL((k_sweep_edges<Op, EFn>), dim3(grid), dim3(kBlock), 0, stream_, gview(), dpre_, tl, colp, slo, va,
+                               vb, tlo, std::min<uint64_t>(thi, ntiles), op, buffer, ef, io);
+            SG_LAUNCHED();
+        });
+    }
+
+    // After a solve or a snapshot: rebuild the HBM edge cache from the last
+    // four snapshots' activations and open a new hotness window.
+    void cache_rebuild() {
+        if (!cache_slots_) return;
+        hipLaunchKernelGGL(k_cache_keys, dim3(grid_for(nv_, 256, 16384)), dim3(256), 0, stream_, nv_, deg_, hotw_[0], hotw_[1],
+                           hotw_[2], hotw_[3], ckeys_);
+        SG_LAUNCHED();
+        size_t b = 0;
+        SG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, b, ckeys_, ckeys2_, (int)nv_, 0, 64, stream_));
+        hipcub::CountingInputIterator<uint32_t> cnt(0);
+        hipcub::TransformInputIterator<unsigned long long, CacheDegAt, hipcub::CountingInputIterator<uint32_t>> in(
+            cnt, CacheDegAt{ckeys2_, deg_, nv_});
+        size_t b2 = 0;
+        SG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, in, cpos2_, (int)nv_ + 1, stream_));
+        ensure(cub_tmp_, cub_cap_, std::max(b, b2) + 256);
+        b = cub_cap_;
+        SG_HIP(hipcub::DeviceRadixSort::SortKeys(cub_tmp_, b, ckeys_, ckeys2_, (int)nv_, 0, 64, stream_));
+        b2 = cub_cap_;
+        SG_HIP(hipcub::DeviceScan::ExclusiveSum(cub_tmp_, b2, in, cpos2_, (int)nv_ + 1, stream_));
+        fill<uint64_t>(coff2_, nv_, ~0ull);
+        fill<unsigned long long>(stats_ + 6, 2, 0ull);
+        hipLaunchKernelGGL(k_cache_assign, dim3(grid_for(nv_, 256, 16384)), dim3(256), 0, stream_, nv_,
+                           (const unsigned long long *)ckeys2_, (const unsigned long long *)cpos2_, deg_, cache_slots_, coff2_,
+                           stats_ + 6);
+        SG_LAUNCHED();
+        hipLaunchKernelGGL(k_cache_fill, dim3(grid_for(nv_, 4, 16384)), dim3(256), 0, stream_, nv_,
+                           (const unsigned long long *)ckeys2_, coff2_, coff_, ccol_[ccur_], col_, off_, deg_,
+                           ccol_[ccur_ ^ 1], stats_ + 7);
+        SG_LAUNCHED();
+        std::swap(coff_, coff2_);
+        ccur_ ^= 1;
+        hot_cur_ = (hot_cur_ + 1) % 4;
+        fill<uint8_t>(hotw_[hot_cur_], nv_, 0);
+        unsigned long long h[2];
+        SG_HIP(hipMemcpyAsync(h_u64_, stats_ + 6, 16, hipMemcpyDeviceToHost, stream_));
+        sync();
+        std::memcpy(h, h_u64_, 16);
+        st_.cached_vertices = h[0];
+        st_.streamed_bytes += 4 * h[1];  // lists fetched into the cache over the host link
     }
 
     // ---- interface ------------------------------------------------------------------------
@@ -1174,6 +1520,33 @@ public:
     virtual void gather_query(uint32_t, uint32_t, void *, size_t) {
         fail(SG_EINVAL, "sg_gather_query needs an engine created with SG_ALGO_MULTI");
     }
+    // Where the 32-bit result word of (query, qi) for vertex v lives:
+    // base[v * stride + word].
+    virtual void word_source(uint32_t q, uint32_t qi, const uint32_t *&base, uint64_t &stride, uint32_t &word) {
+        (void)qi;
+        if (q == SG_Q_DEGREE) { base = deg_; stride = 1; word = 0; return; }
+        fail(SG_EINVAL, "query not available pointwise");
+    }
+    virtual void topk(uint32_t, uint32_t *, float *) { fail(SG_EINVAL, "sg_pr_topk needs a PageRank engine"); }
+
+    uint32_t *gat_v_ = nullptr, *gat_o_ = nullptr;
+    size_t gat_v_cap_ = 0, gat_o_cap_ = 0;
+    void gather_at(uint32_t q, uint32_t qi, const uint32_t *verts, uint64_t n, uint32_t *out) {
+        if (!inited_ && q != SG_Q_DEGREE) fail(SG_EINVAL, "no vertex state before sg_init");
+        if (n == 0) return;
+        const uint32_t *base = nullptr;
+        uint64_t stride = 1;
+        uint32_t word = 0;
+        word_source(q, qi, base, stride, word);
+        ensure(gat_v_, gat_v_cap_, n);
+        ensure(gat_o_, gat_o_cap_, n);
+        SG_HIP(hipMemcpyAsync(gat_v_, verts, n * 4, hipMemcpyHostToDevice, stream_));
+        hipLaunchKernelGGL(k_gather_words, dim3(grid_for(n, 256, 4096)), dim3(256), 0, stream_, base, stride, word, nv_, gat_v_,
+                           n, gat_o_);
+        SG_LAUNCHED();
+        SG_HIP(hipMemcpyAsync(out, gat_o_, n * 4, hipMemcpyDeviceToHost, stream_));
+        sync();
+    }
 
     void gather_graph(uint32_t q, void *out, size_t bytes) {
         if (q == SG_Q_DEGREE) {
@@ -1184,7 +1557,11 @@ public:
             SG_HIP(hipMemcpyAsync(out, off_, ((size_t)nv_ + 1) * 8, hipMemcpyDeviceToHost, stream_));
         } else if (q == SG_Q_SLOTS) {
             if (bytes < (size_t)cap_ * 4) fail(SG_EINVAL, "output too small");
-            SG_HIP(hipMemcpyAsync(out, col_, (size_t)cap_ * 4, hipMemcpyDeviceToHost, stream_));
+            SG_HIP(hipMemcpyAsync(out, col_, (size_t)cap_ * 4, hipMemcpyDefault, stream_));
+        } else if (q == SG_Q_SEGMENTS) {
+            if (bytes < ((size_t)nseg_ + 1) * 4) fail(SG_EINVAL, "output too small");
+            ensure_segments();
+            std::memcpy(out, h_seg_v_.data(), ((size_t)nseg_ + 1) * 4);
         } else {
             fail(SG_EINVAL, "unknown query " + std::to_string(q));
         }
@@ -1230,6 +1607,7 @@ public:
                            buffer_, stamp_, R_, q_[R_ & 1], cnt_ + (R_ % 3), stats_ + 3);
         SG_LAUNCHED();
         st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, false);
+        cache_rebuild();
         inited_ = true;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -1265,11 +1643,7 @@ public:
             const SweepTiles tl = sweep_tiles();
             PassIO io = seed_io(R, nullptr, nullptr, nv_);
             ERelaxInval<Op> ef{ERelax<Op>{op_, buffer_, stamp_, R, cfg_.weight_mode, cfg_.weight_seed}, inval_, coarse_};
-            const uint64_t ntiles = ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile;
-            const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, 16384));
-            hipLaunchKernelGGL((k_sweep_edges<Op, ERelaxInval<Op>>), dim3(grid), dim3(kBlock), 0, stream_, gview(), dpre_, tl,
-                               op_, buffer_, ef, io);
-            SG_LAUNCHED();
+            sweep_all(tl, op_, (const typename Op::buffer_t *)buffer_, ef, io);
             // invalidated vertices active on their own (CC: own label)
             hipLaunchKernelGGL(k_inval_activate<Op>, dim3(grid_for((nv_ + 31) / 32, 256, 4096)), dim3(256), 0, stream_, nv_,
                                op_, inval_, value_, buffer_, stamp_, R, q_[R & 1], cnt_ + (R % 3), nv_, stats_ + 3);
@@ -1286,6 +1660,13 @@ public:
         finish_batch();
         st_.snapshots += 1;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    void word_source(uint32_t q, uint32_t qi, const uint32_t *&base, uint64_t &stride, uint32_t &word) override {
+        if (q == SG_Q_VALUE) { base = value_; stride = 1; word = 0; return; }
+        if (q == SG_Q_PARENT) { base = (const uint32_t *)buffer_; stride = 2; word = 0; return; }  // low word
+        if (q == SG_Q_BUFFER) { base = (const uint32_t *)buffer_; stride = 2; word = 1; return; }  // high word
+        Engine::word_source(q, qi, base, stride, word);
     }
 
     void gather(uint32_t q, void *out, size_t bytes) override {
@@ -1355,6 +1736,7 @@ public:
         SG_LAUNCHED();
         st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, false);
         residual();
+        cache_rebuild();
         inited_ = true;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -1387,6 +1769,38 @@ public:
         st_.snapshots += 1;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         residual();
+    }
+
+    void word_source(uint32_t q, uint32_t qi, const uint32_t *&base, uint64_t &stride, uint32_t &word) override {
+        if (q == SG_Q_VALUE) { base = (const uint32_t *)value_; stride = 1; word = 0; return; }
+        if (q == SG_Q_BUFFER) { base = (const uint32_t *)buffer_; stride = 1; word = 0; return; }
+        Engine::word_source(q, qi, base, stride, word);
+    }
+
+    // The k highest ranks (ties by smaller id), by a descending radix sort of
+    // (rank, id) on the device.
+    float *tk_keys_ = nullptr;
+    uint32_t *tk_ids_ = nullptr, *tk_ids2_ = nullptr;
+    size_t tk_cap_[3] = {0, 0, 0};
+    void topk(uint32_t k, uint32_t *ids, float *ranks) override {
+        if (!inited_) fail(SG_EINVAL, "no ranks before sg_init");
+        k = std::min<uint32_t>(k, nv_);
+        if (k == 0) return;
+        ensure(tk_keys_, tk_cap_[0], nv_);
+        ensure(tk_ids_, tk_cap_[1], nv_);
+        ensure(tk_ids2_, tk_cap_[2], nv_);
+        hipLaunchKernelGGL(k_iota, dim3(grid_for(nv_, 256, 16384)), dim3(256), 0, stream_, tk_ids_, nv_);
+        SG_LAUNCHED();
+        size_t b = 0;
+        SG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b, value_, tk_keys_, tk_ids_, tk_ids2_, (int)nv_, 0, 32,
+                                                            stream_));
+        ensure(cub_tmp_, cub_cap_, b + 256);
+        b = cub_cap_;
+        SG_HIP(hipcub::DeviceRadixSort::SortPairsDescending(cub_tmp_, b, value_, tk_keys_, tk_ids_, tk_ids2_, (int)nv_, 0, 32,
+                                                            stream_));
+        SG_HIP(hipMemcpyAsync(ids, tk_ids2_, (size_t)k * 4, hipMemcpyDeviceToHost, stream_));
+        if (ranks) SG_HIP(hipMemcpyAsync(ranks, tk_keys_, (size_t)k * 4, hipMemcpyDeviceToHost, stream_));
+        sync();
     }
 
     void gather(uint32_t q, void *out, size_t bytes) override {
@@ -1498,15 +1912,18 @@ public:
         while ((1u << qs_log_) < QS_) ++qs_log_;
         G_ = (nq + QS_ - 1) / QS_;
         KP_ = G_ * QS_;
-        if (state_nv_ != nv_ || state_kp_ != KP_) {
+        // state for KP slots; the allocation only grows (a request loop changes
+        // the query count every step), a smaller KP uses a prefix of it
+        if (state_nv_ != nv_ || state_kp_ < KP_) {
             dfree(best_); dfree(act_); dfree(minv_); dfree(ninv_); dfree(kind_); dfree(src_);
             best_ = act_ = minv_ = ninv_ = nullptr;
             kind_ = nullptr;
             src_ = nullptr;
+            const uint32_t G = (KP_ + 63) / 64 > G_ ? (KP_ + 63) / 64 : G_;
             best_ = dalloc<unsigned long long>((size_t)nv_ * KP_);
-            act_ = dalloc<unsigned long long>((size_t)nv_ * G_);
-            minv_ = dalloc<unsigned long long>((size_t)nv_ * G_);
-            ninv_ = dalloc<unsigned long long>((size_t)nv_ * G_);
+            act_ = dalloc<unsigned long long>((size_t)nv_ * G);
+            minv_ = dalloc<unsigned long long>((size_t)nv_ * G);
+            ninv_ = dalloc<unsigned long long>((size_t)nv_ * G);
             kind_ = dalloc<uint8_t>(KP_);
             src_ = dalloc<uint32_t>(KP_);
             state_nv_ = nv_;
@@ -1531,6 +1948,7 @@ public:
         SG_LAUNCHED();
         st_.rounds = mrounds<0>(false);
         sync();  // host vectors above were read by the async copies
+        cache_rebuild();
         inited_ = true;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -1593,6 +2011,17 @@ public:
         sync();
     }
 
+    void word_source(uint32_t q, uint32_t qi, const uint32_t *&base, uint64_t &stride, uint32_t &word) override {
+        if (q == SG_Q_VALUE || q == SG_Q_PARENT) {
+            if (qi >= nq_) fail(SG_EINVAL, "query index out of range");
+            base = (const uint32_t *)best_;
+            stride = 2ull * KP_;
+            word = 2 * qi + (q == SG_Q_VALUE ? 1 : 0);
+            return;
+        }
+        Engine::word_source(q, qi, base, stride, word);
+    }
+
     void gather(uint32_t q, void *out, size_t bytes) override {
         if (q == SG_Q_VALUE || q == SG_Q_PARENT) return gather_query(0, q, out, bytes);
         if (q == SG_Q_BUFFER) fail(SG_EINVAL, "batched queries keep no separate buffer");
@@ -1613,6 +2042,9 @@ template <class F>
 static sg_status guarded(sg_engine *eng, F &&f) {
     if (!eng || !eng->impl) return SG_EINVAL;
     try {
+        // the calling thread's current device may differ from the one the
+        // engine was created on (a request loop calls from its worker thread)
+        if (eng->impl->stream_) SG_HIP(hipSetDevice(eng->impl->cfg_.device));
         f(*eng->impl);
         eng->impl->last_error_ = "no error";
         return SG_OK;
@@ -1647,6 +2079,8 @@ void sg_config_default(sg_config *c) {
     c->flags = 0;
     c->device_budget_gb = 0;
     c->device = 0;
+    c->cache_level = 4;     // app_skeleton.h:100 (-cache 4); used only with host edges
+    c->edge_budget_gb = 0;
 }
 
 sg_status sg_create(const sg_config *cfg, sg_engine **out) {
@@ -1732,10 +2166,40 @@ sg_status sg_gather(sg_engine *eng, uint32_t query, void *out, size_t bytes) {
     });
 }
 
+sg_status sg_gather_at(sg_engine *eng, uint32_t query, uint32_t qi, const uint32_t *vertices, uint64_t n, uint32_t *out) {
+    return guarded(eng, [&](sg::Engine &e) {
+        if (n && (!vertices || !out)) sg::fail(SG_EINVAL, "null vertex list or output");
+        if (!e.loaded_) sg::fail(SG_EINVAL, "no graph loaded");
+        e.gather_at(query, qi, vertices, n, out);
+    });
+}
+
+sg_status sg_device_memory(int32_t device, uint64_t *free_bytes, uint64_t *total_bytes) {
+    if (!free_bytes || !total_bytes) return SG_EINVAL;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return SG_EDEVICE;
+    if (hipSetDevice(device) != hipSuccess) { (void)hipGetLastError(); return SG_EDEVICE; }
+    size_t f = 0, t = 0;
+    const hipError_t e = hipMemGetInfo(&f, &t);
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess) { (void)hipGetLastError(); return SG_EDEVICE; }
+    *free_bytes = f;
+    *total_bytes = t;
+    return SG_OK;
+}
+
+sg_status sg_pr_topk(sg_engine *eng, uint32_t k, uint32_t *ids, float *ranks) {
+    return guarded(eng, [&](sg::Engine &e) {
+        if (k && !ids) sg::fail(SG_EINVAL, "null id output");
+        e.topk(k, ids, ranks);
+    });
+}
+
 sg_status sg_get_stats(sg_engine *eng, sg_stats *out) {
     return guarded(eng, [&](sg::Engine &e) {
         if (!out) sg::fail(SG_EINVAL, "null output");
         e.st_.device_bytes = e.dev_bytes_;
+        e.st_.host_bytes = e.host_bytes_;
         *out = e.st_;
     });
 }
@@ -1747,7 +2211,9 @@ sg_status sg_reset_stats(sg_engine *eng) {
         e.st_.relax_launches = 0;
         e.st_.relax_ms = 0;
         e.st_.relax_bytes = 0;
-        e.timed_edges_ = e.timed_verts_ = 0;
+        e.timed_edges_ = e.timed_verts_ = e.timed_cedges_ = 0;
+        e.st_.edges_from_cache = e.st_.streamed_bytes = 0;
+        e.st_.stream_ms = 0;
         if (e.stats_) {
             e.fill<unsigned long long>(e.stats_, 16, 0ull);
             e.sync();

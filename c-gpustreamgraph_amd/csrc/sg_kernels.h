@@ -44,8 +44,34 @@ struct GraphView {
     uint32_t nv;
     const uint64_t *off;  // [nv + 1], off[nv] = capacity
     const uint32_t *deg;  // [nv]
-    const uint32_t *col;  // [capacity], kGap in unused slots
+    const uint32_t *col;  // [capacity], kGap in unused slots (HBM, or pinned host memory)
+    // HBM edge cache (null when unused): copies of the lists of hot vertices.
+    // A list start s >= cap addresses ccol[s - cap].
+    const uint32_t *ccol;
+    const uint64_t *coff;  // [nv] start of v's copy in ccol, ~0 = not cached
+    uint64_t cap;
+    uint8_t *hot;          // [nv] activations this snapshot (null when unused)
 };
+
+// Where a relax walk reads u's list: its cached copy when there is one.
+__device__ __forceinline__ uint64_t list_start(const GraphView &g, uint32_t u) {
+    if (g.coff) {
+        const uint64_t c = g.coff[u];
+        if (c != ~0ull) return g.cap + c;
+    }
+    return g.off[u];
+}
+__device__ __forceinline__ uint32_t edge_at(const GraphView &g, uint64_t s) {
+    return s < g.cap ? g.col[s] : g.ccol[s - g.cap];
+}
+// Activation count of the cache's hotness window (non-atomic and saturating,
+// like the reference's hotness[0]++ in RelaxCTADB, sync_push_dd.cuh:196-230).
+__device__ __forceinline__ void count_hot(const GraphView &g, uint32_t u) {
+    if (g.hot) {
+        const uint8_t h = g.hot[u];
+        if (h < 255) g.hot[u] = h + 1;
+    }
+}
 
 // Where a pass reads its work items and appends the vertices it activates.
 struct PassIO {
@@ -161,7 +187,7 @@ __device__ __forceinline__ void stage_push(bool pred, uint32_t item, OutStage &s
 
 // Per-wave counters, reduced per block at the end of a kernel.
 struct WaveStats {
-    unsigned long long edges = 0, verts = 0, hedges = 0;
+    unsigned long long edges = 0, verts = 0, hedges = 0, cedges = 0;
 };
 
 // LDS of the light tier (per block).
@@ -171,7 +197,7 @@ struct LightShared {
     uint32_t src[kWavesPerBlock][kWave];
     uint32_t push[kWavesPerBlock][kWave];
     uint32_t out[kWavesPerBlock][kOutStage];
-    unsigned long long red[kWavesPerBlock][3];
+    unsigned long long red[kWavesPerBlock][4];
 };
 
 // Owner of edge k among the 64 staged lists: the largest j with excl[j] <= k
@@ -203,7 +229,7 @@ __device__ __forceinline__ void walk_staged(const GraphView &g, const PassIO &io
             const uint32_t o = staged_owner(s_excl, act[j] ? k : 0);
             src[j] = s_src[o];
             push[j] = s_push[o];
-            v[j] = act[j] ? g.col[s_start[o] + (k - s_excl[o])] : kGap;
+            v[j] = act[j] ? edge_at(g, s_start[o] + (k - s_excl[o])) : kGap;
         }
 #pragma unroll
         for (int j = 0; j < kIlp; ++j) peek[j] = (v[j] != kGap) ? efn.load(v[j]) : 0;
@@ -234,7 +260,11 @@ __device__ __forceinline__ void light_body(const GraphView &g, const PassIO &io,
             u = io.in_list ? io.in_list[idx] : io.range_lo + (uint32_t)idx;
             const uint32_t du = g.deg[u];
             sched = vfn(u, du, push, self);
-            if (sched && du) { d = du; start = g.off[u]; }
+            if (sched && du) {
+                d = du;
+                start = list_start(g, u);
+                count_hot(g, u);
+            }
         }
         stage_push(self, u, st, io);
         ws.verts += sched ? 1 : 0;
@@ -272,6 +302,7 @@ __device__ __forceinline__ void light_body(const GraphView &g, const PassIO &io,
         __builtin_amdgcn_wave_barrier();
         walk_staged(g, io, efn, total, sh.excl[w], sh.start[w], sh.src[w], sh.push[w], st);
         ws.edges += total;
+        if (g.coff) ws.cedges += wave_sum_u64(start >= g.cap ? d : 0u);
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -292,7 +323,7 @@ __device__ __forceinline__ void heavy_body(const GraphView &g, const PassIO &io,
 #pragma unroll
             for (int j = 0; j < kIlp; ++j) {
                 const uint32_t k = k0 + j * kWave + lane;
-                v[j] = k < hlen ? g.col[hstart + k] : kGap;
+                v[j] = k < hlen ? edge_at(g, hstart + k) : kGap;
             }
 #pragma unroll
             for (int j = 0; j < kIlp; ++j) peek[j] = v[j] != kGap ? efn.load(v[j]) : 0;
@@ -302,6 +333,7 @@ __device__ __forceinline__ void heavy_body(const GraphView &g, const PassIO &io,
             for (int j = 0; j < kIlp; ++j) stage_push(enq[j], v[j], st, io);
         }
         ws.hedges += hlen;
+        if (hstart >= g.cap) ws.cedges += hlen;
     }
 }
 
@@ -313,12 +345,14 @@ __device__ __forceinline__ void block_stats(const WaveStats &ws, LightShared &sh
         sh.red[w][0] = ws.edges;
         sh.red[w][1] = v;
         sh.red[w][2] = ws.hedges;
+        sh.red[w][3] = ws.cedges;  // lane 0 holds the wave's cached-edge total
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
+    if (threadIdx.x < 4) {
         unsigned long long t = 0;
         for (int i = 0; i < kWavesPerBlock; ++i) t += sh.red[i][threadIdx.x];
-        const int slot = threadIdx.x == 2 ? 2 : (int)threadIdx.x;
+        // [0] edges, [1] vertices, [2] spilled-chunk edges, [5] edges read from the cache
+        const int slot = threadIdx.x == 3 ? 5 : (int)threadIdx.x;
         if (t) atomicAdd(stats + slot, t);
     }
 }

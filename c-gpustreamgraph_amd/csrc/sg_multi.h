@@ -144,7 +144,7 @@ __device__ void mwalk(const GraphView &g, const MultiState &ms, const PassIO &io
 #pragma unroll
         for (int j = 0; j < kMUnroll; ++j) {
             const uint32_t e = e0 + j * q.per + q.sub;
-            vv[j] = e < len ? g.col[start + e] : kGap;
+            vv[j] = e < len ? edge_at(g, start + e) : kGap;
         }
         if (kMode == 0) {
             // all peeks in flight before the first atomic
@@ -212,8 +212,9 @@ __device__ void mlight(const GraphView &g, const MultiState &ms, const PassIO &i
             m = (uint32_t)atomicExch(&masks[u], 0ull);
             if (m) {
                 deg = g.deg[u];
-                start = g.off[u];
+                start = list_start(g, u);
                 len = min(deg, kMEdgeChunk);
+                count_hot(g, u);
             }
         }
         verts += (uint32_t)__popcll(__ballot(m != 0));
@@ -255,7 +256,7 @@ __device__ void mlight(const GraphView &g, const MultiState &ms, const PassIO &i
                 src[j] = sh.u[o];
                 on[j] = ok && ((sh.mask[o] >> q.slot) & 1u) && kind != kQNone;
                 dv[j] = kMode == 0 ? sh.du[o * ms.QS + q.slot] : 0u;
-                vv[j] = ok ? g.col[sh.start[o] + (k - sh.excl[o])] : kGap;
+                vv[j] = ok ? edge_at(g, sh.start[o] + (k - sh.excl[o])) : kGap;
             }
             if (kMode == 0) {
                 uint64_t pk[kMUnroll];
@@ -329,7 +330,8 @@ __global__ void __launch_bounds__(kBlock) k_mround(GraphView g, MultiState ms, P
         m = __shfl(m, 0, kWave);
         if (m == 0) continue;
         ++verts;
-        const uint64_t start = g.off[u];
+        const uint64_t start = list_start(g, u);
+        if (lane == 0) count_hot(g, u);
         const uint32_t deg = g.deg[u];
         const uint32_t first = min(deg, kMEdgeChunk);
         if (deg > first) {

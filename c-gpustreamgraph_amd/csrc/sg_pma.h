@@ -468,9 +468,14 @@ constexpr int kSweepPerThread = kSweepTile / kBlock;
 constexpr int kSweepGroup = 8;         // bitmap probes in flight per thread
 constexpr uint32_t kSweepOutStage = 128;
 
+// colp / slot_lo: where slot s of the tile's sources is read (colp[s - slot_lo]:
+// the edge array itself, or one streamed segment of it); only sources in
+// [src_lo, src_hi) and tiles in [t_begin, t_end) are walked.
 template <class Op, class EFn>
 __global__ void __launch_bounds__(kBlock) k_sweep_edges(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
-                                                        Op op, const typename Op::buffer_t *buffer, EFn efn, PassIO io) {
+                                                        const uint32_t *colp, uint64_t slot_lo, uint32_t src_lo,
+                                                        uint32_t src_hi, uint64_t t_begin, uint64_t t_end, Op op,
+                                                        const typename Op::buffer_t *buffer, EFn efn, PassIO io) {
     __shared__ uint16_t s_own[kSweepTile];
     __shared__ int32_t s_base[kSweepTile + 1];
     __shared__ uint32_t s_out[kWavesPerBlock][kSweepOutStage];
@@ -481,7 +486,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep_edges(GraphView g, const unsig
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
     OutStage st{s_out[w], 0, kSweepOutStage};
     unsigned long long n_edges = 0;
-    for (unsigned long long t = blockIdx.x; t < ntiles && t + 1 < tiles.cap; t += gridDim.x) {
+    const unsigned long long tend = min<unsigned long long>(t_end, ntiles);
+    for (unsigned long long t = t_begin + blockIdx.x; t < tend && t + 1 < tiles.cap; t += gridDim.x) {
         const uint32_t v0 = tiles.v[t], v1 = tiles.v[t + 1];
         const unsigned long long e0 = tiles.e[t], e1 = tiles.e[t + 1];
         const uint32_t ne = (uint32_t)min<unsigned long long>(e1 - e0, kSweepTile);
@@ -531,14 +537,17 @@ __global__ void __launch_bounds__(kBlock) k_sweep_edges(GraphView g, const unsig
             vv[j] = kGap;
             if (p < ne) {
                 const uint32_t o = s_own[p];
-                uint64_t slot;
-                if (!wide) {
-                    slot = off0 + (uint64_t)((long long)s_base[o] + (long long)p);
-                } else {
-                    const uint32_t v = v0 + o;
-                    slot = g.off[v] + (e0 + p - dpre[v]);
+                const uint32_t src = v0 + o;
+                if (src >= src_lo && src < src_hi) {
+                    ++n_edges;
+                    uint64_t slot;
+                    if (!wide) {
+                        slot = off0 + (uint64_t)((long long)s_base[o] + (long long)p);
+                    } else {
+                        slot = g.off[src] + (e0 + p - dpre[src]);
+                    }
+                    vv[j] = colp[slot - slot_lo];
                 }
-                vv[j] = g.col[slot];
             }
         }
 #pragma unroll
@@ -557,11 +566,11 @@ __global__ void __launch_bounds__(kBlock) k_sweep_edges(GraphView g, const unsig
                 stage_push(enq, v, st, io);
             }
         }
-        n_edges += ne;
         __syncthreads();
     }
     stage_flush(st, io);
-    if (threadIdx.x == 0 && n_edges) atomicAdd(io.stats + 0, n_edges);
+    n_edges = wave_sum_u64(n_edges);
+    if (lane_id() == 0 && n_edges) atomicAdd(io.stats + 0, n_edges);
 }
 
 // Invalidated vertices that are active on their own (CC: own label) join the

@@ -60,8 +60,8 @@ void define_flags() {
     def_s("format", "market_big", "graph format (market_big)");
     def_s("updatefile", "", "update file: 'a u v' / 'd u v' lines");
     def_s("update_size", "", "stream-size file: 'n_add n_del' per snapshot");
-    def_f("cache", 4, "HBM edge cache in GB (accepted; the graph is HBM-resident)");
-    def_i("SEGMENT", 1, "vertex segments");
+    def_i("cache", 4, "HBM edge cache level 0..4 (graph_datum.cuh:151-163); used when the edge array is in host memory");
+    def_i("SEGMENT", 1, "vertex segments streamed from host memory (framework.cuh:310-384)");
     def_i("n_stream", 3, "number of streams");
     def_i("block_size", 512, "block size for traversal kernels");
     def_s("lb_push", "", "load-balancing policy for push");
@@ -111,7 +111,7 @@ void define_flags() {
     def_i("weight_seed", 27491095, "seed of -weight_mode seeded");
     def_i("snapshots", -1, "apply at most this many snapshots (-1: all lines of -update_size)");
     def_s("dump_dir", "", "write the result after every snapshot to <dir>/snapshot_<t>.txt");
-    def_f("device_graph_budget_gb", 0, "cap on device memory (0: none)");
+    def_f("device_graph_budget_gb", 0, "device budget of the edge array in GiB; a larger edge array stays in pinned host memory (0: none)");
     def_f("pma_density", 0.25, "initial PMA edge density");
     def_i("device", 0, "HIP device");
 }
@@ -235,7 +235,8 @@ int main(int argc, char **argv) {
     cfg.segments = (uint32_t)std::max<long long>(1, I("SEGMENT"));
     cfg.n_stream = (uint32_t)std::max<long long>(1, I("n_stream"));
     cfg.block_size = (uint32_t)I("block_size");
-    cfg.device_budget_gb = F("device_graph_budget_gb");
+    cfg.edge_budget_gb = F("device_graph_budget_gb");  // an edge array above this stays in pinned host memory
+    cfg.cache_level = (uint32_t)std::max<long long>(0, std::min<long long>(4, I("cache")));
     cfg.pma_density = F("pma_density");
     cfg.device = (int32_t)I("device");
     sg_engine *eng = nullptr;

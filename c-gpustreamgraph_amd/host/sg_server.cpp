@@ -1,0 +1,402 @@
+// Request loop over one dynamic graph (declared in sgraph_serve.h). No
+// counterpart in the reference, whose drivers run a fixed snapshot list with
+// one source per process (samples/hybrid_*/main.cu). The worker thread owns
+// every engine handle; submitters only touch the request table under the
+// mutex.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "sgraph.h"
+#include "sgraph_serve.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    uint32_t kind = 0, arg = 0, target = 0;
+    std::vector<uint32_t> ins, del;
+    std::vector<uint8_t> result;
+    sg_result_info info{};
+    sg_status status = SG_OK;
+    std::string error;
+    bool done = false;
+    Clock::time_point t_submit;
+};
+
+double pct(std::vector<double> v, double p) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double r = p / 100.0 * (double)(v.size() - 1);
+    const size_t i = (size_t)r;
+    const double f = r - (double)i;
+    return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+}
+
+}  // namespace
+
+struct sg_server {
+    sg_server_config cfg{};
+    uint32_t nv = 0;
+    sg_engine *multi = nullptr, *cc = nullptr, *pr = nullptr;
+    uint32_t slots = 64;
+    uint32_t kp_allocated = 0;  // query slots the batched engine already holds state for
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<uint64_t> q_updates, q_queries, q_lookups;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::thread worker;
+    std::string error;
+
+    uint64_t snapshot = 0, step = 0;
+    std::vector<double> lat_all, lat_q, lat_u;
+    sg_server_stats st{};
+    Clock::time_point t_start;
+
+    void fail_request(Request *r, sg_status code, const std::string &msg) {
+        r->status = code;
+        r->error = msg;
+    }
+
+    std::string engine_error(sg_engine *e) { return std::string(sg_last_error(e)); }
+
+    // Complete requests (under the lock) and wake waiters.
+    void complete(const std::vector<std::pair<uint64_t, Request *>> &done) {
+        const auto now = Clock::now();
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto &d : done) {
+            Request *r = d.second;
+            r->info.latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
+            r->info.step = step;
+            r->done = true;
+            lat_all.push_back(r->info.latency_ms);
+            if (r->kind == SG_REQ_UPDATE) lat_u.push_back(r->info.latency_ms);
+            else if (r->kind == SG_REQ_BFS || r->kind == SG_REQ_SSSP) lat_q.push_back(r->info.latency_ms);
+        }
+        cv_done.notify_all();
+    }
+
+    // How many queries the batched engine may take now: the state of K slots
+    // is about nv * K * 8 + 3 * nv * ceil(K / 64) * 8 bytes; only growth
+    // beyond what it already holds needs free HBM.
+    uint32_t admissible(uint32_t want) {
+        uint64_t fr = 0, tot = 0;
+        if (sg_device_memory(cfg.engine.device, &fr, &tot) != SG_OK) return want;
+        const double reserve = cfg.hbm_reserve_gb * 1073741824.0;
+        auto bytes = [&](uint32_t k) {
+            uint32_t qs = 1;
+            while (qs < std::min<uint32_t>(k, 64)) qs <<= 1;
+            const uint64_t g = (k + qs - 1) / qs, kp = g * qs;
+            return (double)nv * (double)kp * 8.0 + 3.0 * (double)nv * (double)g * 8.0;
+        };
+        uint32_t k = want;
+        while (k > 0) {
+            const double grow = bytes(k) > bytes(kp_allocated) ? bytes(k) : 0.0;  // a new allocation replaces the old
+            if (grow == 0.0 || (double)fr + (kp_allocated ? bytes(kp_allocated) : 0.0) - grow >= reserve) break;
+            k /= 2;
+        }
+        return k;
+    }
+
+    void run_step(std::vector<uint64_t> &U, std::vector<uint64_t> &Q, std::vector<uint64_t> &L) {
+        std::vector<std::pair<uint64_t, Request *>> done;
+        auto req = [&](uint64_t id) {
+            std::lock_guard<std::mutex> lk(mu);
+            return reqs[id].get();
+        };
+        // 1. updates, in submission order, one snapshot each, to every engine
+        for (uint64_t id : U) {
+            Request *r = req(id);
+            const uint64_t ni = r->ins.size() / 2, nd = r->del.size() / 2;
+            for (sg_engine *e : {multi, cc, pr}) {
+                if (!e) continue;
+                const sg_status s = sg_apply_snapshot(e, r->ins.data(), ni, r->del.data(), nd);
+                if (s != SG_OK) { fail_request(r, s, engine_error(e)); break; }
+            }
+            ++snapshot;
+            st.edge_updates += ni + nd;
+            r->info.snapshot = snapshot;
+            r->ins.clear();
+            r->ins.shrink_to_fit();
+            r->del.clear();
+            r->del.shrink_to_fit();
+            done.push_back({id, r});
+        }
+        st.snapshots = snapshot;
+        sg_stats es{};
+        uint64_t ne = 0;
+        if (sg_get_stats(multi, &es) == SG_OK) ne = es.ne;
+        // 2. lookups on the current snapshot
+        std::vector<uint32_t> verts;
+        std::vector<Request *> ccr;
+        uint32_t kmax = 0;
+        for (uint64_t id : L) {
+            Request *r = req(id);
+            r->info.snapshot = snapshot;
+            if (r->kind == SG_REQ_CC) {
+                verts.push_back(r->arg);
+                ccr.push_back(r);
+            } else {
+                kmax = std::max(kmax, r->arg);
+            }
+            done.push_back({id, r});
+        }
+        if (!ccr.empty()) {
+            std::vector<uint32_t> out(verts.size());
+            const sg_status s = sg_gather_at(cc, SG_Q_VALUE, 0, verts.data(), verts.size(), out.data());
+            for (size_t i = 0; i < ccr.size(); ++i) {
+                if (s != SG_OK) { fail_request(ccr[i], s, engine_error(cc)); continue; }
+                ccr[i]->result.resize(4);
+                std::memcpy(ccr[i]->result.data(), &out[i], 4);
+            }
+        }
+        if (kmax) {
+            std::vector<uint32_t> ids(kmax);
+            std::vector<float> rk(kmax);
+            const sg_status s = sg_pr_topk(pr, kmax, ids.data(), rk.data());
+            for (uint64_t id : L) {
+                Request *r = req(id);
+                if (r->kind != SG_REQ_PR_TOPK) continue;
+                if (s != SG_OK) { fail_request(r, s, engine_error(pr)); continue; }
+                const uint32_t k = std::min<uint32_t>(r->arg, nv);
+                r->result.resize((size_t)k * 8);
+                std::memcpy(r->result.data(), ids.data(), (size_t)k * 4);
+                std::memcpy(r->result.data() + (size_t)k * 4, rk.data(), (size_t)k * 4);
+            }
+        }
+        st.lookups += L.size();
+        // 3. queries admitted this step, converged together
+        if (!Q.empty()) {
+            std::vector<uint32_t> src, kinds;
+            std::vector<Request *> qr;
+            for (uint64_t id : Q) {
+                Request *r = req(id);
+                src.push_back(r->arg);
+                kinds.push_back(r->kind == SG_REQ_BFS ? (uint32_t)SG_ALGO_BFS : (uint32_t)SG_ALGO_SSSP);
+                qr.push_back(r);
+            }
+            const sg_status s = sg_init_queries(multi, src.data(), kinds.data(), (uint32_t)src.size());
+            kp_allocated = std::max<uint32_t>(kp_allocated, (uint32_t)src.size());
+            for (size_t i = 0; i < qr.size(); ++i) {
+                Request *r = qr[i];
+                r->info.snapshot = snapshot;
+                if (s != SG_OK) { fail_request(r, s, engine_error(multi)); continue; }
+                sg_status g;
+                if (r->target == 0xFFFFFFFFu) {
+                    r->result.resize((size_t)nv * 4);
+                    g = sg_gather_query(multi, (uint32_t)i, SG_Q_VALUE, r->result.data(), r->result.size());
+                } else {
+                    r->result.resize(4);
+                    g = sg_gather_at(multi, SG_Q_VALUE, (uint32_t)i, &r->target, 1, (uint32_t *)r->result.data());
+                }
+                if (g != SG_OK) fail_request(r, g, engine_error(multi));
+            }
+            for (size_t i = 0; i < Q.size(); ++i) done.push_back({Q[i], qr[i]});
+            st.queries += Q.size();
+            st.edges_answered += ne * Q.size();
+            st.max_batch = std::max<uint64_t>(st.max_batch, Q.size());
+        }
+        complete(done);
+    }
+
+    void loop() {
+        for (;;) {
+            std::vector<uint64_t> U, Q, L;
+            uint32_t deferred = 0;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stop || !q_updates.empty() || !q_queries.empty() || !q_lookups.empty(); });
+                if (stop && q_updates.empty() && q_queries.empty() && q_lookups.empty()) break;
+                U.assign(q_updates.begin(), q_updates.end());
+                q_updates.clear();
+                L.assign(q_lookups.begin(), q_lookups.end());
+                q_lookups.clear();
+                const uint32_t want = (uint32_t)std::min<size_t>(q_queries.size(), slots);
+                lk.unlock();
+                const uint32_t k = want ? std::max<uint32_t>(1, admissible(want)) : 0;
+                lk.lock();
+                deferred = want - k;
+                for (uint32_t i = 0; i < k && !q_queries.empty(); ++i) {
+                    Q.push_back(q_queries.front());
+                    q_queries.pop_front();
+                }
+                ++step;
+            }
+            const auto t0 = Clock::now();
+            run_step(U, Q, L);
+            std::lock_guard<std::mutex> lk(mu);
+            st.busy_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+            st.deferred_for_memory += deferred;
+            st.steps = step;
+        }
+    }
+};
+
+extern "C" {
+
+void sg_server_config_default(sg_server_config *cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof *cfg);
+    sg_config_default(&cfg->engine);
+    cfg->query_slots = 64;
+    cfg->standing = SG_SERVE_CC | SG_SERVE_PR;
+    cfg->hbm_reserve_gb = 1.0;
+}
+
+static sg_status make_engine(const sg_server_config *cfg, uint32_t algo, uint32_t nv, uint64_t ne, const uint64_t *row,
+                             const uint32_t *col, sg_engine **out, std::string &err) {
+    sg_config c = cfg->engine;
+    c.algo = algo;
+    sg_status s = sg_create(&c, out);
+    if (s != SG_OK) { err = sg_last_error(nullptr); return s; }
+    s = sg_load_csr(*out, nv, ne, row, col);
+    if (s == SG_OK && algo != SG_ALGO_MULTI) s = sg_init(*out, nullptr, 0);
+    if (s != SG_OK) err = sg_last_error(*out);
+    return s;
+}
+
+sg_status sg_server_create(const sg_server_config *cfg, uint32_t nv, uint64_t ne, const uint64_t *row,
+                           const uint32_t *col, sg_server **out) {
+    if (!cfg || !out) return SG_EINVAL;
+    *out = nullptr;
+    std::unique_ptr<sg_server> s(new (std::nothrow) sg_server());
+    if (!s) return SG_ENOMEM;
+    s->cfg = *cfg;
+    s->nv = nv;
+    s->slots = cfg->query_slots ? std::min<uint32_t>(cfg->query_slots, 4096) : 64;
+    std::string err;
+    sg_status st = make_engine(cfg, SG_ALGO_MULTI, nv, ne, row, col, &s->multi, err);
+    if (st == SG_OK && (cfg->standing & SG_SERVE_CC)) st = make_engine(cfg, SG_ALGO_CC, nv, ne, row, col, &s->cc, err);
+    if (st == SG_OK && (cfg->standing & SG_SERVE_PR)) st = make_engine(cfg, SG_ALGO_PR, nv, ne, row, col, &s->pr, err);
+    if (st != SG_OK) {
+        sg_destroy(s->multi);
+        sg_destroy(s->cc);
+        sg_destroy(s->pr);
+        return st;
+    }
+    s->t_start = Clock::now();
+    sg_server *raw = s.get();
+    try {
+        s->worker = std::thread([raw] { raw->loop(); });
+    } catch (...) {
+        sg_destroy(s->multi);
+        sg_destroy(s->cc);
+        sg_destroy(s->pr);
+        return SG_ENOMEM;
+    }
+    *out = s.release();
+    return SG_OK;
+}
+
+sg_status sg_server_submit(sg_server *s, uint32_t kind, uint32_t arg, uint32_t target, const uint32_t *ins_uv,
+                           uint64_t nins, const uint32_t *del_uv, uint64_t ndel, uint64_t *ticket) {
+    if (!s || !ticket) return SG_EINVAL;
+    std::unique_ptr<Request> r(new (std::nothrow) Request());
+    if (!r) return SG_ENOMEM;
+    r->kind = kind;
+    r->arg = arg;
+    r->target = target;
+    switch (kind) {
+        case SG_REQ_UPDATE:
+            if ((nins && !ins_uv) || (ndel && !del_uv)) { s->error = "null update array"; return SG_EINVAL; }
+            for (uint64_t i = 0; i < 2 * nins; ++i)
+                if (ins_uv[i] >= s->nv) { s->error = "insert id >= nv"; return SG_EINVAL; }
+            for (uint64_t i = 0; i < 2 * ndel; ++i)
+                if (del_uv[i] >= s->nv) { s->error = "delete id >= nv"; return SG_EINVAL; }
+            r->ins.assign(ins_uv, ins_uv + 2 * nins);
+            r->del.assign(del_uv, del_uv + 2 * ndel);
+            break;
+        case SG_REQ_BFS:
+        case SG_REQ_SSSP:
+            if (arg >= s->nv || (target != 0xFFFFFFFFu && target >= s->nv)) { s->error = "vertex id >= nv"; return SG_EINVAL; }
+            break;
+        case SG_REQ_CC:
+            if (!s->cc) { s->error = "CC labels are not maintained (SG_SERVE_CC)"; return SG_EINVAL; }
+            if (arg >= s->nv) { s->error = "vertex id >= nv"; return SG_EINVAL; }
+            break;
+        case SG_REQ_PR_TOPK:
+            if (!s->pr) { s->error = "PageRank is not maintained (SG_SERVE_PR)"; return SG_EINVAL; }
+            if (arg == 0) { s->error = "k must be at least 1"; return SG_EINVAL; }
+            break;
+        default:
+            s->error = "unknown request kind";
+            return SG_EINVAL;
+    }
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) { s->error = "server is stopping"; return SG_EINVAL; }
+    const uint64_t id = s->next_id++;
+    s->reqs[id] = std::move(r);
+    if (kind == SG_REQ_UPDATE) s->q_updates.push_back(id);
+    else if (kind == SG_REQ_BFS || kind == SG_REQ_SSSP) s->q_queries.push_back(id);
+    else s->q_lookups.push_back(id);
+    *ticket = id;
+    s->cv_work.notify_one();
+    return SG_OK;
+}
+
+sg_status sg_server_wait(sg_server *s, uint64_t ticket, void *out, size_t bytes, sg_result_info *info) {
+    if (!s) return SG_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) { s->error = "unknown ticket"; return SG_EINVAL; }
+    Request *r = it->second.get();
+    s->cv_done.wait(lk, [&] { return r->done; });
+    sg_status st = r->status;
+    if (st != SG_OK) s->error = r->error;
+    else if (!r->result.empty()) {
+        if (!out || bytes < r->result.size()) {
+            s->error = "output smaller than the result (" + std::to_string(r->result.size()) + " bytes)";
+            st = SG_EINVAL;
+        } else {
+            std::memcpy(out, r->result.data(), r->result.size());
+        }
+    }
+    if (info) *info = r->info;
+    if (st != SG_EINVAL || r->status != SG_OK) s->reqs.erase(it);  // a too-small buffer may retry
+    return st;
+}
+
+sg_status sg_server_get_stats(sg_server *s, sg_server_stats *out) {
+    if (!s || !out) return SG_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->st;
+    out->p50_ms = pct(s->lat_all, 50);
+    out->p99_ms = pct(s->lat_all, 99);
+    out->query_p50_ms = pct(s->lat_q, 50);
+    out->query_p99_ms = pct(s->lat_q, 99);
+    out->update_p50_ms = pct(s->lat_u, 50);
+    out->update_p99_ms = pct(s->lat_u, 99);
+    out->wall_ms = std::chrono::duration<double, std::milli>(Clock::now() - s->t_start).count();
+    return SG_OK;
+}
+
+const char *sg_server_last_error(const sg_server *s) { return s ? s->error.c_str() : "null server"; }
+
+void sg_server_destroy(sg_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    sg_destroy(s->multi);
+    sg_destroy(s->cc);
+    sg_destroy(s->pr);
+    delete s;
+}
+
+}  // extern "C"

@@ -19,8 +19,9 @@ SG_OK, SG_EINVAL, SG_ENOMEM, SG_EBUDGET, SG_EDEVICE, SG_EIO = range(6)
 STATUS_NAMES = {0: "SG_OK", 1: "SG_EINVAL", 2: "SG_ENOMEM", 3: "SG_EBUDGET", 4: "SG_EDEVICE", 5: "SG_EIO"}
 ALGO = {"pr": 0, "sssp": 1, "bfs": 2, "cc": 3, "multi": 4}
 WEIGHT = {"hash128": 0, "seeded": 1, "one": 2}
-Q_VALUE, Q_BUFFER, Q_PARENT, Q_DEGREE, Q_OFFSETS, Q_SLOTS = range(6)
+Q_VALUE, Q_BUFFER, Q_PARENT, Q_DEGREE, Q_OFFSETS, Q_SLOTS, Q_SEGMENTS = range(7)
 FLAG_KERNEL_TIMING = 0x1
+FLAG_HOST_EDGES = 0x2
 INF = 0xFFFFFFFF
 DEFAULT_SEED = 27491095  # kRandSeed of the reference (groute/graphs/util.h:68)
 
@@ -38,7 +39,8 @@ class Config(C.Structure):
         ("segments", C.c_uint32), ("n_stream", C.c_uint32), ("block_size", C.c_uint32),
         ("max_rounds", C.c_uint32), ("rounds_per_sync", C.c_uint32), ("flags", C.c_uint32),
         ("device_budget_gb", C.c_double), ("device", C.c_int32),
-        ("wave_budget", C.c_uint32), ("heavy_degree", C.c_uint32), ("reserved", C.c_uint32 * 5),
+        ("wave_budget", C.c_uint32), ("heavy_degree", C.c_uint32), ("cache_level", C.c_uint32),
+        ("edge_budget_gb", C.c_double), ("reserved", C.c_uint32 * 2),
     ]
 
 
@@ -49,12 +51,41 @@ class Stats(C.Structure):
         ("invalidated", C.c_uint64), ("rebalances", C.c_uint64), ("resizes", C.c_uint64),
         ("dropped_updates", C.c_uint64), ("relax_launches", C.c_uint64), ("relax_ms", C.c_double),
         ("relax_bytes", C.c_uint64), ("last_phase_ms", C.c_double), ("pr_residual_l1", C.c_double),
-        ("pr_residual_max", C.c_double), ("device_bytes", C.c_uint64), ("reserved", C.c_uint64 * 8),
+        ("pr_residual_max", C.c_double), ("device_bytes", C.c_uint64), ("host_edges", C.c_uint64),
+        ("cache_slots", C.c_uint64), ("cached_vertices", C.c_uint64), ("edges_from_cache", C.c_uint64),
+        ("streamed_bytes", C.c_uint64), ("stream_ms", C.c_double), ("host_bytes", C.c_uint64),
+        ("reserved", C.c_uint64 * 1),
     ]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
+
+class ServerConfig(C.Structure):
+    _fields_ = [("engine", Config), ("query_slots", C.c_uint32), ("standing", C.c_uint32),
+                ("hbm_reserve_gb", C.c_double), ("reserved", C.c_uint32 * 4)]
+
+
+class ResultInfo(C.Structure):
+    _fields_ = [("snapshot", C.c_uint64), ("latency_ms", C.c_double), ("step", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_uint64), ("snapshots", C.c_uint64), ("queries", C.c_uint64), ("lookups", C.c_uint64),
+        ("edge_updates", C.c_uint64), ("edges_answered", C.c_uint64), ("max_batch", C.c_uint64),
+        ("deferred_for_memory", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+        ("query_p50_ms", C.c_double), ("query_p99_ms", C.c_double), ("update_p50_ms", C.c_double),
+        ("update_p99_ms", C.c_double), ("busy_ms", C.c_double), ("wall_ms", C.c_double),
+        ("reserved", C.c_uint64 * 4),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+REQ_UPDATE, REQ_BFS, REQ_SSSP, REQ_CC, REQ_PR_TOPK = range(5)
+SERVE_CC, SERVE_PR = 1, 2
 
 _P = C.c_void_p
 _lib = None
@@ -71,6 +102,17 @@ SIGNATURES = [
     ("sg_apply_snapshot_device", C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint64]),
     ("sg_gather", C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
     ("sg_init_queries", C.c_int, [_P, _P, _P, C.c_uint32]),
+    ("sg_gather_at", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint64, _P]),
+    ("sg_pr_topk", C.c_int, [_P, C.c_uint32, _P, _P]),
+    ("sg_device_memory", C.c_int, [C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("sg_server_config_default", None, [C.POINTER(ServerConfig)]),
+    ("sg_server_create", C.c_int, [C.POINTER(ServerConfig), C.c_uint32, C.c_uint64, _P, _P, C.POINTER(_P)]),
+    ("sg_server_submit", C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, C.c_uint64, _P, C.c_uint64,
+                                   C.POINTER(C.c_uint64)]),
+    ("sg_server_wait", C.c_int, [_P, C.c_uint64, _P, C.c_size_t, C.POINTER(ResultInfo)]),
+    ("sg_server_get_stats", C.c_int, [_P, C.POINTER(ServerStats)]),
+    ("sg_server_last_error", C.c_char_p, [_P]),
+    ("sg_server_destroy", None, [_P]),
     ("sg_gather_query", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_size_t]),
     ("sg_get_stats", C.c_int, [_P, C.POINTER(Stats)]),
     ("sg_reset_stats", C.c_int, [_P]),
@@ -249,6 +291,18 @@ class Engine:
         _check(lib().sg_gather_query(self.h, qi, Q_VALUE if q is None else q, _ptr(out), out.nbytes), self.h)
         return out
 
+    def gather_at(self, vertices, q=None, qi=0):
+        v = np.ascontiguousarray(vertices, dtype=np.uint32)
+        out = np.empty(len(v), dtype=np.uint32)
+        _check(lib().sg_gather_at(self.h, Q_VALUE if q is None else q, qi, _ptr(v), len(v), _ptr(out)), self.h)
+        return out
+
+    def topk(self, k):
+        ids = np.empty(k, dtype=np.uint32)
+        ranks = np.empty(k, dtype=np.float32)
+        _check(lib().sg_pr_topk(self.h, k, _ptr(ids), _ptr(ranks)), self.h)
+        return ids, ranks
+
     def apply(self, ins, dele):
         ins = np.ascontiguousarray(ins, dtype=np.uint32).reshape(-1)
         dele = np.ascontiguousarray(dele, dtype=np.uint32).reshape(-1)
@@ -277,6 +331,8 @@ class Engine:
             out = np.empty(self.nv + 1, dtype=np.uint64)
         elif q == Q_SLOTS:
             out = np.empty(self.stats()["capacity"], dtype=np.uint32)
+        elif q == Q_SEGMENTS:
+            out = np.empty(max(self.cfg.segments, 1) + 1, dtype=np.uint32)
         elif q in (Q_VALUE, Q_BUFFER) and self.algo == "pr":
             out = np.empty(self.nv, dtype=np.float32)
         else:
@@ -307,3 +363,75 @@ class Engine:
 
     def synchronize(self):
         _check(lib().sg_synchronize(self.h), self.h)
+
+
+class Server:
+    """Request loop over one graph (sgraph_serve.h): updates, BFS/SSSP queries,
+    CC label lookups and PageRank top-k, from any thread."""
+
+    def __init__(self, row, col, query_slots=64, standing=SERVE_CC | SERVE_PR, hbm_reserve_gb=1.0, **engine):
+        cfg = ServerConfig()
+        lib().sg_server_config_default(C.byref(cfg))
+        for k, v in engine.items():
+            if k == "weight_mode" and isinstance(v, str):
+                v = WEIGHT[v]
+            setattr(cfg.engine, k, v)
+        cfg.query_slots = query_slots
+        cfg.standing = standing
+        cfg.hbm_reserve_gb = hbm_reserve_gb
+        row = np.ascontiguousarray(row, dtype=np.uint64)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        self.nv = len(row) - 1
+        h = _P()
+        st = lib().sg_server_create(C.byref(cfg), self.nv, len(col), _ptr(row), _ptr(col), C.byref(h))
+        if st != SG_OK:
+            raise SgError(st, lib().sg_last_error(None).decode())
+        self.h = h
+
+    def _err(self, st):
+        if st != SG_OK:
+            raise SgError(st, lib().sg_server_last_error(self.h).decode())
+
+    def submit(self, kind, arg=0, target=INF, ins=None, dele=None):
+        t = C.c_uint64()
+        if kind == REQ_UPDATE:
+            ins = np.ascontiguousarray(ins if ins is not None else [], dtype=np.uint32).reshape(-1)
+            dele = np.ascontiguousarray(dele if dele is not None else [], dtype=np.uint32).reshape(-1)
+            self._err(lib().sg_server_submit(self.h, kind, 0, 0, _ptr(ins), len(ins) // 2, _ptr(dele), len(dele) // 2,
+                                             C.byref(t)))
+        else:
+            self._err(lib().sg_server_submit(self.h, kind, arg, target, None, 0, None, 0, C.byref(t)))
+        return t.value
+
+    def wait(self, ticket, kind=None, arg=0, target=INF):
+        """Result array (or None for updates) and the ResultInfo as a dict."""
+        info = ResultInfo()
+        if kind in (REQ_BFS, REQ_SSSP):
+            out = np.empty(self.nv if target == INF else 1, dtype=np.uint32)
+        elif kind == REQ_CC:
+            out = np.empty(1, dtype=np.uint32)
+        elif kind == REQ_PR_TOPK:
+            out = np.empty(2 * arg, dtype=np.uint32)
+        else:
+            out = None
+        self._err(lib().sg_server_wait(self.h, ticket, _ptr(out) if out is not None else None,
+                                       out.nbytes if out is not None else 0, C.byref(info)))
+        if kind == REQ_PR_TOPK:
+            out = (out[:arg].copy(), out[arg:].view(np.float32).copy())
+        return out, {"snapshot": info.snapshot, "latency_ms": info.latency_ms, "step": info.step}
+
+    def stats(self):
+        s = ServerStats()
+        self._err(lib().sg_server_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().sg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -69,11 +69,20 @@ typedef struct sg_config {
     int32_t device;           /* HIP device ordinal                                       */
     uint32_t wave_budget;     /* light-tier edges per 64-vertex batch of a wave (0 = 1024) */
     uint32_t heavy_degree;    /* lists longer than this always become chunks (0 = 2048)  */
-    uint32_t reserved[5];
+    uint32_t cache_level;     /* -cache: HBM edge cache of {0, 268435455, 536870921,
+                                 805306365, 2^30} u32 slots for levels 0..4
+                                 (graph_datum.cuh:151-163); used with host edges   */
+    double edge_budget_gb;    /* device budget of the edge array in GiB
+                                 (-device_graph_budget_gb): an edge array larger than
+                                 this lives in pinned host memory (0 = no cap)       */
+    uint32_t reserved[2];
 } sg_config;
 
 /* Time the dominant relax kernel with HIP events recorded on its stream. */
 #define SG_FLAG_KERNEL_TIMING 0x1u
+/* Keep the edge array in pinned host memory whatever its size (the
+ * out-of-memory path of -cache / -SEGMENT, framework.cuh:310-384). */
+#define SG_FLAG_HOST_EDGES 0x2u
 
 typedef struct sg_stats {
     uint32_t nv;                  /* vertices                                          */
@@ -94,7 +103,14 @@ typedef struct sg_stats {
     double pr_residual_l1;        /* PR: sum |residual| after convergence (FP64 on GPU) */
     double pr_residual_max;       /* PR: max |residual| after convergence (FP64 on GPU) */
     uint64_t device_bytes;        /* device memory held by the engine                  */
-    uint64_t reserved[8];
+    uint64_t host_edges;          /* 1 when the edge array is in pinned host memory    */
+    uint64_t cache_slots;         /* HBM edge cache capacity (u32 slots)               */
+    uint64_t cached_vertices;     /* vertices whose list is in the cache now           */
+    uint64_t edges_from_cache;    /* relaxed edges read from the cache (cumulative)     */
+    uint64_t streamed_bytes;      /* host-to-device bytes of streamed segments (cumul.) */
+    double stream_ms;             /* wall time of the streamed sweeps (cumulative)      */
+    uint64_t host_bytes;          /* pinned host memory held by the engine             */
+    uint64_t reserved[1];
 } sg_stats;
 
 /* What sg_gather copies out. */
@@ -107,7 +123,9 @@ typedef enum sg_query {
                          graph_datum.cuh:317-335                                       */
     SG_Q_DEGREE = 3,  /* per-vertex u32 out-degree                                      */
     SG_Q_OFFSETS = 4, /* u64[nv + 1] PMA slot offsets (last entry = capacity)           */
-    SG_Q_SLOTS = 5    /* u32[capacity] PMA slots (UINT32_MAX = gap)                     */
+    SG_Q_SLOTS = 5,   /* u32[capacity] PMA slots (UINT32_MAX = gap)                     */
+    SG_Q_SEGMENTS = 6 /* u32[segments + 1] first vertex of each SEGMENT (last = nv), the
+                         greedy split by PMA span of Engine::LoadGraph, framework.cuh:310-362 */
 } sg_query;
 
 typedef struct sg_engine sg_engine;
@@ -182,6 +200,21 @@ sg_status sg_apply_snapshot_device(sg_engine *eng,
  * Replaces GatherValue/GatherParent/GatherBuffer (graph_datum.cuh:297-355). */
 sg_status sg_gather(sg_engine *eng, uint32_t query, void *out, size_t bytes);
 
+/* Pointwise gather: out[i] = the u32 result word of vertices[i] (UINT32_MAX
+ * for an id >= nv). query: SG_Q_VALUE, SG_Q_PARENT, SG_Q_BUFFER or
+ * SG_Q_DEGREE (PageRank values and residuals are f32 bit patterns); qi is
+ * the query index of an SG_ALGO_MULTI engine (ignored otherwise). For
+ * lookups that need a few vertices rather than all of them (serving).
+ * GatherValue / GatherParent (graph_datum.cuh:297-335) restricted to a list. */
+sg_status sg_gather_at(sg_engine *eng, uint32_t query, uint32_t qi, const uint32_t *vertices, uint64_t n,
+                       uint32_t *out);
+
+/* PageRank engines: ids[0..k) = the k vertices of highest rank (ties: smaller
+ * id first), ranks (optional) their values. The reference prints the top
+ * ranks on the host after a full copy (hybrid_pr_host.cu:161-229, -top_ranks);
+ * here the ranking is a device sort and only k entries cross the link. */
+sg_status sg_pr_topk(sg_engine *eng, uint32_t k, uint32_t *ids, float *ranks);
+
 /* Counters of the engine (see sg_stats). */
 sg_status sg_get_stats(sg_engine *eng, sg_stats *out);
 
@@ -193,6 +226,10 @@ sg_status sg_reset_stats(sg_engine *eng);
 sg_status sg_device_alloc(sg_engine *eng, size_t bytes, void **dptr);
 sg_status sg_device_free(sg_engine *eng, void *dptr);
 sg_status sg_copy_to_device(sg_engine *eng, void *dst, const void *src, size_t bytes);
+
+/* Free and total memory of HIP device `device` (admission control of the
+ * request loop). */
+sg_status sg_device_memory(int32_t device, uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* Wait for all work of the engine. */
 sg_status sg_synchronize(sg_engine *eng);

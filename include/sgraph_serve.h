@@ -1,0 +1,108 @@
+/*
+ * sgraph_serve.h — request loop over one dynamic graph (libsgraph.so).
+ *
+ * No counterpart in the reference: its drivers apply a fixed list of
+ * snapshots and answer one source per process (samples/hybrid_{pr,sssp,bfs,cc}/main.cu,
+ * HybridSSSP/HybridBFS/HybridCC/HybridPageRank). The server keeps the graph
+ * and vertex state resident on the device and takes requests as they come:
+ *
+ *   update batches   applied in submission order, one snapshot each (the
+ *                    engine's sg_apply_snapshot semantics: inserts, then
+ *                    deletes);
+ *   BFS / SSSP       one-shot queries from a source on the current snapshot,
+ *                    answered together by a batched engine (SG_ALGO_MULTI):
+ *                    continuous batching, queries that arrive while a step is
+ *                    on the device are admitted at the next step, as many as
+ *                    the query slots and the free HBM allow;
+ *   CC label lookup  label of one vertex from a standing CC computation that
+ *                    every update keeps at its fixed point;
+ *   PR top-k         the k highest ranks of a standing PageRank.
+ *
+ * A worker thread runs steps: apply pending updates, answer lookups, admit
+ * queries, converge them in one launch per round, complete their requests.
+ * Submission and waiting are thread-safe; every call returns an sg_status.
+ */
+#ifndef SGRAPH_SERVE_H
+#define SGRAPH_SERVE_H
+
+#include "sgraph.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sg_request_kind {
+    SG_REQ_UPDATE = 0,  /* update batch; no result                                   */
+    SG_REQ_BFS = 1,     /* levels from arg; u32[nv], or u32 at `target`               */
+    SG_REQ_SSSP = 2,    /* distances from arg (cfg weights); u32[nv], or u32 at target */
+    SG_REQ_CC = 3,      /* CC label of vertex arg; u32                                */
+    SG_REQ_PR_TOPK = 4  /* arg = k: u32[k] ids then f32[k] ranks                       */
+} sg_request_kind;
+
+#define SG_SERVE_CC 0x1u  /* maintain CC labels (needed by SG_REQ_CC)         */
+#define SG_SERVE_PR 0x2u  /* maintain PageRank (needed by SG_REQ_PR_TOPK)      */
+
+typedef struct sg_server_config {
+    sg_config engine;       /* device, weights, PMA density, budgets (algo ignored) */
+    uint32_t query_slots;   /* BFS/SSSP queries per step (1..4096, 0 = 64)          */
+    uint32_t standing;      /* SG_SERVE_* programs kept at their fixed point        */
+    double hbm_reserve_gb;  /* admission leaves this much HBM free                  */
+    uint32_t reserved[4];
+} sg_server_config;
+
+typedef struct sg_result_info {
+    uint64_t snapshot;   /* snapshots applied before the answer (updates: after) */
+    double latency_ms;   /* submission to completion                            */
+    uint64_t step;       /* worker step that completed the request              */
+} sg_result_info;
+
+typedef struct sg_server_stats {
+    uint64_t steps;              /* worker steps run                              */
+    uint64_t snapshots;          /* update batches applied                        */
+    uint64_t queries;            /* BFS/SSSP queries answered                     */
+    uint64_t lookups;            /* CC / PR requests answered                     */
+    uint64_t edge_updates;       /* inserted + deleted pairs applied              */
+    uint64_t edges_answered;     /* sum over answered queries of the edges of the
+                                    snapshot they were answered on                */
+    uint64_t max_batch;          /* most queries admitted in one step             */
+    uint64_t deferred_for_memory;/* admissions postponed by the HBM reserve       */
+    double p50_ms, p99_ms;       /* request latency, all completed requests        */
+    double query_p50_ms, query_p99_ms;
+    double update_p50_ms, update_p99_ms;
+    double busy_ms;              /* time the worker spent in steps                */
+    double wall_ms;              /* since the server started                      */
+    uint64_t reserved[4];
+} sg_server_stats;
+
+typedef struct sg_server sg_server;
+
+void sg_server_config_default(sg_server_config *cfg);
+
+/* Load G_0, bring the standing programs to their fixed points, start the
+ * worker. row/col as sg_load_csr. */
+sg_status sg_server_create(const sg_server_config *cfg, uint32_t nv, uint64_t ne, const uint64_t *row,
+                           const uint32_t *col, sg_server **out);
+
+/* Queue a request. UPDATE: (ins_uv, nins, del_uv, ndel) pairs, copied; ids >=
+ * nv are rejected here. BFS/SSSP: arg = source, target = a vertex for a
+ * single-value answer or UINT32_MAX for the whole vector. CC: arg = vertex.
+ * PR_TOPK: arg = k. *ticket identifies the request for sg_server_wait. */
+sg_status sg_server_submit(sg_server *s, uint32_t kind, uint32_t arg, uint32_t target, const uint32_t *ins_uv,
+                           uint64_t nins, const uint32_t *del_uv, uint64_t ndel, uint64_t *ticket);
+
+/* Block until the request completes, copy its result to out (bytes at least
+ * the result's size; NULL for updates) and release it. */
+sg_status sg_server_wait(sg_server *s, uint64_t ticket, void *out, size_t bytes, sg_result_info *info);
+
+sg_status sg_server_get_stats(sg_server *s, sg_server_stats *out);
+
+/* Text of the last error (the worker's, or of the last failed call). */
+const char *sg_server_last_error(const sg_server *s);
+
+/* Finish the queued requests, stop the worker, release everything. */
+void sg_server_destroy(sg_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SGRAPH_SERVE_H */

@@ -12,7 +12,7 @@ from conftest import ROOT
 
 def declared_functions():
     names = set()
-    for h in ("sgraph.h", "sgraph_io.h"):
+    for h in sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(sg_[a-z0-9_]+)\s*\(", text):
@@ -23,7 +23,7 @@ def declared_functions():
 def test_every_declared_symbol_is_exported():
     lib = sgraph.lib()
     names = declared_functions()
-    assert len(names) >= 24
+    assert len(names) >= 36
     for n in names:
         assert hasattr(lib, n), n
     # and the binding covers them all
@@ -35,6 +35,20 @@ def test_config_defaults_are_the_references():
     assert cfg.pr_alpha == 0.85 and cfg.pr_epsilon == 1e-6
     assert cfg.pma_density == 0.25 and cfg.n_stream == 3 and cfg.block_size == 512
     assert cfg.segments == 1 and cfg.weight_seed == 27491095
+    assert cfg.cache_level == 4 and cfg.edge_budget_gb == 0  # app_skeleton.h:100 -cache 4
+
+
+def test_struct_layouts_match_the_header():
+    # the ctypes mirrors must have the C structs' sizes (sg_config 104 bytes, sg_stats 200)
+    assert C.sizeof(sgraph.Config) == 104
+    assert C.sizeof(sgraph.Stats) == 200
+
+
+def test_server_config_defaults():
+    cfg = sgraph.ServerConfig()
+    sgraph.lib().sg_server_config_default(C.byref(cfg))
+    assert cfg.query_slots == 64 and cfg.standing == 3 and cfg.hbm_reserve_gb == 1.0
+    assert cfg.engine.pr_alpha == 0.85
 
 
 def test_null_handles_are_einval():

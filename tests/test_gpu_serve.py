@@ -1,0 +1,121 @@
+"""Request loop (sgraph_serve.h): update batches, BFS/SSSP queries, CC label
+lookups and PageRank top-k submitted from several threads while the worker
+runs. Every answer must be the one for the snapshot it reports: BFS/SSSP
+equal to the oracle and to a single-source engine on that snapshot (the same
+result alone and in the mix), CC labels equal to the oracle's, top-k ranks
+within the PageRank tolerance of the FP64 fixed point."""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sgraph
+
+pytestmark = pytest.mark.gpu
+
+
+def snapshots(row, col, ins, dele):
+    graphs = [(row, col)]
+    r, c = row, col
+    for t in range(len(ins)):
+        r, c = O.apply_updates(r, c, ins[t], dele[t])
+        graphs.append((r, c))
+    return graphs
+
+
+def test_mixed_requests_match_their_snapshots():
+    scale, nsnap = 10, 4
+    nv, row, col = sgraph.gen_rmat(scale, 8, seed=91)
+    ins, dele = sgraph.gen_updates(row, col, scale, nsnap, 150, 150, seed=92)
+    graphs = snapshots(row, col, ins, dele)
+    srv = sgraph.Server(row, col, query_slots=16, weight_mode="seeded", weight_seed=8)
+    rng = np.random.default_rng(1)
+    results = []
+    lock = threading.Lock()
+
+    def client(cid):
+        r = np.random.default_rng(100 + cid)
+        mine = []
+        for i in range(12):
+            kind = [sgraph.REQ_BFS, sgraph.REQ_SSSP, sgraph.REQ_CC, sgraph.REQ_PR_TOPK][int(r.integers(0, 4))]
+            arg = int(r.integers(0, nv)) if kind != sgraph.REQ_PR_TOPK else 5
+            mine.append((kind, arg, srv.submit(kind, arg)))
+        for kind, arg, t in mine:
+            out, info = srv.wait(t, kind, arg)
+            with lock:
+                results.append((kind, arg, out, info))
+
+    threads = [threading.Thread(target=client, args=(i,)) for i in range(3)]
+    for th in threads:
+        th.start()
+    # updates from one thread, in order: snapshot t is graphs[t]
+    for t in range(nsnap):
+        _, info = srv.wait(srv.submit(sgraph.REQ_UPDATE, ins=ins[t], dele=dele[t]))
+        assert info["snapshot"] == t + 1
+    for th in threads:
+        th.join()
+    st = srv.stats()
+    srv.close()
+    assert st["snapshots"] == nsnap and st["queries"] + st["lookups"] == len(results) == 36
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    singles = {}
+    for kind, arg, out, info in results:
+        r, c = graphs[info["snapshot"]]
+        tag = f"kind {kind} arg {arg} snapshot {info['snapshot']}"
+        if kind == sgraph.REQ_BFS:
+            assert np.array_equal(out, O.bfs(r, c, arg)), tag
+        elif kind == sgraph.REQ_SSSP:
+            exp = O.sssp(r, c, arg, "seeded", 8)
+            assert np.array_equal(out, exp), tag
+            if info["snapshot"] not in singles:  # alone: a single-source engine on the same snapshot
+                e = sgraph.Engine("sssp", weight_mode="seeded", weight_seed=8)
+                e.load_csr(r, c)
+                e.init(arg)
+                assert np.array_equal(e.values(), out), tag
+                e.close()
+                singles[info["snapshot"]] = True
+        elif kind == sgraph.REQ_CC:
+            assert out[0] == O.cc(r, c)[arg], tag
+        else:
+            ids, ranks = out
+            exp = O.pagerank(r, c)
+            assert np.all(np.abs(ranks - exp[ids]) <= 5e-5 * exp[ids]), tag
+            kth = np.sort(exp)[::-1][len(ids) - 1]
+            assert ranks.min() >= kth * (1 - 1e-4), tag
+
+
+def test_queries_are_batched_and_target_answers():
+    nv, row, col = sgraph.gen_rmat(10, 8, seed=5)
+    srv = sgraph.Server(row, col, query_slots=64, standing=0)
+    tickets = [(s, srv.submit(sgraph.REQ_BFS, s, target=7)) for s in range(40)]
+    for s, t in tickets:
+        out, info = srv.wait(t, sgraph.REQ_BFS, s, target=7)
+        assert out[0] == O.bfs(row, col, s)[7]
+    st = srv.stats()
+    assert st["max_batch"] > 1
+    with pytest.raises(sgraph.SgError):
+        srv.submit(sgraph.REQ_CC, 0)  # CC not maintained
+    with pytest.raises(sgraph.SgError):
+        srv.submit(sgraph.REQ_BFS, nv)
+    srv.close()
+
+
+def test_pointwise_gather_and_topk_on_engines():
+    nv, row, col = sgraph.gen_rmat(10, 8, seed=13)
+    e = sgraph.Engine("cc")
+    e.load_csr(row, col)
+    e.init()
+    vs = np.array([0, 5, nv - 1, nv + 3], dtype=np.uint32)
+    got = e.gather_at(vs)
+    lab = O.cc(row, col)
+    assert got[:3].tolist() == lab[vs[:3]].tolist() and got[3] == sgraph.INF
+    e.close()
+    p = sgraph.Engine("pr")
+    p.load_csr(row, col)
+    p.init()
+    ids, ranks = p.topk(10)
+    vals = p.values()
+    assert np.array_equal(ranks, vals[ids])
+    assert ranks.min() >= np.sort(vals)[::-1][9]
+    p.close()
