@@ -1857,7 +1857,9 @@ public:
     }
 
     void ensure_mspill() {
-        const size_t need = (size_t)((ne_ + nins_) / kMEdgeChunk + 1) * G_ + 4096;
+        // per round: one descriptor per kMEdgeChunk edges of a list, and at least
+        // one per frontier vertex whose list passes a wave's budget
+        const size_t need = (size_t)((ne_ + nins_) / kMEdgeChunk + nv_ + 1) * G_ + 4096;
         if (need <= mspill_cap_) return;
         for (auto &p : mspill_) { dfree(p); p = nullptr; }
         for (auto &p : mspill_) p = dalloc<MDesc>(need);

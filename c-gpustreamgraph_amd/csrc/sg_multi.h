@@ -218,7 +218,10 @@ __device__ void mlight(const GraphView &g, const MultiState &ms, const PassIO &i
             }
         }
         verts += (uint32_t)__popcll(__ballot(m != 0));
-        // spill the tails of long lists
+        // lists past the wave's edge budget go to other waves whole, as
+        // descriptors; of the others only the tail past kMEdgeChunk does
+        const uint32_t pre = wave_incl_scan(len);
+        if (len && pre > io.budget) len = 0;
         const uint32_t nch = deg > len ? (deg - len + kMEdgeChunk - 1) / kMEdgeChunk : 0u;
         const uint32_t incl = wave_incl_scan(nch);
         const uint32_t tot = __shfl(incl, kWave - 1, kWave);
@@ -302,7 +305,7 @@ template <int kMode>
 __global__ void __launch_bounds__(kBlock) k_mround(GraphView g, MultiState ms, PassIO io, MDesc *spill, const MDesc *sin,
                                                    uint32_t spill_cap, uint32_t *stamp) {
     __shared__ uint32_t s_out[kWavesPerBlock][kOutStage];
-    __shared__ unsigned long long s_red[kWavesPerBlock];
+    __shared__ unsigned long long s_red[kWavesPerBlock], s_red2[kWavesPerBlock];
     __shared__ MLight s_light[kWavesPerBlock];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (io.zero_cnt) *io.zero_cnt = 0;
@@ -352,14 +355,21 @@ __global__ void __launch_bounds__(kBlock) k_mround(GraphView g, MultiState ms, P
         mwalk<kMode>(g, ms, io, st, stamp, io.round + 1, d.u, d.grp, d.mask, d.start, d.len, edges);
     }
     stage_flush(st, io);
-    if (lane == 0) s_red[w] = edges;
+    // one atomic per counter and block
+    if (lane == 0) {
+        s_red[w] = edges;
+        s_red2[w] = verts;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long a = 0;
-        for (int i = 0; i < kWavesPerBlock; ++i) a += s_red[i];
+        unsigned long long a = 0, b = 0;
+        for (int i = 0; i < kWavesPerBlock; ++i) {
+            a += s_red[i];
+            b += s_red2[i];
+        }
         if (a) atomicAdd(io.stats + 0, a);
+        if (b) atomicAdd(io.stats + 1, b);
     }
-    if (lane == 0 && verts) atomicAdd(io.stats + 1, verts);
 }
 
 // Pairs (u, v) processed per wave step (one per item, QS slots each):

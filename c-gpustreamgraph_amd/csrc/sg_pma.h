@@ -481,6 +481,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep_edges(GraphView g, const unsig
     __shared__ uint32_t s_out[kWavesPerBlock][kSweepOutStage];
     __shared__ uint32_t s_w[kWavesPerBlock];
     __shared__ uint32_t s_wide;
+    __shared__ unsigned long long s_red[kWavesPerBlock];
     const unsigned long long E = dpre[g.nv];
     const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
@@ -569,8 +570,15 @@ __global__ void __launch_bounds__(kBlock) k_sweep_edges(GraphView g, const unsig
         __syncthreads();
     }
     stage_flush(st, io);
+    // one counter atomic per block (per-wave atomics on one address serialise)
     n_edges = wave_sum_u64(n_edges);
-    if (lane_id() == 0 && n_edges) atomicAdd(io.stats + 0, n_edges);
+    if (lane_id() == 0) s_red[w] = n_edges;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kWavesPerBlock; ++i) t += s_red[i];
+        if (t) atomicAdd(io.stats + 0, t);
+    }
 }
 
 // Invalidated vertices that are active on their own (CC: own label) join the

@@ -60,7 +60,7 @@ def test_host_edges_without_cache(algo):
 
 def test_edge_budget_moves_edges_to_host_and_forces_rebalance():
     # a budget far below the edge array; a dense PMA so the batches rebalance and resize
-    st = run_host("bfs", 11, 8, 3, 1500, segments=16, cache_level=2, edge_budget_gb=1e-4, pma_density=0.9)
+    st = run_host("bfs", 11, 8, 3, 1500, segments=16, cache_level=2, edge_budget_gb=1e-6, pma_density=0.9)
     assert st["rebalances"] + st["resizes"] > 0
 
 
