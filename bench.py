@@ -33,6 +33,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "k_sweep_edges_sssp_bfs_rmat22_ef16_b100k.pmc.txt")
+    if not os.path.exists(path):
+        return None, None
+    vals = []
+    for line in open(path):
+        if line.startswith(kernel):
+            vals.append(float(line.split()[-1]) * 1e6)
+    return (round(sum(vals) / len(vals)), os.path.relpath(path, ROOT)) if vals else (None, None)
+
+
 def cpu_baseline(row, col, seed, budget_s):
     """The reference's CPU validation path (SSSPHostNaive + BFSHost, restated in
     oracle/oracle.c) from scratch on the snapshot graph, one core."""
@@ -241,6 +254,10 @@ def main():
     ap.add_argument("--heavy-degree", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (shared-graph) measurement")
     ap.add_argument("--batched-only", action="store_true", help="measure only the batched path (profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of 256 MiB (5 times) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per launch of the roofline kernel measured with rocprofv3 FETCH_SIZE + WRITE_SIZE")
     ap.add_argument("--workload", default="incremental", choices=["incremental", "oom", "serve"],
                     help="incremental: the headline line; oom: out-of-memory path; serve: request loop")
     ap.add_argument("--edge-budget-gb", type=float, default=8.0, help="oom: device budget of the edge array")
@@ -258,6 +275,16 @@ def main():
               f"(every box has a single MI355X); run with --gpus 1", file=sys.stderr)
         return 2
 
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 26, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read": x.numel() * 4, "bytes_written": x.numel() * 4,
+                                               "copies": 5}}), flush=True)
+        return 0
     if args.algos is None:
         args.algos = "pr,cc" if args.workload == "oom" else "sssp,bfs"
     if args.workload == "oom":
@@ -323,10 +350,27 @@ def main():
     value = sum(edges_per_step) / t_sum / 1e9
     all_lat = np.array([x for a in algos for x in lat[a]]) * 1e3
     stats = {a: engines[a].stats() for a in algos}
-    relax_ms = sum(stats[a]["relax_ms"] for a in algos)
-    relax_bytes = sum(stats[a]["relax_bytes"] for a in algos)
-    launches = sum(stats[a]["relax_launches"] for a in algos)
-    achieved = relax_bytes / (relax_ms * 1e-3) / 1e9 if relax_ms > 0 else None
+    # HIP-event time and algorithmic bytes per kernel class, both engines
+    kt = {}
+    for a in algos:
+        for cls, (n, ms, b) in engines[a].kernel_times().items():
+            x = kt.setdefault(cls, [0, 0.0, 0])
+            x[0] += n
+            x[1] += ms
+            x[2] += b
+    kernels = {cls: {"launches": n, "ms_per_step": round(ms / args.steps, 4),
+                     "avg_launch_us": round(ms * 1e3 / n, 2) if n else None,
+                     "achieved_gbs": round(b / (ms * 1e-3) / 1e9, 2) if ms and b else None}
+               for cls, (n, ms, b) in kt.items()}
+    # the roofline is quoted for the class with the most kernel time that has a byte model
+    dom = max((c for c in kt if kt[c][2] > 0), key=lambda c: kt[c][1])
+    dn, dms, db = kt[dom]
+    achieved = db / (dms * 1e-3) / 1e9 if dms > 0 else None
+    traffic, traffic_src = (args.traffic, "--traffic") if args.traffic else (
+        measured_traffic("sg::k_sweep_edges") if dom == "sweep" and args.scale == 22 and args.edge_factor == 16
+        else (None, None))
+    names = {"sweep": "k_sweep_edges (boundary sweep over all live edges: 4 B per live edge + 16 B per vertex)",
+             "relax": "k_round (convergence rounds: 12 B per relaxed edge + 32 B per expanded vertex)"}
     out = {
         "metric": "updated-edge GTEPS (incremental SSSP+BFS per snapshot, 1 MI355X, plain per-snapshot path)",
         "value": round(value, 4),
@@ -352,17 +396,19 @@ def main():
         "latency_p50_ms": round(float(np.percentile(all_lat, 50)), 3),
         "latency_p99_ms": round(float(np.percentile(all_lat, 99)), 3),
         "roofline": {
-            "kernel": "k_round (convergence round: light items + spilled chunks of the previous round)",
+            "kernel": names.get(dom, dom),
             "bound": "hbm",
             "achieved": round(achieved, 2) if achieved else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None,
-            "launches": launches,
-            "avg_launch_us": round(relax_ms * 1e3 / launches, 2) if launches else None,
-            "algorithmic_bytes_per_launch": round(relax_bytes / launches) if launches else None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "launches": dn,
+            "avg_launch_us": round(dms * 1e3 / dn, 2) if dn else None,
+            "algorithmic_bytes_per_launch": round(db / dn) if dn else None,
         },
+        "kernels": kernels,
         "engine": {a: {k: stats[a][k] for k in ("rounds", "invalidated", "rebalances", "resizes", "edges_relaxed",
                                                "vertices_processed", "device_bytes")} for a in algos},
     }

@@ -72,6 +72,7 @@ struct Ctl {
     unsigned long long nwin;      // maximal windows
     unsigned long long inval;     // invalidated vertices
     unsigned long long pad[2];
+    unsigned long long seqcur;    // staging cursor of the filtered sequences
     uint32_t tcount;              // touched vertices
     uint32_t nchunks;             // merge chunks
     uint32_t pad32[2];
@@ -382,8 +383,8 @@ __global__ void __launch_bounds__(kBlock) k_win_gather(GraphView g, BatchView bv
                                                        const unsigned long long *wv_off, uint64_t Wv,
                                                        const unsigned long long *P, const uint32_t *mark,
                                                        const uint32_t *tidx, const TInfo *tinfo,
-                                                       const unsigned long long *cpos, const uint32_t *merged,
-                                                       uint32_t *out) {
+                                                       const unsigned long long *cpos, const MChunk *chunks,
+                                                       const uint32_t *merged, uint32_t *out) {
     const uint64_t gw = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
     const uint32_t lane = lane_id();
@@ -397,10 +398,13 @@ __global__ void __launch_bounds__(kBlock) k_win_gather(GraphView g, BatchView bv
         uint64_t ntail = 0;
         if (bit_test(mark, v)) {
             const TInfo ti = tinfo[tidx[v]];
-            if (ti.de > ti.db) {  // survivors of old list + inserts, packed by the filter
-                keep_old = 0;
-                tail = merged + cpos[ti.cbase];
-                ntail = cpos[ti.cbase + ti.nch] - cpos[ti.cbase];
+            if (ti.de > ti.db) {  // survivors of old list + inserts, staged by the filter chunk by chunk
+                for (uint32_t c = ti.cbase; c < ti.cbase + ti.nch; ++c) {
+                    const uint32_t *src = merged + chunks[c].ibase;
+                    const uint64_t at = cpos[c] - cpos[ti.cbase], cnt = cpos[c + 1] - cpos[c];
+                    for (uint64_t k = lane; k < cnt; k += kWave) dst[at + k] = src[k];
+                }
+                continue;
             } else {  // old list unchanged, inserts straight from the sorted batch
                 tail = bv.ins_val + ti.ib;
                 ntail = ti.ni;
@@ -794,7 +798,11 @@ public:
 
     // --- kernel timing --------------------------------------------------------
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
+    std::vector<uint32_t> ev_cls_;  // kernel class of each recorded pair
     size_t ev_used_ = 0;
+    // per kernel class (sg_kernel_class): launches, summed event time, algorithmic bytes
+    uint64_t kt_launches_[SG_KCLASS_COUNT] = {}, kt_bytes_[SG_KCLASS_COUNT] = {};
+    double kt_ms_[SG_KCLASS_COUNT] = {};
 
     static constexpr uint32_t kGridList = 2048;   // light-kernel blocks for device-sized lists
     static constexpr uint32_t kGridHeavy = 1024;
@@ -1049,19 +1057,22 @@ public:
     }
 
     // HIP events around one launch on the engine's stream.
-    void begin_timed(bool timed) {
+    void begin_timed(bool timed, uint32_t cls = SG_KCLASS_RELAX) {
         if (!timed) return;
         if (ev_used_ == events_.size()) {
             hipEvent_t a, b;
             SG_HIP(hipEventCreate(&a));
             SG_HIP(hipEventCreate(&b));
             events_.push_back({a, b});
+            ev_cls_.push_back(0);
         }
+        ev_cls_[ev_used_] = cls;
         SG_HIP(hipEventRecord(events_[ev_used_].first, stream_));
     }
     void end_timed(bool timed) {
         if (timed) SG_HIP(hipEventRecord(events_[ev_used_++].second, stream_));
     }
+    bool timing() const { return (cfg_.flags & SG_FLAG_KERNEL_TIMING) != 0; }
 
     template <class VFn, class EFn>
     void launch_pass(const PassIO &io, VFn vfn, EFn efn, bool timed) {
@@ -1093,8 +1104,13 @@ public:
         for (size_t i = 0; i < ev_used_; ++i) {
             float ms = 0.f;
             SG_HIP(hipEventElapsedTime(&ms, events_[i].first, events_[i].second));
-            st_.relax_ms += ms;
-            st_.relax_launches += 1;
+            const uint32_t c = ev_cls_[i];
+            kt_ms_[c] += ms;
+            kt_launches_[c] += 1;
+            if (c == SG_KCLASS_RELAX) {
+                st_.relax_ms += ms;
+                st_.relax_launches += 1;
+            }
         }
         ev_used_ = 0;
     }
@@ -1172,7 +1188,10 @@ public:
         if (timed) {
             timed_edges_ += s[8] + s[10];
             timed_verts_ += s[9];
-            if (cfg_.flags & SG_FLAG_KERNEL_TIMING) st_.relax_bytes += 12ull * (s[8] + s[10]) + 32ull * s[9];
+            if (cfg_.flags & SG_FLAG_KERNEL_TIMING) {
+                st_.relax_bytes += 12ull * (s[8] + s[10]) + 32ull * s[9];
+                kt_bytes_[SG_KCLASS_RELAX] += 12ull * (s[8] + s[10]) + 32ull * s[9];
+            }
         }
         if (timed) timed_cedges_ += s[13];
         st_.edges_relaxed = s[0] + s[2] + timed_edges_;
@@ -1214,6 +1233,8 @@ public:
         if (chunk_bound > 0xFFFFFFF0ull) fail(SG_EINVAL, "batch too large for the merge chunk list");
         chunk_bound_ = chunk_bound;
         ensure(tinfo_, tinfo_cap_, (size_t)nins + ndel + 1);
+        // filtered sequences are staged at their own ranges: at most every live edge plus the inserts
+        ensure(tmp_, tmp_cap_, std::max<uint64_t>(ne_ + nins, 1));
         ensure(chunks_, chunks_cap_, chunk_bound);
         ensure(ccount_, ccount_cap_, chunk_bound + 1);
         ensure(cpos_, cpos_cap_, chunk_bound + 1);
@@ -1244,11 +1265,13 @@ public:
             SG_LAUNCHED();
             const uint32_t gt = grid_for((uint64_t)nins + ndel, kBlock, 8192);
             hipLaunchKernelGGL(k_touch_info, dim3(gt), dim3(kBlock), 0, stream_, gview(), br_, touched_, &ctl_->tcount, tinfo_,
-                               tidx_, chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, &ctl_->pad[1]);
+                               tidx_, chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, &ctl_->seqcur, &ctl_->pad[1]);
             SG_LAUNCHED();
-            hipLaunchKernelGGL((k_chunk_filter<0>), dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_, tinfo_, chunks_,
-                               &ctl_->nchunks, (uint32_t)chunk_bound, ccount_, cpos_, tmp_);
+            begin_timed(timing(), SG_KCLASS_MERGE);
+            hipLaunchKernelGGL(k_chunk_filter, dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_,
+                               (const MChunk *)chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, ccount_, tmp_);
             SG_LAUNCHED();
+            end_timed(timing());
         }
         b = cub_cap_;
         SG_HIP(hipcub::DeviceScan::ExclusiveSum(cub_tmp_, b, ccount_, cpos_, (int)(chunk_bound + 1), stream_));
@@ -1270,15 +1293,13 @@ public:
         const Ctl c = *h_ctl_;
         const uint64_t ne_new = ne_ + (uint64_t)(long long)c.ne_delta;
         if (nins_ + ndel_) {
-            ensure(tmp_, tmp_cap_, std::max<uint64_t>(merged_total_, 1));
-            hipLaunchKernelGGL((k_chunk_filter<1>), dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_, tinfo_, chunks_,
-                               &ctl_->nchunks, (uint32_t)chunk_bound_, ccount_, cpos_, tmp_);
-            SG_LAUNCHED();
             if (overflow) rebalance(ne_new);  // moves the touched vertices of its windows
+            begin_timed(timing(), SG_KCLASS_MERGE);
             hipLaunchKernelGGL(k_chunk_place, dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_, col_, tinfo_, chunks_,
                                &ctl_->nchunks, (uint32_t)chunk_bound_, cpos_, tmp_, newdeg_, maximal_, nleaves_, L_,
                                overflow ? 1 : 0);
             SG_LAUNCHED();
+            end_timed(timing());
             hipLaunchKernelGGL(k_lane_place, dim3(grid_for(c.tcount, 256, 4096)), dim3(256), 0, stream_, gview(), bv_, col_,
                                tinfo_, &ctl_->tcount, maximal_, nleaves_, L_, overflow ? 1 : 0);
             SG_LAUNCHED();
@@ -1299,6 +1320,10 @@ public:
     // The batch's per-vertex ranges and marks are kept until the snapshot's
     // recompute has used them; then they are cleared for the next batch.
     void finish_batch() {
+        if (timing()) {
+            sync();
+            harvest_events();
+        }
         cache_rebuild();
         if (nins_ + ndel_ == 0) return;
         hipLaunchKernelGGL(k_clear_ranges, dim3(1024), dim3(256), 0, stream_, tinfo_, &ctl_->tcount, br_);
@@ -1375,7 +1400,7 @@ public:
         ensure(tmp2_, tmp2_cap_, std::max<uint64_t>(ne_new, 1));
         const uint32_t gw = grid_for(Wv, kWavesPerBlock, 16384);
         hipLaunchKernelGGL(k_win_gather, dim3(gw), dim3(kBlock), 0, stream_, gview(), bv_, win_, nwin, wvoff_, Wv, P_arr_, mark_,
-                           tidx_, tinfo_, cpos_, tmp_, tmp2_);
+                           tidx_, tinfo_, cpos_, (const MChunk *)chunks_, tmp_, tmp2_);
         SG_LAUNCHED();
         uint32_t *dst = col_;
         if (resize) {
@@ -1456,9 +1481,13 @@ public:
         const uint64_t ntiles = ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile;
         if (!host_edges_) {
             const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, 16384));
+            begin_timed(timing(), SG_KCLASS_SWEEP);
             hipLaunchKernelGGL((k_sweep_edges<Op, EFn>), dim3(grid), dim3(kBlock), 0, stream_, gview(), dpre_, tl,
                                (const uint32_t *)col_, (uint64_t)0, 0u, nv_, (uint64_t)0, ntiles, op, buffer, ef, io);
             SG_LAUNCHED();
+            end_timed(timing());
+            // algorithmic bytes: every live target id, and off + dpre of every source
+            if (timing()) kt_bytes_[SG_KCLASS_SWEEP] += 4ull * ne_ + 16ull * nv_;
             return;
         }
         streamed_sweep([&](const uint32_t *colp, uint64_t slo, uint32_t va, uint32_t vb, uint64_t tlo, uint64_t thi) {
@@ -2197,6 +2226,15 @@ sg_status sg_pr_topk(sg_engine *eng, uint32_t k, uint32_t *ids, float *ranks) {
     });
 }
 
+sg_status sg_kernel_times(sg_engine *eng, uint32_t cls, uint64_t *launches, double *ms, uint64_t *bytes) {
+    return guarded(eng, [&](sg::Engine &e) {
+        if (cls >= SG_KCLASS_COUNT || !launches || !ms || !bytes) sg::fail(SG_EINVAL, "bad kernel class or null output");
+        *launches = e.kt_launches_[cls];
+        *ms = e.kt_ms_[cls];
+        *bytes = e.kt_bytes_[cls];
+    });
+}
+
 sg_status sg_get_stats(sg_engine *eng, sg_stats *out) {
     return guarded(eng, [&](sg::Engine &e) {
         if (!out) sg::fail(SG_EINVAL, "null output");
@@ -2214,6 +2252,10 @@ sg_status sg_reset_stats(sg_engine *eng) {
         e.st_.relax_ms = 0;
         e.st_.relax_bytes = 0;
         e.timed_edges_ = e.timed_verts_ = e.timed_cedges_ = 0;
+        for (int c = 0; c < SG_KCLASS_COUNT; ++c) {
+            e.kt_launches_[c] = e.kt_bytes_[c] = 0;
+            e.kt_ms_[c] = 0;
+        }
         e.st_.edges_from_cache = e.st_.streamed_bytes = 0;
         e.st_.stream_ms = 0;
         if (e.stats_) {

@@ -112,6 +112,15 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
     }
     return x;
 }
+__device__ __forceinline__ unsigned long long wave_incl_scan(unsigned long long x) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned long long y = __shfl_up(x, d, kWave);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x;
+}
 // Wave-aggregated append: one atomicAdd per wave. Every active lane of the
 // wave must call it (pred may differ per lane).
 __device__ __forceinline__ void wave_append(bool pred, uint32_t item, uint32_t *list, uint32_t *cnt,

@@ -59,7 +59,7 @@ struct MChunk {
     uint32_t lo, hi;  // positions in the vertex's sequence (old list ++ inserts, or inserts only)
     uint32_t d0, ib, db, de;  // as TInfo
     uint32_t first;   // the vertex's first chunk (TInfo::cbase)
-    uint32_t pad[2];
+    uint64_t ibase;   // where the filter writes this chunk's survivors (vertices with deletes)
 };
 __device__ __forceinline__ TInfo chunk_info(const MChunk &m) { return TInfo{m.u, m.d0, 0, m.ib, m.db, m.de, m.first, 0}; }
 __device__ __forceinline__ MChunk load_chunk(const MChunk *chunks, uint64_t c, uint64_t n) {
@@ -130,7 +130,7 @@ __device__ __forceinline__ bool chunked(const TInfo &t) { return has_del(t) || t
 __global__ void __launch_bounds__(kBlock) k_touch_info(GraphView g, BatchRanges br, const uint32_t *touched,
                                                        const uint32_t *tcount, TInfo *tinfo, uint32_t *tidx,
                                                        MChunk *chunks, uint32_t *nchunks, uint32_t chunk_cap,
-                                                       unsigned long long *overflow) {
+                                                       unsigned long long *seqcur, unsigned long long *overflow) {
     const uint32_t n = *tcount;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -152,6 +152,13 @@ __global__ void __launch_bounds__(kBlock) k_touch_info(GraphView g, BatchRanges 
         uint32_t cb = 0;
         if (lane_id() == 0 && tot) cb = atomicAdd(nchunks, tot);
         cb = __shfl(cb, 0, kWave) + incl - nch;
+        // staging range of the filtered sequence (vertices with deletes)
+        const unsigned long long sl = (t < n && has_del(ti)) ? seq_len(ti) : 0ull;
+        const unsigned long long sincl = wave_incl_scan(sl);
+        const unsigned long long stot = __shfl(sincl, kWave - 1, kWave);
+        unsigned long long sb = 0;
+        if (lane_id() == 0 && stot) sb = atomicAdd(seqcur, stot);
+        sb = __shfl(sb, 0, kWave) + sincl - sl;
         if (t < n) {
             ti.cbase = cb;
             if (cb + nch > chunk_cap) {  // never by construction; flagged, and no chunk is read
@@ -173,6 +180,7 @@ __global__ void __launch_bounds__(kBlock) k_touch_info(GraphView g, BatchRanges 
                 m.db = ti.db;
                 m.de = ti.de;
                 m.first = ti.cbase;
+                m.ibase = sb + m.lo;
                 chunks[ti.cbase + c] = m;
             }
         }
@@ -221,15 +229,13 @@ __device__ __forceinline__ bool survives(uint32_t x, const BatchView &bv, const 
     return !in_sorted_u64(bv.del_key, ti.db, ti.de, ((uint64_t)ti.u << 32) | x);
 }
 
-// One wave per chunk of a vertex with deletes: count survivors (pass 0) or
-// write them in order to out[cpos[c] ...] (pass 1). Chunks of insert-only
-// vertices keep everything: pass 0 counts them without reading, pass 1 skips
-// them (they are placed straight from the sorted batch).
-template <int kPass>
-__global__ void __launch_bounds__(kBlock) k_chunk_filter(GraphView g, BatchView bv, const TInfo *tinfo,
-                                                         const MChunk *chunks, const uint32_t *nchunks, uint32_t chunk_cap,
-                                                         unsigned long long *ccount, const unsigned long long *cpos,
-                                                         uint32_t *out) {
+// One wave per chunk of a vertex with deletes: write the survivors, in
+// order, to out[ibase ...] and count them. Chunks of insert-only vertices
+// keep everything: counted without reads (they are placed straight from the
+// sorted batch).
+__global__ void __launch_bounds__(kBlock) k_chunk_filter(GraphView g, BatchView bv, const MChunk *chunks,
+                                                         const uint32_t *nchunks, uint32_t chunk_cap,
+                                                         unsigned long long *ccount, uint32_t *out) {
     __shared__ uint32_t s_del[kWavesPerBlock][kDelStage];
     __shared__ uint32_t s_mask[kWavesPerBlock][kDelMaskWords];
     const uint32_t n = min(*nchunks, chunk_cap);
@@ -242,7 +248,7 @@ __global__ void __launch_bounds__(kBlock) k_chunk_filter(GraphView g, BatchView 
         mn = load_chunk(chunks, c + nw, n);
         const TInfo ti = chunk_info(m);
         if (!has_del(ti)) {
-            if (!kPass && lane == 0) ccount[c] = m.hi - m.lo;
+            if (lane == 0) ccount[c] = m.hi - m.lo;
             continue;
         }
         const uint32_t *old = g.col + m.base;
@@ -256,16 +262,16 @@ __global__ void __launch_bounds__(kBlock) k_chunk_filter(GraphView g, BatchView 
             x[j] = i < m.hi ? (i < ti.d0 ? old[i] : ins[i - ti.d0]) : 0;
         }
         uint32_t cnt = 0;
-        uint32_t *dst = kPass ? out + cpos[c] : nullptr;
+        uint32_t *dst = out + m.ibase;
 #pragma unroll
         for (int j = 0; j < kMergePerLane; ++j) {  // survivors stay in sequence order
             const uint32_t i = m.lo + j * kWave + lane;
             const bool keep = i < m.hi && survives(x[j], bv, ti, s_del[w], s_mask[w], ns);
             const uint64_t b = __ballot(keep);
-            if (kPass && keep) dst[cnt + __popcll(b & lanemask_lt())] = x[j];
+            if (keep) dst[cnt + __popcll(b & lanemask_lt())] = x[j];
             cnt += (uint32_t)__popcll(b);
         }
-        if (!kPass && lane == 0) ccount[c] = cnt;
+        if (lane == 0) ccount[c] = cnt;
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -300,7 +306,7 @@ __global__ void __launch_bounds__(kBlock) k_chunk_place(GraphView g, BatchView b
         const bool hd = has_del(ti);
         const uint64_t p0 = (hd ? 0 : ti.d0) + (cpos[c] - cpos[ti.cbase]);
         const uint64_t len = cpos[c + 1] - cpos[c];
-        const uint32_t *srcp = hd ? merged + cpos[c] : bv.ins_val + ti.ib + m.lo;
+        const uint32_t *srcp = hd ? merged + m.ibase : bv.ins_val + ti.ib + m.lo;
         uint32_t x[kMergePerLane];  // len <= kMergeChunk: one load per lane and slot, all in flight
 #pragma unroll
         for (int j = 0; j < kMergePerLane; ++j) {
@@ -349,16 +355,6 @@ struct SweepTiles {
     unsigned long long *e;   // [cap] first live-edge index of tile t
     uint64_t cap;            // entries of v and e
 };
-
-__device__ __forceinline__ unsigned long long wave_incl_scan(unsigned long long x) {
-    const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const unsigned long long y = __shfl_up(x, d, kWave);
-        if (lane >= d) x += y;
-    }
-    return x;
-}
 
 // Block-wide exclusive scan of one value per thread; *total gets the sum.
 // s_w holds kScanBlock / kWave words; callers alternate two of them.

@@ -22,6 +22,7 @@ WEIGHT = {"hash128": 0, "seeded": 1, "one": 2}
 Q_VALUE, Q_BUFFER, Q_PARENT, Q_DEGREE, Q_OFFSETS, Q_SLOTS, Q_SEGMENTS = range(7)
 FLAG_KERNEL_TIMING = 0x1
 FLAG_HOST_EDGES = 0x2
+KCLASS = ("relax", "sweep", "merge")
 INF = 0xFFFFFFFF
 DEFAULT_SEED = 27491095  # kRandSeed of the reference (groute/graphs/util.h:68)
 
@@ -115,6 +116,7 @@ SIGNATURES = [
     ("sg_server_destroy", None, [_P]),
     ("sg_gather_query", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_size_t]),
     ("sg_get_stats", C.c_int, [_P, C.POINTER(Stats)]),
+    ("sg_kernel_times", C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     ("sg_reset_stats", C.c_int, [_P]),
     ("sg_device_alloc", C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     ("sg_device_free", C.c_int, [_P, _P]),
@@ -357,6 +359,15 @@ class Engine:
         s = Stats()
         _check(lib().sg_get_stats(self.h, C.byref(s)), self.h)
         return s.as_dict()
+
+    def kernel_times(self):
+        """{class: (launches, ms, algorithmic bytes)} of the HIP-event-timed kernel classes."""
+        out = {}
+        for c, name in enumerate(KCLASS):
+            n, ms, b = C.c_uint64(), C.c_double(), C.c_uint64()
+            _check(lib().sg_kernel_times(self.h, c, C.byref(n), C.byref(ms), C.byref(b)), self.h)
+            out[name] = (n.value, ms.value, b.value)
+        return out
 
     def reset_stats(self):
         _check(lib().sg_reset_stats(self.h), self.h)

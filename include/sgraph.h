@@ -215,6 +215,21 @@ sg_status sg_gather_at(sg_engine *eng, uint32_t query, uint32_t qi, const uint32
  * here the ranking is a device sort and only k entries cross the link. */
 sg_status sg_pr_topk(sg_engine *eng, uint32_t k, uint32_t *ids, float *ranks);
 
+/* Kernel classes timed with HIP events when SG_FLAG_KERNEL_TIMING is set
+ * (events recorded on the engine's stream around each launch). */
+typedef enum sg_kernel_class {
+    SG_KCLASS_RELAX = 0,  /* convergence rounds of a snapshot (k_round / k_mround);
+                             bytes: 12 per relaxed edge + 32 per expanded vertex   */
+    SG_KCLASS_SWEEP = 1,  /* boundary sweep over all live edges (resident edges);
+                             bytes: 4 per live edge + 16 per vertex (off, dpre)    */
+    SG_KCLASS_MERGE = 2,  /* PMA batch merge: chunk count, chunk write, place       */
+    SG_KCLASS_COUNT = 3
+} sg_kernel_class;
+
+/* Launches, summed HIP-event milliseconds and algorithmic bytes (0 where the
+ * class has no byte model) of a kernel class since the last sg_reset_stats. */
+sg_status sg_kernel_times(sg_engine *eng, uint32_t cls, uint64_t *launches, double *ms, uint64_t *bytes);
+
 /* Counters of the engine (see sg_stats). */
 sg_status sg_get_stats(sg_engine *eng, sg_stats *out);
 
