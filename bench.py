@@ -166,6 +166,64 @@ def run_oom(args):
     }
 
 
+def run_batch(args):
+    """Many BFS + SSSP sources on one graph under a stream of update batches of
+    log-uniform sizes: the batched engine (all queries, one PMA update, one
+    sweep and one launch per round) against plain single-source engines."""
+    rng = np.random.default_rng(args.seed)
+    t = time.perf_counter()
+    nv, row, col = sgraph.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
+    nsnap = args.warmup + args.steps
+    ins, dele = sgraph.gen_updates(row, col, args.scale, nsnap, args.max_update // 2, args.max_update // 2,
+                                   seed=args.seed + 1)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(args.max_update), nsnap)).astype(np.int64)
+    batches = [(ins[i][: sizes[i] // 2], dele[i][: sizes[i] - sizes[i] // 2]) for i in range(nsnap)]
+    nq = args.queries
+    sources = rng.integers(0, nv, nq).astype(np.uint32)
+    kinds = ["bfs"] * (nq // 2) + ["sssp"] * (nq - nq // 2)
+    wseed = args.seed + 2
+    log(f"[bench] batch graph nv={nv} ne={len(col)} queries={nq} sizes={sizes.tolist()} gen {time.perf_counter() - t:.1f}s")
+
+    def run(engine_algo, init):
+        e = sgraph.Engine(engine_algo, weight_mode="seeded", weight_seed=wseed)
+        e.load_csr(row, col)
+        init(e)
+        for s in range(args.warmup):
+            e.apply(*batches[s])
+        e.synchronize()
+        lat, num = [], 0
+        for s in range(args.warmup, nsnap):
+            t0 = time.perf_counter()
+            e.apply(*batches[s])
+            lat.append(time.perf_counter() - t0)
+            num += e.stats()["ne"]
+        e.close()
+        return num, lat
+
+    num, lat_b = run("multi", lambda e: e.init_queries(sources, kinds))
+    batched = num * nq / sum(lat_b) / 1e9
+    # plain: one single-source engine per kind, from the first source of each kind
+    plain_t, plain_e = 0.0, 0
+    for k, src in (("bfs", int(sources[0])), ("sssp", int(sources[-1]))):
+        n1, lat1 = run(k, lambda e, s=src: e.init(s))
+        plain_t += sum(lat1)
+        plain_e += n1
+    plain = plain_e / plain_t / 1e9
+    lat_ms = np.array(lat_b) * 1e3
+    return {
+        "metric": "updated-edge GTEPS of batched BFS+SSSP queries (E_t x queries / T_t), 1 MI355X",
+        "value": round(batched, 4), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "dtype": "u32", "data": "synthetic (seeded R-MAT, log-uniform batch sizes)",
+        "config": {"workload": f"{nq // 2} BFS + {nq - nq // 2} SSSP queries (seeded uniform sources) on R-MAT scale "
+                               f"{args.scale} EF{args.edge_factor}, batches log-uniform 1000..{args.max_update} pairs "
+                               f"(50/50 insert/delete)", "nv": nv, "ne0": len(col)},
+        "latency_p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
+        "latency_p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "plain": {"value": round(plain, 4), "what": "single-source engines (one BFS, one SSSP) on the same batches"},
+        "vs_plain": round(batched / plain, 2),
+    }
+
+
 def run_serve(args):
     """Request loop: Poisson update batches and mixed queries against one
     resident graph; throughput and p50/p99 request latency."""
@@ -258,7 +316,7 @@ def main():
                     help="run only a plain device copy of 256 MiB (5 times) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per launch of the roofline kernel measured with rocprofv3 FETCH_SIZE + WRITE_SIZE")
-    ap.add_argument("--workload", default="incremental", choices=["incremental", "oom", "serve"],
+    ap.add_argument("--workload", default="incremental", choices=["incremental", "oom", "serve", "batch"],
                     help="incremental: the headline line; oom: out-of-memory path; serve: request loop")
     ap.add_argument("--edge-budget-gb", type=float, default=8.0, help="oom: device budget of the edge array")
     ap.add_argument("--cache", type=int, default=4, help="oom: -cache level")
@@ -269,6 +327,7 @@ def main():
     ap.add_argument("--max-update", type=int, default=100000)
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--clients", type=int, default=4)
+    ap.add_argument("--queries", type=int, default=128, help="batch: BFS + SSSP queries")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU "
@@ -292,6 +351,9 @@ def main():
         return 0
     if args.workload == "serve":
         print(json.dumps(run_serve(args)), flush=True)
+        return 0
+    if args.workload == "batch":
+        print(json.dumps(run_batch(args)), flush=True)
         return 0
     algos = args.algos.split(",")
     nsnap = args.warmup + args.steps
