@@ -310,6 +310,7 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="do not time the relax kernel with HIP events")
     ap.add_argument("--wave-budget", type=int, default=0)
     ap.add_argument("--heavy-degree", type=int, default=0)
+    ap.add_argument("--round-blocks", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (shared-graph) measurement")
     ap.add_argument("--batched-only", action="store_true", help="measure only the batched path (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -373,7 +374,7 @@ def main():
     for a in algos:
         flags = 0 if args.no_timing else sgraph.FLAG_KERNEL_TIMING
         e = sgraph.Engine(a, weight_mode="seeded", weight_seed=wseed, flags=flags, wave_budget=args.wave_budget,
-                          heavy_degree=args.heavy_degree)
+                          heavy_degree=args.heavy_degree, round_blocks=args.round_blocks)
         t = time.perf_counter()
         e.load_csr(row, col)
         e.init(0)

@@ -991,6 +991,7 @@ public:
         return g;
     }
     uint32_t budget() const { return cfg_.wave_budget ? cfg_.wave_budget : kWaveBudget; }
+    uint32_t round_blocks() const { return cfg_.round_blocks ? std::min<uint32_t>(cfg_.round_blocks, 65535) : kGridList; }
 
     // dpre_[v] = sum of deg[0..v) for v in [0, nv] (64-bit) and the merge-path
     // tiles of the edge-centric sweep, in three kernels.
@@ -1076,7 +1077,7 @@ public:
 
     template <class VFn, class EFn>
     void launch_pass(const PassIO &io, VFn vfn, EFn efn, bool timed) {
-        const uint32_t g1 = io.in_list ? kGridList : grid_for(io.range_n, kWave * kWavesPerBlock, kGridList);
+        const uint32_t g1 = io.in_list ? round_blocks() : grid_for(io.range_n, kWave * kWavesPerBlock, round_blocks());
         begin_timed(timed);
         hipLaunchKernelGGL((k_round<VFn, EFn>), dim3(g1), dim3(kBlock), 0, stream_, gview(), io, vfn, efn);
         SG_LAUNCHED();
@@ -1898,7 +1899,7 @@ public:
     template <int kMode>
     void mround(const PassIO &io, bool timed) {
         begin_timed(timed);
-        hipLaunchKernelGGL((k_mround<kMode>), dim3(kGridList), dim3(kBlock), 0, stream_, gview(), mstate(), io,
+        hipLaunchKernelGGL((k_mround<kMode>), dim3(round_blocks()), dim3(kBlock), 0, stream_, gview(), mstate(), io,
                            mspill_[io.round % 3], (const MDesc *)mspill_[(io.round + 2) % 3],
                            (uint32_t)std::min<size_t>(mspill_cap_, 0xFFFFFFFFu), stamp_);
         SG_LAUNCHED();

@@ -41,7 +41,7 @@ class Config(C.Structure):
         ("max_rounds", C.c_uint32), ("rounds_per_sync", C.c_uint32), ("flags", C.c_uint32),
         ("device_budget_gb", C.c_double), ("device", C.c_int32),
         ("wave_budget", C.c_uint32), ("heavy_degree", C.c_uint32), ("cache_level", C.c_uint32),
-        ("edge_budget_gb", C.c_double), ("reserved", C.c_uint32 * 2),
+        ("edge_budget_gb", C.c_double), ("round_blocks", C.c_uint32), ("reserved", C.c_uint32 * 1),
     ]
 
 
