@@ -465,6 +465,7 @@ __global__ void __launch_bounds__(kBlock) k_seed_active(uint32_t nv, Op op, cons
                                                         const typename Op::buffer_t *buffer, uint32_t *stamp,
                                                         uint32_t stamp_val, uint32_t *list, uint32_t *cnt,
                                                         unsigned long long *overflow) {
+    __shared__ BlockAppend sh;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nv; base += stride) {
         const uint64_t v = base + threadIdx.x;
@@ -473,7 +474,7 @@ __global__ void __launch_bounds__(kBlock) k_seed_active(uint32_t nv, Op op, cons
             p = op.IsActiveNode((uint32_t)v, buffer[v], value[v]);
             if (p) stamp[v] = stamp_val;
         }
-        wave_append(p, (uint32_t)v, list, cnt, nv, overflow);
+        block_append(p, (uint32_t)v, list, cnt, nv, overflow, sh);
     }
 }
 
@@ -483,6 +484,7 @@ __global__ void __launch_bounds__(kBlock) k_inval_seed(Op op, const unsigned lon
                                                        typename Op::value_t *value, typename Op::buffer_t *buffer,
                                                        uint32_t *inval, uint32_t *coarse, uint32_t *list, uint32_t *cnt,
                                                        uint32_t cap, unsigned long long *stats, unsigned long long *count) {
+    __shared__ BlockAppend sh;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < ndel; base += stride) {
         const uint64_t i = base + threadIdx.x;
@@ -499,12 +501,12 @@ __global__ void __launch_bounds__(kBlock) k_inval_seed(Op op, const unsigned lon
                     typename Op::buffer_t bb = b;
                     op.AccumulateBuffer_del(u, v, &bb, &value[v]);
                     buffer[v] = bb;
-                    atomicAdd(count, 1ull);
                     p = true;
                 }
             }
         }
-        wave_append(p, v, list, cnt, cap, stats + 3);
+        const uint32_t n = block_append(p, v, list, cnt, cap, stats + 3, sh);
+        if (threadIdx.x == 0 && n) atomicAdd(count, (unsigned long long)n);
     }
 }
 
@@ -514,6 +516,7 @@ __global__ void __launch_bounds__(kBlock) k_insert_relax(Op op, BatchView bv, Ba
                                                          uint32_t *stamp, uint32_t stamp_val, uint32_t wmode,
                                                          uint64_t wseed, uint32_t *list, uint32_t *cnt, uint32_t cap,
                                                          unsigned long long *stats) {
+    __shared__ BlockAppend sh;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < bv.nins; base += stride) {
         const uint64_t i = base + threadIdx.x;
@@ -530,7 +533,7 @@ __global__ void __launch_bounds__(kBlock) k_insert_relax(Op op, BatchView bv, Ba
                     p = atomicExch(&stamp[v], stamp_val) != stamp_val;
             }
         }
-        wave_append(p, v, list, cnt, cap, stats + 3);
+        block_append(p, v, list, cnt, cap, stats + 3, sh);
     }
 }
 

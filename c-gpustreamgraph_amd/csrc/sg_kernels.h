@@ -137,6 +137,40 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t item, uint32_t *
         else atomicOr(overflow, 1ull);
     }
 }
+// Block-wide append: one global atomic per block and call instead of one per
+// wave (per-wave atomics on one counter serialise across the chip). Every
+// thread of a kBlock-thread block must call it; returns the block's count.
+struct BlockAppend {
+    uint32_t wsum[kWavesPerBlock];
+    uint32_t base, total;
+};
+__device__ __forceinline__ uint32_t block_append(bool pred, uint32_t item, uint32_t *list, uint32_t *cnt, uint32_t cap,
+                                                 unsigned long long *overflow, BlockAppend &sh) {
+    const uint64_t m = __ballot(pred);
+    const uint32_t w = threadIdx.x / kWave;
+    if (lane_id() == 0) sh.wsum[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < kWavesPerBlock; ++i) {
+            const uint32_t c = sh.wsum[i];
+            sh.wsum[i] = t;  // exclusive offset of each wave
+            t += c;
+        }
+        sh.total = t;
+        sh.base = t ? atomicAdd(cnt, t) : 0u;
+    }
+    __syncthreads();
+    if (pred) {
+        const uint32_t pos = sh.base + sh.wsum[w] + (uint32_t)__popcll(m & lanemask_lt());
+        if (pos < cap) list[pos] = item;
+        else atomicOr(overflow, 1ull);
+    }
+    const uint32_t total = sh.total;
+    __syncthreads();  // the LDS words are reused by the next call
+    return total;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);

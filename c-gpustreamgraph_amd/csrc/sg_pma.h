@@ -90,6 +90,7 @@ struct BatchRanges {
 
 __global__ void __launch_bounds__(kBlock) k_touched(BatchView bv, BatchRanges br, uint32_t *mark, uint32_t *touched,
                                                     uint32_t *tcount, uint32_t cap, unsigned long long *overflow) {
+    __shared__ BlockAppend sh;
     const uint64_t total = (uint64_t)bv.nins + bv.ndel;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < total; base += stride) {
@@ -119,7 +120,7 @@ __global__ void __launch_bounds__(kBlock) k_touched(BatchView bv, BatchRanges br
                 p = !(atomicOr(&mark[u >> 5], bit) & bit);
             }
         }
-        wave_append(p, u, touched, tcount, cap, overflow);
+        block_append(p, u, touched, tcount, cap, overflow, sh);
     }
 }
 
@@ -131,6 +132,8 @@ __global__ void __launch_bounds__(kBlock) k_touch_info(GraphView g, BatchRanges 
                                                        const uint32_t *tcount, TInfo *tinfo, uint32_t *tidx,
                                                        MChunk *chunks, uint32_t *nchunks, uint32_t chunk_cap,
                                                        unsigned long long *seqcur, unsigned long long *overflow) {
+    __shared__ uint32_t s_c[kWavesPerBlock], s_cb;
+    __shared__ unsigned long long s_s[kWavesPerBlock], s_sb;
     const uint32_t n = *tcount;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -147,18 +150,35 @@ __global__ void __launch_bounds__(kBlock) k_touch_info(GraphView g, BatchRanges 
             nch = chunked(ti) ? (seq_len(ti) + kMergeChunk - 1) / kMergeChunk : 0;
             ti.nch = nch;
         }
+        // chunk indexes and staging ranges: wave scans, then one atomic per
+        // counter and block
         const uint32_t incl = wave_incl_scan(nch);
-        const uint32_t tot = __shfl(incl, kWave - 1, kWave);
-        uint32_t cb = 0;
-        if (lane_id() == 0 && tot) cb = atomicAdd(nchunks, tot);
-        cb = __shfl(cb, 0, kWave) + incl - nch;
-        // staging range of the filtered sequence (vertices with deletes)
         const unsigned long long sl = (t < n && has_del(ti)) ? seq_len(ti) : 0ull;
         const unsigned long long sincl = wave_incl_scan(sl);
-        const unsigned long long stot = __shfl(sincl, kWave - 1, kWave);
-        unsigned long long sb = 0;
-        if (lane_id() == 0 && stot) sb = atomicAdd(seqcur, stot);
-        sb = __shfl(sb, 0, kWave) + sincl - sl;
+        const uint32_t w = threadIdx.x / kWave;
+        if (lane_id() == kWave - 1) {
+            s_c[w] = incl;
+            s_s[w] = sincl;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t ct = 0;
+            unsigned long long st = 0;
+            for (int i = 0; i < kWavesPerBlock; ++i) {
+                const uint32_t c = s_c[i];
+                const unsigned long long q = s_s[i];
+                s_c[i] = ct;
+                s_s[i] = st;
+                ct += c;
+                st += q;
+            }
+            s_cb = ct ? atomicAdd(nchunks, ct) : 0u;
+            s_sb = st ? atomicAdd(seqcur, st) : 0ull;
+        }
+        __syncthreads();
+        const uint32_t cb = s_cb + s_c[w] + incl - nch;
+        const unsigned long long sb = s_sb + s_s[w] + sincl - sl;
+        __syncthreads();
         if (t < n) {
             ti.cbase = cb;
             if (cb + nch > chunk_cap) {  // never by construction; flagged, and no chunk is read
