@@ -247,6 +247,8 @@ __device__ __forceinline__ uint64_t off_at(const uint64_t *off, uint32_t nv, uin
 __global__ void __launch_bounds__(kBlock) k_newdeg(GraphView g, const TInfo *tinfo, const uint32_t *tcount,
                                                    const unsigned long long *cpos, uint32_t *newdeg, uint8_t *leaf_dirty,
                                                    uint32_t L, unsigned long long *leaf_cnt, Ctl *ctl) {
+    __shared__ unsigned long long s_sum[kWavesPerBlock];
+    __shared__ bool s_over[kWavesPerBlock];
     const uint32_t n = *tcount;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -264,9 +266,26 @@ __global__ void __launch_bounds__(kBlock) k_newdeg(GraphView g, const TInfo *tin
             delta = (long long)nd - (long long)ti.d0;
             if (delta) atomicAdd(&leaf_cnt[ti.u / L], (unsigned long long)delta);
         }
+        // one atomic per counter and block
         const unsigned long long wsum = wave_sum_u64((unsigned long long)delta);
-        if (lane_id() == 0 && wsum) atomicAdd(&ctl->ne_delta, wsum);
-        if (__ballot(over) && lane_id() == 0) atomicOr(&ctl->overflow, 1ull);
+        const bool wover = __ballot(over) != 0;
+        const uint32_t w = threadIdx.x / kWave;
+        if (lane_id() == 0) {
+            s_sum[w] = wsum;
+            s_over[w] = wover;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long a = 0;
+            bool o = false;
+            for (int i = 0; i < kWavesPerBlock; ++i) {
+                a += s_sum[i];
+                o = o || s_over[i];
+            }
+            if (a) atomicAdd(&ctl->ne_delta, a);
+            if (o) atomicOr(&ctl->overflow, 1ull);
+        }
+        __syncthreads();
     }
 }
 
@@ -505,8 +524,7 @@ __global__ void __launch_bounds__(kBlock) k_inval_seed(Op op, const unsigned lon
                 }
             }
         }
-        const uint32_t n = block_append(p, v, list, cnt, cap, stats + 3, sh);
-        if (threadIdx.x == 0 && n) atomicAdd(count, (unsigned long long)n);
+        block_append(p, v, list, cnt, cap, stats + 3, sh);
     }
 }
 
@@ -726,6 +744,7 @@ public:
     uint32_t *hcnt_ = nullptr;   // [3] rotating spill counters
     uint32_t *rlog_ = nullptr;
     uint64_t timed_edges_ = 0, timed_verts_ = 0, timed_cedges_ = 0;  // host totals of the timed-round stats
+    uint64_t last_appended_ = 0;  // vertices the last run of rounds (and its seeds) put on frontiers
     HeavyDesc *heavy_[3] = {nullptr, nullptr, nullptr};  // rotating spill queues
     size_t heavy_cap_ = 0;
     unsigned long long *stats_ = nullptr;  // [0..3] pass stats, [4] inval count
@@ -1149,6 +1168,8 @@ public:
         const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
         if (timed) fill<unsigned long long>(stats_ + 8, 8, 0ull);
         uint32_t rounds = 0;
+        bool first = true;
+        last_appended_ = 0;
         for (;;) {
             const uint32_t R0 = R_;
             for (uint32_t i = 0; i < k; ++i) {
@@ -1161,6 +1182,10 @@ public:
             }
             read_block();
             if (do_time) harvest_events();
+            if (first) {  // what the seed passes put on the first frontier (logged by the first round)
+                last_appended_ += h_blk_[8 + 2 * ((R0 + kRoundLog - 1) % kRoundLog)];
+                first = false;
+            }
             for (uint32_t i = 0; i < k; ++i) {
                 ++rounds;
                 const uint32_t R = R0 + i;
@@ -1172,6 +1197,7 @@ public:
                     front = h_blk_[(R + 1) % 3];
                     spill = h_blk_[3 + R % 3];
                 }
+                last_appended_ += front;
                 if (front == 0 && spill == 0) {
                     finish_rounds(timed);
                     return rounds;
@@ -1666,7 +1692,8 @@ public:
             }, false);
         }
         // the invalidation rounds' last read of the control block holds the count
-        const uint64_t ninval = ndel_ ? h_stats()[4] : 0;
+        // every vertex the invalidation seeds and rounds appended was reset once
+        const uint64_t ninval = ndel_ ? last_appended_ : 0;
         st_.invalidated = ninval;
         // 2. seed the recompute: edges into invalidated vertices + inserted edges
         frontier_reset();
@@ -2001,7 +2028,7 @@ public:
         if (ndel_) {
             mpairs<1>(ndel_, R_, nullptr);
             mrounds<1>(false);
-            ninval = h_stats()[4];
+            ninval = last_appended_;  // vertices put on the invalidation frontiers (> 0: some slot reset)
         }
         st_.invalidated = ninval;
         // 2. seed the recompute: live edges into reset slots (one sweep for all queries) + inserted edges

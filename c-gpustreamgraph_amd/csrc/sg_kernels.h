@@ -572,8 +572,7 @@ struct EInvalidate {
         typename Op::buffer_t bb = b;
         op.AccumulateBuffer_del(u, v, &bb, &value[v]);
         buffer[v] = bb;
-        atomicAdd(count, 1ull);
-        return true;
+        return true;  // counted by the frontier it joins
     }
 };
 

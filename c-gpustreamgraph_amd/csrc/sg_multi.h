@@ -113,7 +113,6 @@ __device__ __forceinline__ void reset_children(const MultiState &ms, const PassI
     fresh = __shfl(fresh, (int)(q.sub * ms.QS), kWave);
     bool app = false;
     if (q.slot == 0 && fresh) {
-        atomicAdd(io.stats + 4, (unsigned long long)__popcll(fresh));
         atomicOr(&ms.ninv[(uint64_t)v * ms.G + grp], fresh);
         atomicOr(&ms.invbit[v >> 5], 1u << (v & 31));
         atomicOr(&ms.coarse[v >> 11], 1u << ((v >> 6) & 31));
