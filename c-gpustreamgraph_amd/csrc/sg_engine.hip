@@ -1027,8 +1027,11 @@ public:
         SG_LAUNCHED();
         hipLaunchKernelGGL(k_deg_scan_sums, dim3(1), dim3(1024), 0, stream_, nb, bsum_);
         SG_LAUNCHED();
-        hipLaunchKernelGGL(k_deg_scan_write, dim3(nb), dim3(kScanBlock), 0, stream_, nv_, deg_, bsum_, dpre_, t,
-                           stats_ + 3);
+        hipLaunchKernelGGL(k_deg_scan_write, dim3(nb), dim3(kScanBlock), 0, stream_, nv_, deg_, bsum_, dpre_);
+        SG_LAUNCHED();
+        const uint64_t ntiles = ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile;
+        hipLaunchKernelGGL(k_sweep_tiles, dim3(grid_for(ntiles + 1, 256, 16384)), dim3(256), 0, stream_, nv_,
+                           (const unsigned long long *)dpre_, t, stats_ + 3);
         SG_LAUNCHED();
         return t;
     }

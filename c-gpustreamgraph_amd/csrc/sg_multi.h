@@ -430,115 +430,77 @@ __global__ void __launch_bounds__(kBlock) k_mpairs(MultiState ms, BatchView bv, 
 }
 
 // Edge-centric sweep for the batch: every live edge (u, v) whose target has a
-// reset slot becomes a hit (u << 32 | v), appended in wave-sized groups.
-// Same merge-path tiles as k_sweep_edges.
+// reset slot becomes a hit (u << 32 | v). Same wave tiles as k_sweep_edges;
+// hits are gathered per wave in LDS and appended kHitStage at a time (one
+// atomic per flush).
+constexpr uint32_t kHitStage = 512;
+
+__device__ __forceinline__ void hit_flush(unsigned long long *stage, uint32_t &n, unsigned long long *hits,
+                                          uint32_t *nhits, uint32_t hit_cap, unsigned long long *overflow) {
+    if (n == 0) return;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(nhits, n);
+    base = __shfl(base, 0, kWave);
+    for (uint32_t i = lane_id(); i < n; i += kWave) {
+        if (base + i < hit_cap) hits[base + i] = stage[i];
+        else atomicOr(overflow, 8ull);
+    }
+    n = 0;
+}
+
 __global__ void __launch_bounds__(kBlock) k_msweep(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
                                                    const uint32_t *invbit, const uint32_t *coarse,
                                                    unsigned long long *hits, uint32_t *nhits, uint32_t hit_cap,
                                                    unsigned long long *overflow) {
-    __shared__ uint16_t s_own[kSweepTile];
-    __shared__ int32_t s_base[kSweepTile + 1];
-    __shared__ uint32_t s_w[kWavesPerBlock];
-    __shared__ uint32_t s_wide;
-    __shared__ unsigned long long s_hits[kSweepTile];
-    __shared__ uint32_t s_nh, s_gbase;
+    __shared__ SweepWave s_wave[kWavesPerBlock];
+    __shared__ unsigned long long s_hits[kWavesPerBlock][kHitStage];
+    __shared__ uint32_t s_cnt[kWavesPerBlock], s_base;
     const unsigned long long E = dpre[g.nv];
     const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
-    if (threadIdx.x == 0) s_nh = 0;
-    __syncthreads();
-    for (unsigned long long t = blockIdx.x; t < ntiles && t + 1 < tiles.cap; t += gridDim.x) {
-        const uint32_t v0 = tiles.v[t], v1 = tiles.v[t + 1];
-        const unsigned long long e0 = tiles.e[t], e1 = tiles.e[t + 1];
-        const uint32_t ne = (uint32_t)min<unsigned long long>(e1 - e0, kSweepTile);
-        const uint32_t span = min<uint32_t>(v1 - v0 + 1, kSweepTile + 1);
-        const uint64_t off0 = g.off[v0];
-        if (threadIdx.x == 0) s_wide = (g.off[v0 + span] - off0) >= 0x40000000ull;
+    SweepWave &sh = s_wave[w];
+    uint32_t nst = 0;  // wave-uniform fill of this wave's hit stage
+    unsigned long long n_edges = 0;
+    const unsigned long long gw = (unsigned long long)blockIdx.x * kWavesPerBlock + w;
+    const unsigned long long nw = (unsigned long long)gridDim.x * kWavesPerBlock;
+    for (unsigned long long t = gw; t < ntiles && t + 1 < tiles.cap; t += nw) {
+        uint32_t vv[kSweepPerLane];
+        const uint32_t v0 = sweep_tile(g, dpre, tiles, t, g.col, 0, 0, g.nv, sh, vv, n_edges);
 #pragma unroll
-        for (int j = 0; j < kSweepPerThread; ++j) s_own[j * kBlock + threadIdx.x] = 0;
-        __syncthreads();
-        const bool wide = s_wide;
-        for (uint32_t i = threadIdx.x; i < span; i += kBlock) {
-            const uint32_t v = v0 + i;
-            const unsigned long long a = dpre[v], b = dpre[v + 1];
-            if (!wide) s_base[i] = (int32_t)((long long)(g.off[v] - off0) - (long long)(a - e0));
-            if (b > a && a >= e0 && a < e0 + ne) s_own[a - e0] = (uint16_t)i;
-        }
-        __syncthreads();
-        uint32_t own[kSweepPerThread];
-        uint32_t m = 0;
-#pragma unroll
-        for (int j = 0; j < kSweepPerThread; ++j) {
-            own[j] = s_own[threadIdx.x * kSweepPerThread + j];
-            m = max(m, own[j]);
-        }
-        uint32_t x = m;
-#pragma unroll
-        for (int d = 1; d < kWave; d <<= 1) {
-            const uint32_t y = __shfl_up(x, d, kWave);
-            if (lane >= (uint32_t)d) x = max(x, y);
-        }
-        if (lane == kWave - 1) s_w[w] = x;
-        uint32_t prev = __shfl_up(x, 1, kWave);
-        if (lane == 0) prev = 0;
-        __syncthreads();
-        for (uint32_t i = 0; i < w; ++i) prev = max(prev, s_w[i]);
-#pragma unroll
-        for (int j = 0; j < kSweepPerThread; ++j) {
-            prev = max(prev, own[j]);
-            s_own[threadIdx.x * kSweepPerThread + j] = (uint16_t)prev;
-        }
-        __syncthreads();
-        uint32_t vv[kSweepPerThread];
-#pragma unroll
-        for (int j = 0; j < kSweepPerThread; ++j) {
-            const uint32_t p = j * kBlock + threadIdx.x;
-            vv[j] = kGap;
-            if (p < ne) {
-                const uint32_t o = s_own[p];
-                uint64_t slot;
-                if (!wide) {
-                    slot = off0 + (uint64_t)((long long)s_base[o] + (long long)p);
-                } else {
-                    const uint32_t v = v0 + o;
-                    slot = g.off[v] + (e0 + p - dpre[v]);
-                }
-                vv[j] = g.col[slot];
-            }
-        }
-        // hits are gathered in LDS (a tile has at most kSweepTile) and
-        // appended with one global atomic per tile
-#pragma unroll
-        for (int j = 0; j < kSweepPerThread; ++j) {
+        for (int j = 0; j < kSweepPerLane; ++j) {
             const uint32_t v = vv[j];
             bool hit = false;
             if (v != kGap && ((coarse[v >> 11] >> ((v >> 6) & 31)) & 1u)) hit = (invbit[v >> 5] >> (v & 31)) & 1u;
             const uint64_t bm = __ballot(hit);
             if (bm) {
-                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)bm) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(&s_nh, (uint32_t)__popcll(bm));
-                base = __shfl(base, (int)leader, kWave);
+                const uint32_t c = (uint32_t)__popcll(bm);
+                if (nst + c > kHitStage) hit_flush(s_hits[w], nst, hits, nhits, hit_cap, overflow);
                 if (hit) {
-                    const uint32_t src = v0 + s_own[j * kBlock + threadIdx.x];
-                    s_hits[base + (uint32_t)__popcll(bm & lanemask_lt())] = ((unsigned long long)src << 32) | v;
+                    const uint32_t src = v0 + sh.own[j * kWave + lane];
+                    s_hits[w][nst + (uint32_t)__popcll(bm & lanemask_lt())] = ((unsigned long long)src << 32) | v;
                 }
+                nst += c;
             }
         }
-        __syncthreads();
-        const uint32_t nh = s_nh;
-        if (nh) {
-            if (threadIdx.x == 0) s_gbase = atomicAdd(nhits, nh);
-            __syncthreads();
-            const uint32_t gb = s_gbase;
-            for (uint32_t i = threadIdx.x; i < nh; i += kBlock) {
-                if (gb + i < hit_cap) hits[gb + i] = s_hits[i];
-                else atomicOr(overflow, 8ull);
-            }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // the partly filled stages of the block's waves: one atomic for all of them
+    if (lane == 0) s_cnt[w] = nst;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int i = 0; i < kWavesPerBlock; ++i) {
+            const uint32_t c = s_cnt[i];
+            s_cnt[i] = tot;
+            tot += c;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) s_nh = 0;
-        __syncthreads();
+        s_base = tot ? atomicAdd(nhits, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t b0 = s_base + s_cnt[w];
+    for (uint32_t i = lane; i < nst; i += kWave) {
+        if (b0 + i < hit_cap) hits[b0 + i] = s_hits[w][i];
+        else atomicOr(overflow, 8ull);
     }
 }
 

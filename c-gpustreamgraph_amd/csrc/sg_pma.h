@@ -365,7 +365,7 @@ __global__ void k_lane_place(GraphView g, BatchView bv, uint32_t *col, const TIn
 // edge of vertex v, the position after its last edge is v's end mark. A tile
 // is kSweepTile consecutive path positions, so it holds at most kSweepTile
 // edges and kSweepTile + 1 sources whatever the degree distribution.
-constexpr uint32_t kSweepTile = 2048;
+constexpr uint32_t kSweepTile = 512;   // path positions per tile (one wave)
 constexpr uint32_t kScanBlock = 256;
 constexpr uint32_t kScanIters = 16;
 constexpr uint32_t kScanSpan = kScanBlock * kScanIters;  // vertices per block of the degree scan
@@ -434,13 +434,9 @@ __global__ void __launch_bounds__(1024) k_deg_scan_sums(uint32_t nb, unsigned lo
     }
 }
 
-// Phase 3: dpre[v] for v in [0, nv] and the merge-path tile boundaries:
-// tile t starts at path position d = t * kSweepTile, inside vertex v with
-// v + dpre[v] <= d < v + 1 + dpre[v + 1], at edge min(d - v, dpre[v + 1]).
-// Tile ntiles (= ceil((nv + E) / kSweepTile)) is the sentinel {nv - 1, E}.
+// Phase 3: dpre[v] for v in [0, nv].
 __global__ void __launch_bounds__(kScanBlock) k_deg_scan_write(uint32_t nv, const uint32_t *deg,
-                                                               const unsigned long long *bsum, unsigned long long *dpre,
-                                                               SweepTiles tiles, unsigned long long *overflow) {
+                                                               const unsigned long long *bsum, unsigned long long *dpre) {
     __shared__ unsigned long long s_w[2][kScanBlock / kWave];
     const uint64_t base = (uint64_t)blockIdx.x * kScanSpan;
     unsigned long long carry = bsum[blockIdx.x];
@@ -452,149 +448,172 @@ __global__ void __launch_bounds__(kScanBlock) k_deg_scan_write(uint32_t nv, cons
         carry += total;
         if (v < nv) {
             dpre[v] = a;
-            const unsigned long long b = a + d;
-            const unsigned long long p0 = v + a, p1 = v + 1 + b;  // path range of v
-            for (unsigned long long t = (p0 + kSweepTile - 1) / kSweepTile; t * kSweepTile < p1; ++t) {
-                if (t >= tiles.cap - 1) { atomicOr(overflow, 4ull); break; }
-                tiles.v[t] = (uint32_t)v;
-                tiles.e[t] = min(t * kSweepTile - v, b);
-            }
-            if (v + 1 == nv) {
-                dpre[nv] = b;
-                const unsigned long long nt = (p1 + kSweepTile - 1) / kSweepTile;
-                if (nt < tiles.cap) {
-                    tiles.v[nt] = (uint32_t)v;
-                    tiles.e[nt] = b;
-                } else {
-                    atomicOr(overflow, 4ull);
-                }
-            }
+            if (v + 1 == nv) dpre[nv] = a + d;
         }
     }
 }
 
-// One block per tile (grid-stride). Each source of the tile gets its slot base
-// relative to the tile's first source (off[v] - off[v0] - (dpre[v] - e0), 32
-// bits) in LDS, the first edge of every source with edges is marked with its
-// 16-bit index, a block max-scan fills the owners forward, and the tile's
-// edges are loaded coalesced, kSweepPerThread per thread, all in flight. A
-// tile whose slot range does not fit 32 bits takes the slot of each edge from
-// off and dpre in global memory instead.
-constexpr int kSweepPerThread = kSweepTile / kBlock;
-constexpr int kSweepGroup = 8;         // bitmap probes in flight per thread
+// Phase 4, one thread per tile: tile t starts at path position d = t *
+// kSweepTile, inside the vertex v with v + dpre[v] <= d < v + 1 + dpre[v + 1]
+// (the largest v with v + dpre[v] <= d), at edge min(d - v, dpre[v + 1]).
+// Tile ntiles = ceil((nv + E) / kSweepTile) is the sentinel {nv - 1, E}.
+__global__ void k_sweep_tiles(uint32_t nv, const unsigned long long *dpre, SweepTiles tiles, unsigned long long *overflow) {
+    const unsigned long long E = dpre[nv];
+    const unsigned long long nt = ((unsigned long long)nv + E + kSweepTile - 1) / kSweepTile;
+    if (nt >= tiles.cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(overflow, 4ull);
+        return;
+    }
+    for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t <= nt;
+         t += (unsigned long long)gridDim.x * blockDim.x) {
+        if (t == nt) {
+            tiles.v[t] = nv - 1;
+            tiles.e[t] = E;
+            continue;
+        }
+        const unsigned long long d = t * kSweepTile;
+        uint32_t lo = 0, hi = nv - 1;  // largest v in [0, nv - 1] with v + dpre[v] <= d
+        while (lo < hi) {
+            const uint32_t m = lo + (hi - lo + 1) / 2;
+            if ((unsigned long long)m + dpre[m] <= d) lo = m; else hi = m - 1;
+        }
+        tiles.v[t] = lo;
+        tiles.e[t] = min(d - lo, dpre[lo + 1]);
+    }
+}
+
+// One wave per tile (grid-stride over waves, no block-wide barrier, so the
+// waves of a block overlap their memory phases). Each source of the tile gets
+// its slot base relative to the tile's first source (off[v] - off[v0] -
+// (dpre[v] - e0), 32 bits) in the wave's LDS, the first edge of every source
+// with edges is marked with its 16-bit index, a wave max-scan fills the owners
+// forward, and the tile's edges are loaded coalesced, kSweepPerLane per lane,
+// all in flight. A tile whose slot range does not fit 32 bits takes the slot
+// of each edge from off and dpre in global memory instead.
+constexpr int kSweepPerLane = kSweepTile / kWave;
 constexpr uint32_t kSweepOutStage = 128;
+
+struct SweepWave {
+    uint16_t own[kSweepTile];
+    int32_t base[kSweepTile + 1];
+    uint32_t out[kSweepOutStage];
+};
 
 // colp / slot_lo: where slot s of the tile's sources is read (colp[s - slot_lo]:
 // the edge array itself, or one streamed segment of it); only sources in
-// [src_lo, src_hi) and tiles in [t_begin, t_end) are walked.
+// [src_lo, src_hi) are walked. vv[j] = target at tile position j * 64 + lane
+// (kGap when none); the owner index of a position stays in sh.own.
+__device__ __forceinline__ uint32_t sweep_tile(const GraphView &g, const unsigned long long *dpre,
+                                               const SweepTiles &tiles, unsigned long long t, const uint32_t *colp,
+                                               uint64_t slot_lo, uint32_t src_lo, uint32_t src_hi, SweepWave &sh,
+                                               uint32_t vv[kSweepPerLane], unsigned long long &n_edges) {
+    const uint32_t lane = lane_id();
+    const uint32_t v0 = tiles.v[t], v1 = tiles.v[t + 1];
+    const unsigned long long e0 = tiles.e[t], e1 = tiles.e[t + 1];
+    const uint32_t ne = (uint32_t)min<unsigned long long>(e1 - e0, kSweepTile);
+    const uint32_t span = min<uint32_t>(v1 - v0 + 1, kSweepTile + 1);
+    const uint64_t off0 = g.off[v0];
+    const bool wide = (g.off[v0 + span] - off0) >= 0x40000000ull;
+#pragma unroll
+    for (int j = 0; j < kSweepPerLane; ++j) sh.own[j * kWave + lane] = 0;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = lane; i < span; i += kWave) {
+        const uint32_t v = v0 + i;
+        const unsigned long long a = dpre[v], b = dpre[v + 1];
+        if (!wide) sh.base[i] = (int32_t)((long long)(g.off[v] - off0) - (long long)(a - e0));
+        if (b > a && a >= e0 && a < e0 + ne) sh.own[a - e0] = (uint16_t)i;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // forward fill of the owners: lane k holds positions [k * P, k * P + P)
+    uint32_t own[kSweepPerLane];
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < kSweepPerLane; ++j) {
+        own[j] = sh.own[lane * kSweepPerLane + j];
+        m = max(m, own[j]);
+    }
+    uint32_t x = m;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, kWave);
+        if (lane >= (uint32_t)d) x = max(x, y);
+    }
+    uint32_t prev = __shfl_up(x, 1, kWave);
+    if (lane == 0) prev = 0;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int j = 0; j < kSweepPerLane; ++j) {
+        prev = max(prev, own[j]);
+        sh.own[lane * kSweepPerLane + j] = (uint16_t)prev;
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int j = 0; j < kSweepPerLane; ++j) {
+        const uint32_t p = j * kWave + lane;
+        vv[j] = kGap;
+        if (p < ne) {
+            const uint32_t o = sh.own[p];
+            const uint32_t src = v0 + o;
+            if (src >= src_lo && src < src_hi) {
+                ++n_edges;
+                const uint64_t slot = !wide ? off0 + (uint64_t)((long long)sh.base[o] + (long long)p)
+                                            : g.off[src] + (e0 + p - dpre[src]);
+                vv[j] = colp[slot - slot_lo];
+            }
+        }
+    }
+    return v0;
+}
+
+// Block-wide sum of a per-lane counter, one atomic per block.
+__device__ __forceinline__ void block_count(unsigned long long x, unsigned long long *s_red, unsigned long long *dst) {
+    x = wave_sum_u64(x);
+    if (lane_id() == 0) s_red[threadIdx.x / kWave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < kWavesPerBlock; ++i) t += s_red[i];
+        if (t) atomicAdd(dst, t);
+    }
+}
+
+// Tiles in [t_begin, t_end) are walked.
 template <class Op, class EFn>
 __global__ void __launch_bounds__(kBlock) k_sweep_edges(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
                                                         const uint32_t *colp, uint64_t slot_lo, uint32_t src_lo,
                                                         uint32_t src_hi, uint64_t t_begin, uint64_t t_end, Op op,
                                                         const typename Op::buffer_t *buffer, EFn efn, PassIO io) {
-    __shared__ uint16_t s_own[kSweepTile];
-    __shared__ int32_t s_base[kSweepTile + 1];
-    __shared__ uint32_t s_out[kWavesPerBlock][kSweepOutStage];
-    __shared__ uint32_t s_w[kWavesPerBlock];
-    __shared__ uint32_t s_wide;
+    __shared__ SweepWave s_wave[kWavesPerBlock];
     __shared__ unsigned long long s_red[kWavesPerBlock];
     const unsigned long long E = dpre[g.nv];
     const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
-    OutStage st{s_out[w], 0, kSweepOutStage};
+    SweepWave &sh = s_wave[w];
+    OutStage st{sh.out, 0, kSweepOutStage};
     unsigned long long n_edges = 0;
     const unsigned long long tend = min<unsigned long long>(t_end, ntiles);
-    for (unsigned long long t = t_begin + blockIdx.x; t < tend && t + 1 < tiles.cap; t += gridDim.x) {
-        const uint32_t v0 = tiles.v[t], v1 = tiles.v[t + 1];
-        const unsigned long long e0 = tiles.e[t], e1 = tiles.e[t + 1];
-        const uint32_t ne = (uint32_t)min<unsigned long long>(e1 - e0, kSweepTile);
-        const uint32_t span = min<uint32_t>(v1 - v0 + 1, kSweepTile + 1);
-        const uint64_t off0 = g.off[v0];
-        if (threadIdx.x == 0) s_wide = (g.off[v0 + span] - off0) >= 0x40000000ull;
+    const unsigned long long gw = (unsigned long long)blockIdx.x * kWavesPerBlock + w;
+    const unsigned long long nw = (unsigned long long)gridDim.x * kWavesPerBlock;
+    for (unsigned long long t = t_begin + gw; t < tend && t + 1 < tiles.cap; t += nw) {
+        uint32_t vv[kSweepPerLane];
+        const uint32_t v0 = sweep_tile(g, dpre, tiles, t, colp, slot_lo, src_lo, src_hi, sh, vv, n_edges);
+        uint64_t peek[kSweepPerLane];
 #pragma unroll
-        for (int j = 0; j < kSweepPerThread; ++j) s_own[j * kBlock + threadIdx.x] = 0;
-        __syncthreads();
-        const bool wide = s_wide;
-        for (uint32_t i = threadIdx.x; i < span; i += kBlock) {
-            const uint32_t v = v0 + i;
-            const unsigned long long a = dpre[v], b = dpre[v + 1];
-            if (!wide) s_base[i] = (int32_t)((long long)(g.off[v] - off0) - (long long)(a - e0));
-            if (b > a && a >= e0 && a < e0 + ne) s_own[a - e0] = (uint16_t)i;
-        }
-        __syncthreads();
-        // forward fill of the owners: thread k holds positions [k * P, k * P + P)
-        uint32_t own[kSweepPerThread];
-        uint32_t m = 0;
+        for (int j = 0; j < kSweepPerLane; ++j) peek[j] = vv[j] != kGap ? efn.load(vv[j]) : 0;
 #pragma unroll
-        for (int j = 0; j < kSweepPerThread; ++j) {
-            own[j] = s_own[threadIdx.x * kSweepPerThread + j];
-            m = max(m, own[j]);
-        }
-        uint32_t x = m;
-#pragma unroll
-        for (int d = 1; d < kWave; d <<= 1) {
-            const uint32_t y = __shfl_up(x, d, kWave);
-            if (lane >= (uint32_t)d) x = max(x, y);
-        }
-        if (lane == kWave - 1) s_w[w] = x;
-        uint32_t prev = __shfl_up(x, 1, kWave);
-        if (lane == 0) prev = 0;
-        __syncthreads();
-        for (uint32_t i = 0; i < w; ++i) prev = max(prev, s_w[i]);
-#pragma unroll
-        for (int j = 0; j < kSweepPerThread; ++j) {
-            prev = max(prev, own[j]);
-            s_own[threadIdx.x * kSweepPerThread + j] = (uint16_t)prev;
-        }
-        __syncthreads();
-        uint32_t vv[kSweepPerThread];
-#pragma unroll
-        for (int j = 0; j < kSweepPerThread; ++j) {
-            const uint32_t p = j * kBlock + threadIdx.x;
-            vv[j] = kGap;
-            if (p < ne) {
-                const uint32_t o = s_own[p];
-                const uint32_t src = v0 + o;
-                if (src >= src_lo && src < src_hi) {
-                    ++n_edges;
-                    uint64_t slot;
-                    if (!wide) {
-                        slot = off0 + (uint64_t)((long long)s_base[o] + (long long)p);
-                    } else {
-                        slot = g.off[src] + (e0 + p - dpre[src]);
-                    }
-                    vv[j] = colp[slot - slot_lo];
-                }
+        for (int j = 0; j < kSweepPerLane; ++j) {
+            const uint32_t v = vv[j];
+            bool enq = false;
+            if (v != kGap && ((peek[j] >> (v & 31)) & 1u)) {
+                const uint32_t src = v0 + sh.own[j * kWave + lane];
+                enq = efn.apply(src, v, op.PushValue(ld_peek(&buffer[src])), peek[j]);
             }
+            stage_push(enq, v, st, io);
         }
-#pragma unroll
-        for (int j0 = 0; j0 < kSweepPerThread; j0 += kSweepGroup) {
-            uint64_t peek[kSweepGroup];
-#pragma unroll
-            for (int j = 0; j < kSweepGroup; ++j) peek[j] = vv[j0 + j] != kGap ? efn.load(vv[j0 + j]) : 0;
-#pragma unroll
-            for (int j = 0; j < kSweepGroup; ++j) {
-                const uint32_t v = vv[j0 + j];
-                bool enq = false;
-                if (v != kGap && ((peek[j] >> (v & 31)) & 1u)) {
-                    const uint32_t src = v0 + s_own[(j0 + j) * kBlock + threadIdx.x];
-                    enq = efn.apply(src, v, op.PushValue(ld_peek(&buffer[src])), peek[j]);
-                }
-                stage_push(enq, v, st, io);
-            }
-        }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
     }
     stage_flush(st, io);
-    // one counter atomic per block (per-wave atomics on one address serialise)
-    n_edges = wave_sum_u64(n_edges);
-    if (lane_id() == 0) s_red[w] = n_edges;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int i = 0; i < kWavesPerBlock; ++i) t += s_red[i];
-        if (t) atomicAdd(io.stats + 0, t);
-    }
+    block_count(n_edges, s_red, io.stats + 0);
 }
 
 // Invalidated vertices that are active on their own (CC: own label) join the
