@@ -30,6 +30,7 @@
 #include <memory>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "sgraph.h"
@@ -501,8 +502,8 @@ __global__ void __launch_bounds__(kBlock) k_seed_active(uint32_t nv, Op op, cons
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_inval_seed(Op op, const unsigned long long *del_k, uint32_t ndel,
                                                        typename Op::value_t *value, typename Op::buffer_t *buffer,
-                                                       uint32_t *inval, uint32_t *coarse, uint32_t *list, uint32_t *cnt,
-                                                       uint32_t cap, unsigned long long *stats, unsigned long long *count) {
+                                                       uint32_t *inval, uint32_t *coarse, uint32_t cshift, uint32_t *list,
+                                                       uint32_t *cnt, uint32_t cap, unsigned long long *stats) {
     __shared__ BlockAppend sh;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < ndel; base += stride) {
@@ -516,7 +517,7 @@ __global__ void __launch_bounds__(kBlock) k_inval_seed(Op op, const unsigned lon
             if (Op::Parent(b) == u) {
                 const uint32_t bit = 1u << (v & 31);
                 if (!(atomicOr(&inval[v >> 5], bit) & bit)) {
-                    atomicOr(&coarse[v >> 11], 1u << ((v >> 6) & 31));
+                    coarse_mark(coarse, cshift, v);
                     typename Op::buffer_t bb = b;
                     op.AccumulateBuffer_del(u, v, &bb, &value[v]);
                     buffer[v] = bb;
@@ -750,7 +751,9 @@ public:
     unsigned long long *stats_ = nullptr;  // [0..3] pass stats, [4] inval count
     uint32_t *stamp_ = nullptr;
     uint32_t *inval_ = nullptr;   // bitmap of invalidated vertices
-    uint32_t *coarse_ = nullptr;  // one bit per 64 vertices of inval_
+    uint32_t *coarse_ = nullptr;  // coarse filter of inval_: one bit per 2^cshift_ vertices
+    uint32_t cshift_ = 6;
+    std::unordered_map<const void *, int> occ_;  // resident blocks per CU of a kernel
     int num_cus_ = 0;
     uint32_t R_ = 1;             // global round counter (stamps)
 
@@ -921,7 +924,8 @@ public:
         stats_ = (unsigned long long *)(cnt_ + 8 + 2 * kRoundLog);
         stamp_ = dalloc<uint32_t>(nv);
         inval_ = dalloc<uint32_t>(((size_t)nv + 31) / 32);
-        coarse_ = dalloc<uint32_t>(((size_t)nv + 2047) / 2048);
+        cshift_ = coarse_shift_for(nv);
+        coarse_ = dalloc<uint32_t>(coarse_words(nv, cshift_));
         mark_ = dalloc<uint32_t>(((size_t)nv + 31) / 32);
         tidx_ = dalloc<uint32_t>(nv);
         br_.ins_b = dalloc<uint32_t>(nv);
@@ -1034,6 +1038,24 @@ public:
                            (const unsigned long long *)dpre_, t, stats_ + 3);
         SG_LAUNCHED();
         return t;
+    }
+    // Blocks of a sweep kernel for ntiles tiles: one wave per tile, at most as
+    // many blocks as fit on the device at once (the kernels stride over tiles,
+    // so every block stays resident and loads its LDS filter once).
+    template <class K>
+    uint32_t sweep_grid(K kernel, uint64_t ntiles) {
+        const void *key = reinterpret_cast<const void *>(kernel);
+        auto it = occ_.find(key);
+        int per_cu = 0;
+        if (it == occ_.end()) {
+            SG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kSweepBlock, 0));
+            per_cu = std::max(per_cu, 1);
+            occ_[key] = per_cu;
+        } else {
+            per_cu = it->second;
+        }
+        const uint64_t resident = (uint64_t)per_cu * (uint64_t)std::max(num_cus_, 1);
+        return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ntiles + kSweepWaves - 1) / kSweepWaves, resident));
     }
     uint32_t heavy_deg() const { return std::max<uint32_t>(kChunk, cfg_.heavy_degree ? cfg_.heavy_degree : kHeavyDeg); }
 
@@ -1513,9 +1535,9 @@ public:
     void sweep_all(const SweepTiles &tl, Op op, const typename Op::buffer_t *buffer, EFn ef, const PassIO &io) {
         const uint64_t ntiles = ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile;
         if (!host_edges_) {
-            const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, 16384));
+            const uint32_t grid = sweep_grid(k_sweep_edges<Op, EFn>, ntiles);
             begin_timed(timing(), SG_KCLASS_SWEEP);
-            hipLaunchKernelGGL((k_sweep_edges<Op, EFn>), dim3(grid), dim3(kBlock), 0, stream_, gview(), dpre_, tl,
+            hipLaunchKernelGGL((k_sweep_edges<Op, EFn>), dim3(grid), dim3(kSweepBlock), 0, stream_, gview(), dpre_, tl,
                                (const uint32_t *)col_, (uint64_t)0, 0u, nv_, (uint64_t)0, ntiles, op, buffer, ef, io);
             SG_LAUNCHED();
             end_timed(timing());
@@ -1524,8 +1546,8 @@ public:
             return;
         }
         streamed_sweep([&](const uint32_t *colp, uint64_t slo, uint32_t va, uint32_t vb, uint64_t tlo, uint64_t thi) {
-            const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(thi - tlo, 16384));
-            hipLaunchKernelGGL((k_sweep_edges<Op, EFn>), dim3(grid), dim3(kBlock), 0, stream_, gview(), dpre_, tl, colp, slo, va,
+            const uint32_t grid = sweep_grid(k_sweep_edges<Op, EFn>, thi - tlo);
+            hipLaunchKernelGGL((k_sweep_edges<Op, EFn>), dim3(grid), dim3(kSweepBlock), 0, stream_, gview(), dpre_, tl, colp, slo, va,
                                vb, tlo, std::min<uint64_t>(thi, ntiles), op, buffer, ef, io);
             SG_LAUNCHED();
         });
@@ -1682,16 +1704,15 @@ public:
         // 1. invalidate the subtrees hanging from deleted tree edges (fixed point)
         const uint32_t nwords = (nv_ + 31) / 32;
         fill<uint32_t>(inval_, nwords, 0u);
-        fill<uint32_t>(coarse_, ((uint64_t)nv_ + 2047) / 2048, 0u);
-        fill<unsigned long long>(stats_ + 4, 1, 0ull);
+        fill<uint32_t>(coarse_, coarse_words(nv_, cshift_), 0u);
         frontier_reset();
         if (ndel_) {
             hipLaunchKernelGGL(k_inval_seed<Op>, dim3(grid_for(ndel_, kBlock, 8192)), dim3(kBlock), 0, stream_, op_,
                                (const unsigned long long *)bv_.del_key, ndel_, value_, buffer_, inval_, coarse_,
-                               q_[R_ & 1], cnt_ + (R_ % 3), nv_, stats_, stats_ + 4);
+                               cshift_, q_[R_ & 1], cnt_ + (R_ % 3), nv_, stats_);
             SG_LAUNCHED();
             run_rounds([this](uint32_t) {
-                return std::make_pair(VAll{}, EInvalidate<Op>{op_, value_, buffer_, inval_, coarse_, stats_ + 4});
+                return std::make_pair(VAll{}, EInvalidate<Op>{op_, value_, buffer_, inval_, coarse_, cshift_});
             }, false);
         }
         // the invalidation rounds' last read of the control block holds the count
@@ -1705,7 +1726,7 @@ public:
             // edges into invalidated vertices: edge-centric sweep over all live edges
             const SweepTiles tl = sweep_tiles();
             PassIO io = seed_io(R, nullptr, nullptr, nv_);
-            ERelaxInval<Op> ef{ERelax<Op>{op_, buffer_, stamp_, R, cfg_.weight_mode, cfg_.weight_seed}, inval_, coarse_};
+            ERelaxInval<Op> ef{ERelax<Op>{op_, buffer_, stamp_, R, cfg_.weight_mode, cfg_.weight_seed}, inval_, coarse_, cshift_};
             sweep_all(tl, op_, (const typename Op::buffer_t *)buffer_, ef, io);
             // invalidated vertices active on their own (CC: own label)
             hipLaunchKernelGGL(k_inval_activate<Op>, dim3(grid_for((nv_ + 31) / 32, 256, 4096)), dim3(256), 0, stream_, nv_,
@@ -1909,6 +1930,7 @@ public:
         m.ninv = ninv_;
         m.invbit = inval_;
         m.coarse = coarse_;
+        m.cshift = cshift_;
         m.kind = kind_;
         m.G = G_;
         m.KP = KP_;
@@ -2004,7 +2026,7 @@ public:
         fill<unsigned long long>(minv_, (uint64_t)nv_ * G_, 0ull);
         fill<unsigned long long>(ninv_, (uint64_t)nv_ * G_, 0ull);
         fill<uint32_t>(inval_, ((uint64_t)nv_ + 31) / 32, 0u);
-        fill<uint32_t>(coarse_, ((uint64_t)nv_ + 2047) / 2048, 0u);
+        fill<uint32_t>(coarse_, coarse_words(nv_, cshift_), 0u);
         nins_ = 0;
         ensure_mspill();
         frontier_reset();
@@ -2025,7 +2047,6 @@ public:
         commit(overflow);
         ensure_mspill();
         // 1. reset the slots whose tree edge was deleted, then their tree children (fixed point)
-        fill<unsigned long long>(stats_ + 4, 1, 0ull);
         frontier_reset();
         uint64_t ninval = 0;
         if (ndel_) {
@@ -2042,9 +2063,9 @@ public:
             const SweepTiles tl = sweep_tiles();
             ensure(hits_, hits_cap_, ne_ + 1);
             const uint64_t ntiles = ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile;
-            const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, 16384));
-            hipLaunchKernelGGL(k_msweep, dim3(grid), dim3(kBlock), 0, stream_, gview(), dpre_, tl, (const uint32_t *)inval_,
-                               (const uint32_t *)coarse_, hits_, nhits, (uint32_t)std::min<size_t>(hits_cap_, 0xFFFFFFFFu),
+            const uint32_t grid = sweep_grid(k_msweep, ntiles);
+            hipLaunchKernelGGL(k_msweep, dim3(grid), dim3(kSweepBlock), 0, stream_, gview(), dpre_, tl, (const uint32_t *)inval_,
+                               (const uint32_t *)coarse_, cshift_, hits_, nhits, (uint32_t)std::min<size_t>(hits_cap_, 0xFFFFFFFFu),
                                stats_ + 3);
             SG_LAUNCHED();
             // every live edge is at most one hit: the list length is bounded by ne
@@ -2053,7 +2074,7 @@ public:
                                (const uint32_t *)inval_, minv_);
             SG_LAUNCHED();
             fill<uint32_t>(inval_, ((uint64_t)nv_ + 31) / 32, 0u);
-            fill<uint32_t>(coarse_, ((uint64_t)nv_ + 2047) / 2048, 0u);
+            fill<uint32_t>(coarse_, coarse_words(nv_, cshift_), 0u);
         }
         mpairs<0>(nins_, R, nullptr);
         // 3. converge

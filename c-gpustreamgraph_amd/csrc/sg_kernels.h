@@ -177,6 +177,25 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
     return x;
 }
 
+// Coarse filter over a vertex bitmap: one bit per 2^shift vertices with
+// shift = max(6, ceil(log2 nv) - 16), so the filter has at most 65536 bits
+// (kFilterWords words) and a block can keep a copy in LDS.
+constexpr uint32_t kFilterWords = 2048;
+__host__ __device__ inline uint32_t coarse_shift_for(uint32_t nv) {
+    uint32_t s = 6;
+    while ((((uint64_t)nv + (1ull << s) - 1) >> s) > 32ull * kFilterWords) ++s;
+    return s;
+}
+__host__ __device__ inline uint32_t coarse_words(uint32_t nv, uint32_t shift) {
+    return (uint32_t)(((((uint64_t)nv + (1ull << shift) - 1) >> shift) + 31) / 32);
+}
+__device__ __forceinline__ void coarse_mark(uint32_t *coarse, uint32_t shift, uint32_t v) {
+    atomicOr(&coarse[v >> (shift + 5)], 1u << ((v >> shift) & 31));
+}
+__device__ __forceinline__ bool coarse_test(const uint32_t *coarse, uint32_t shift, uint32_t v) {
+    return (coarse[v >> (shift + 5)] >> ((v >> shift) & 31)) & 1u;
+}
+
 template <class T> __device__ __forceinline__ uint32_t to_bits(T x);
 template <> __device__ __forceinline__ uint32_t to_bits<uint32_t>(uint32_t x) { return x; }
 template <> __device__ __forceinline__ uint32_t to_bits<float>(float x) { return __float_as_uint(x); }
@@ -536,16 +555,14 @@ template <class Op>
 struct ERelaxInval {
     ERelax<Op> r;
     const uint32_t *inval;
-    const uint32_t *coarse;
+    const uint32_t *coarse;  // coarse filter of inval (the sweep tests it first, in LDS)
+    uint32_t cshift;
     __device__ ERelaxInval with_round(uint32_t stamp_next) const {
         ERelaxInval e = *this;
         e.r.stamp_val = stamp_next;
         return e;
     }
-    __device__ uint64_t load(uint32_t v) const {
-        if (!((coarse[v >> 11] >> ((v >> 6) & 31)) & 1u)) return 0;
-        return inval[v >> 5];
-    }
+    __device__ uint64_t load(uint32_t v) const { return inval[v >> 5]; }
     __device__ bool apply(uint32_t u, uint32_t v, uint32_t push, uint64_t peek) const {
         if (!((peek >> (v & 31)) & 1u)) return false;
         return r.apply(u, v, push, r.load(v));
@@ -561,14 +578,14 @@ struct EInvalidate {
     typename Op::buffer_t *buffer;
     uint32_t *inval;
     uint32_t *coarse;
-    unsigned long long *count;
+    uint32_t cshift;
     __device__ EInvalidate with_round(uint32_t) const { return *this; }
     __device__ uint64_t load(uint32_t v) const { return ld_relaxed(&buffer[v]); }
     __device__ bool apply(uint32_t u, uint32_t v, uint32_t, uint64_t b) const {
         if (Op::Parent(b) != u) return false;
         const uint32_t bit = 1u << (v & 31);
         if (atomicOr(&inval[v >> 5], bit) & bit) return false;
-        atomicOr(&coarse[v >> 11], 1u << ((v >> 6) & 31));
+        coarse_mark(coarse, cshift, v);
         typename Op::buffer_t bb = b;
         op.AccumulateBuffer_del(u, v, &bb, &value[v]);
         buffer[v] = bb;

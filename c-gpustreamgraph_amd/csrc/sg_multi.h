@@ -37,7 +37,8 @@ struct MultiState {
     unsigned long long *inv;   // [nv][G]: slots reset by this batch's deletions
     unsigned long long *ninv;  // [nv][G]: slots reset and not yet propagated to tree children
     uint32_t *invbit;          // [nv / 32]: v has a reset slot in some group
-    uint32_t *coarse;          // [nv / 2048]: one bit per 64 vertices of invbit
+    uint32_t *coarse;          // coarse filter of invbit: one bit per 2^cshift vertices
+    uint32_t cshift;
     const uint8_t *kind;       // [KP]: kQWeighted, kQUnit or kQNone (padding)
     uint32_t G, KP, QS, qs_log;
     uint32_t wmode;
@@ -115,7 +116,7 @@ __device__ __forceinline__ void reset_children(const MultiState &ms, const PassI
     if (q.slot == 0 && fresh) {
         atomicOr(&ms.ninv[(uint64_t)v * ms.G + grp], fresh);
         atomicOr(&ms.invbit[v >> 5], 1u << (v & 31));
-        atomicOr(&ms.coarse[v >> 11], 1u << ((v >> 6) & 31));
+        coarse_mark(ms.coarse, ms.cshift, v);
         app = atomicExch(&stamp[v], stamp_val) != stamp_val;
     }
     if ((fresh >> q.slot) & 1u) ms.best[(uint64_t)v * ms.KP + grp * ms.QS + q.slot] = ~0ull;
@@ -448,29 +449,40 @@ __device__ __forceinline__ void hit_flush(unsigned long long *stage, uint32_t &n
     n = 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_msweep(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
-                                                   const uint32_t *invbit, const uint32_t *coarse,
-                                                   unsigned long long *hits, uint32_t *nhits, uint32_t hit_cap,
-                                                   unsigned long long *overflow) {
-    __shared__ SweepWave s_wave[kWavesPerBlock];
-    __shared__ unsigned long long s_hits[kWavesPerBlock][kHitStage];
-    __shared__ uint32_t s_cnt[kWavesPerBlock], s_base;
+__global__ void __launch_bounds__(kSweepBlock) k_msweep(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
+                                                        const uint32_t *invbit, const uint32_t *coarse, uint32_t cshift,
+                                                        unsigned long long *hits, uint32_t *nhits, uint32_t hit_cap,
+                                                        unsigned long long *overflow) {
+    __shared__ SweepWave s_wave[kSweepWaves];
+    __shared__ unsigned long long s_hits[kSweepWaves][kHitStage];
+    __shared__ uint32_t s_filter[kFilterWords];
+    __shared__ uint32_t s_cnt[kSweepWaves], s_base;
     const unsigned long long E = dpre[g.nv];
     const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
     SweepWave &sh = s_wave[w];
     uint32_t nst = 0;  // wave-uniform fill of this wave's hit stage
     unsigned long long n_edges = 0;
-    const unsigned long long gw = (unsigned long long)blockIdx.x * kWavesPerBlock + w;
-    const unsigned long long nw = (unsigned long long)gridDim.x * kWavesPerBlock;
-    for (unsigned long long t = gw; t < ntiles && t + 1 < tiles.cap; t += nw) {
+    load_filter(s_filter, coarse, coarse_words(g.nv, cshift));
+    const unsigned long long tend = min<unsigned long long>(ntiles, tiles.cap - 1);
+    const unsigned long long nw = (unsigned long long)gridDim.x * kSweepWaves;
+    unsigned long long t = (unsigned long long)blockIdx.x * kSweepWaves + w;
+    unsigned long long head = t < tend ? tile_head_load(tiles, t) : 0;
+    for (; t < tend; t += nw) {
+        const unsigned long long cur = head;
+        if (t + nw < tend) head = tile_head_load(tiles, t + nw);
         uint32_t vv[kSweepPerLane];
-        const uint32_t v0 = sweep_tile(g, dpre, tiles, t, g.col, 0, 0, g.nv, sh, vv, n_edges);
+        const uint32_t v0 = sweep_tile(g, dpre, cur, g.col, 0, 0, g.nv, sh, vv, n_edges);
+        bool pass[kSweepPerLane];
+#pragma unroll
+        for (int j = 0; j < kSweepPerLane; ++j) pass[j] = vv[j] != kGap && coarse_test(s_filter, cshift, vv[j]);
+        uint32_t word[kSweepPerLane];
+#pragma unroll
+        for (int j = 0; j < kSweepPerLane; ++j) word[j] = pass[j] ? invbit[vv[j] >> 5] : 0u;
 #pragma unroll
         for (int j = 0; j < kSweepPerLane; ++j) {
             const uint32_t v = vv[j];
-            bool hit = false;
-            if (v != kGap && ((coarse[v >> 11] >> ((v >> 6) & 31)) & 1u)) hit = (invbit[v >> 5] >> (v & 31)) & 1u;
+            const bool hit = pass[j] && ((word[j] >> (v & 31)) & 1u);
             const uint64_t bm = __ballot(hit);
             if (bm) {
                 const uint32_t c = (uint32_t)__popcll(bm);
@@ -489,7 +501,7 @@ __global__ void __launch_bounds__(kBlock) k_msweep(GraphView g, const unsigned l
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t tot = 0;
-        for (int i = 0; i < kWavesPerBlock; ++i) {
+        for (int i = 0; i < kSweepWaves; ++i) {
             const uint32_t c = s_cnt[i];
             s_cnt[i] = tot;
             tot += c;

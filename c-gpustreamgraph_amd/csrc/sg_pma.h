@@ -489,9 +489,12 @@ __global__ void k_sweep_tiles(uint32_t nv, const unsigned long long *dpre, Sweep
 // with edges is marked with its 16-bit index, a wave max-scan fills the owners
 // forward, and the tile's edges are loaded coalesced, kSweepPerLane per lane,
 // all in flight. A tile whose slot range does not fit 32 bits takes the slot
-// of each edge from off and dpre in global memory instead.
+// of each edge from off and dpre in global memory instead. The next tile's
+// bounds are loaded while the current tile is walked.
 constexpr int kSweepPerLane = kSweepTile / kWave;
 constexpr uint32_t kSweepOutStage = 128;
+constexpr int kSweepWaves = 4;                    // waves per block of the sweep kernels
+constexpr int kSweepBlock = kSweepWaves * kWave;
 
 struct SweepWave {
     uint16_t own[kSweepTile];
@@ -499,30 +502,54 @@ struct SweepWave {
     uint32_t out[kSweepOutStage];
 };
 
+// Bounds of tile t, loaded by lanes 0-3 with vector loads so that the load
+// stays in flight while the wave works on the current tile: lane 0 v[t],
+// lane 1 v[t + 1], lane 2 e[t], lane 3 e[t + 1].
+__device__ __forceinline__ unsigned long long tile_head_load(const SweepTiles &tiles, unsigned long long t) {
+    const uint32_t lane = lane_id();
+    unsigned long long x = 0;
+    if (lane < 2) x = tiles.v[t + lane];
+    else if (lane < 4) x = tiles.e[t + lane - 2];
+    return x;
+}
+
 // colp / slot_lo: where slot s of the tile's sources is read (colp[s - slot_lo]:
 // the edge array itself, or one streamed segment of it); only sources in
-// [src_lo, src_hi) are walked. vv[j] = target at tile position j * 64 + lane
-// (kGap when none); the owner index of a position stays in sh.own.
+// [src_lo, src_hi) are walked. head = tile_head_load() of the tile. vv[j] =
+// target at tile position j * 64 + lane (kGap when none); the owner index of a
+// position stays in sh.own.
 __device__ __forceinline__ uint32_t sweep_tile(const GraphView &g, const unsigned long long *dpre,
-                                               const SweepTiles &tiles, unsigned long long t, const uint32_t *colp,
-                                               uint64_t slot_lo, uint32_t src_lo, uint32_t src_hi, SweepWave &sh,
+                                               unsigned long long head, const uint32_t *colp, uint64_t slot_lo,
+                                               uint32_t src_lo, uint32_t src_hi, SweepWave &sh,
                                                uint32_t vv[kSweepPerLane], unsigned long long &n_edges) {
     const uint32_t lane = lane_id();
-    const uint32_t v0 = tiles.v[t], v1 = tiles.v[t + 1];
-    const unsigned long long e0 = tiles.e[t], e1 = tiles.e[t + 1];
+    const uint32_t v0 = (uint32_t)__shfl(head, 0, kWave), v1 = (uint32_t)__shfl(head, 1, kWave);
+    const unsigned long long e0 = __shfl(head, 2, kWave), e1 = __shfl(head, 3, kWave);
     const uint32_t ne = (uint32_t)min<unsigned long long>(e1 - e0, kSweepTile);
-    const uint32_t span = min<uint32_t>(v1 - v0 + 1, kSweepTile + 1);
-    const uint64_t off0 = g.off[v0];
-    const bool wide = (g.off[v0 + span] - off0) >= 0x40000000ull;
+    const uint32_t span = min<uint32_t>(v1 - v0 + 1, kSweepTile + 1);  // v0 + span <= v1 + 1 <= nv
 #pragma unroll
     for (int j = 0; j < kSweepPerLane; ++j) sh.own[j * kWave + lane] = 0;
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t i = lane; i < span; i += kWave) {
-        const uint32_t v = v0 + i;
-        const unsigned long long a = dpre[v], b = dpre[v + 1];
-        if (!wide) sh.base[i] = (int32_t)((long long)(g.off[v] - off0) - (long long)(a - e0));
-        if (b > a && a >= e0 && a < e0 + ne) sh.own[a - e0] = (uint16_t)i;
+    // sources v0 .. v0 + span - 1; position span only reads off (the width check)
+    uint64_t off0 = 0;
+    bool far = false;
+    for (uint32_t i0 = 0; i0 <= span; i0 += kWave) {
+        const uint32_t i = i0 + lane;
+        uint64_t ov = 0;
+        unsigned long long a = 0, b = 0;
+        if (i <= span) ov = g.off[v0 + i];
+        if (i < span) {
+            a = dpre[v0 + i];
+            b = dpre[v0 + i + 1];
+        }
+        if (i0 == 0) off0 = __shfl(ov, 0, kWave);
+        far |= i <= span && ov - off0 >= 0x40000000ull;
+        if (i < span) {
+            sh.base[i] = (int32_t)((long long)(ov - off0) - (long long)(a - e0));
+            if (b > a && a >= e0 && a < e0 + ne) sh.own[a - e0] = (uint16_t)i;
+        }
     }
+    const bool wide = __ballot(far) != 0;
     __builtin_amdgcn_wave_barrier();
     // forward fill of the owners: lane k holds positions [k * P, k * P + P)
     uint32_t own[kSweepPerLane];
@@ -565,46 +592,63 @@ __device__ __forceinline__ uint32_t sweep_tile(const GraphView &g, const unsigne
     return v0;
 }
 
-// Block-wide sum of a per-lane counter, one atomic per block.
+// Block-wide sum of a per-lane counter, one atomic per block of kWaves waves.
+template <int kWaves>
 __device__ __forceinline__ void block_count(unsigned long long x, unsigned long long *s_red, unsigned long long *dst) {
     x = wave_sum_u64(x);
     if (lane_id() == 0) s_red[threadIdx.x / kWave] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
-        for (int i = 0; i < kWavesPerBlock; ++i) t += s_red[i];
+        for (int i = 0; i < kWaves; ++i) t += s_red[i];
         if (t) atomicAdd(dst, t);
     }
 }
 
-// Tiles in [t_begin, t_end) are walked.
+// The block's copy of a coarse vertex filter (coarse_words(nv, shift) words).
+__device__ __forceinline__ void load_filter(uint32_t *s_filter, const uint32_t *coarse, uint32_t nwords) {
+    for (uint32_t i = threadIdx.x; i < nwords; i += blockDim.x) s_filter[i] = coarse[i];
+    __syncthreads();
+}
+
+// Tiles in [t_begin, t_end) are walked. The edge functor's coarse filter is
+// tested in LDS; only targets it passes read the functor's word (efn.load).
 template <class Op, class EFn>
-__global__ void __launch_bounds__(kBlock) k_sweep_edges(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
-                                                        const uint32_t *colp, uint64_t slot_lo, uint32_t src_lo,
-                                                        uint32_t src_hi, uint64_t t_begin, uint64_t t_end, Op op,
-                                                        const typename Op::buffer_t *buffer, EFn efn, PassIO io) {
-    __shared__ SweepWave s_wave[kWavesPerBlock];
-    __shared__ unsigned long long s_red[kWavesPerBlock];
+__global__ void __launch_bounds__(kSweepBlock) k_sweep_edges(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
+                                                             const uint32_t *colp, uint64_t slot_lo, uint32_t src_lo,
+                                                             uint32_t src_hi, uint64_t t_begin, uint64_t t_end, Op op,
+                                                             const typename Op::buffer_t *buffer, EFn efn, PassIO io) {
+    __shared__ SweepWave s_wave[kSweepWaves];
+    __shared__ uint32_t s_filter[kFilterWords];
+    __shared__ unsigned long long s_red[kSweepWaves];
     const unsigned long long E = dpre[g.nv];
     const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
     SweepWave &sh = s_wave[w];
     OutStage st{sh.out, 0, kSweepOutStage};
     unsigned long long n_edges = 0;
-    const unsigned long long tend = min<unsigned long long>(t_end, ntiles);
-    const unsigned long long gw = (unsigned long long)blockIdx.x * kWavesPerBlock + w;
-    const unsigned long long nw = (unsigned long long)gridDim.x * kWavesPerBlock;
-    for (unsigned long long t = t_begin + gw; t < tend && t + 1 < tiles.cap; t += nw) {
+    const uint32_t cs = efn.cshift;
+    load_filter(s_filter, efn.coarse, coarse_words(g.nv, cs));
+    const unsigned long long tend = min<unsigned long long>(min<unsigned long long>(t_end, ntiles), tiles.cap - 1);
+    const unsigned long long nw = (unsigned long long)gridDim.x * kSweepWaves;
+    unsigned long long t = t_begin + (unsigned long long)blockIdx.x * kSweepWaves + w;
+    unsigned long long head = t < tend ? tile_head_load(tiles, t) : 0;
+    for (; t < tend; t += nw) {
+        const unsigned long long cur = head;
+        if (t + nw < tend) head = tile_head_load(tiles, t + nw);
         uint32_t vv[kSweepPerLane];
-        const uint32_t v0 = sweep_tile(g, dpre, tiles, t, colp, slot_lo, src_lo, src_hi, sh, vv, n_edges);
+        const uint32_t v0 = sweep_tile(g, dpre, cur, colp, slot_lo, src_lo, src_hi, sh, vv, n_edges);
+        bool pass[kSweepPerLane];
+#pragma unroll
+        for (int j = 0; j < kSweepPerLane; ++j) pass[j] = vv[j] != kGap && coarse_test(s_filter, cs, vv[j]);
         uint64_t peek[kSweepPerLane];
 #pragma unroll
-        for (int j = 0; j < kSweepPerLane; ++j) peek[j] = vv[j] != kGap ? efn.load(vv[j]) : 0;
+        for (int j = 0; j < kSweepPerLane; ++j) peek[j] = pass[j] ? efn.load(vv[j]) : 0;
 #pragma unroll
         for (int j = 0; j < kSweepPerLane; ++j) {
             const uint32_t v = vv[j];
             bool enq = false;
-            if (v != kGap && ((peek[j] >> (v & 31)) & 1u)) {
+            if (pass[j] && ((peek[j] >> (v & 31)) & 1u)) {
                 const uint32_t src = v0 + sh.own[j * kWave + lane];
                 enq = efn.apply(src, v, op.PushValue(ld_peek(&buffer[src])), peek[j]);
             }
@@ -613,7 +657,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep_edges(GraphView g, const unsig
         __builtin_amdgcn_wave_barrier();
     }
     stage_flush(st, io);
-    block_count(n_edges, s_red, io.stats + 0);
+    block_count<kSweepWaves>(n_edges, s_red, io.stats + 0);
 }
 
 // Invalidated vertices that are active on their own (CC: own label) join the
