@@ -562,11 +562,10 @@ struct ERelaxInval {
         e.r.stamp_val = stamp_next;
         return e;
     }
-    __device__ uint64_t load(uint32_t v) const { return inval[v >> 5]; }
-    __device__ bool apply(uint32_t u, uint32_t v, uint32_t push, uint64_t peek) const {
-        if (!((peek >> (v & 31)) & 1u)) return false;
-        return r.apply(u, v, push, r.load(v));
-    }
+    __device__ uint32_t load(uint32_t v) const { return inval[v >> 5]; }
+    __device__ bool test(uint32_t v, uint32_t word) const { return (word >> (v & 31)) & 1u; }
+    // v is invalidated: relax it from u
+    __device__ bool apply(uint32_t u, uint32_t v, uint32_t push) const { return r.apply(u, v, push, r.load(v)); }
 };
 
 // Reset tree children of u (AccumulateBuffer_del via PushFunctorDB_Del,

@@ -434,7 +434,7 @@ __global__ void __launch_bounds__(kBlock) k_mpairs(MultiState ms, BatchView bv, 
 // reset slot becomes a hit (u << 32 | v). Same wave tiles as k_sweep_edges;
 // hits are gathered per wave in LDS and appended kHitStage at a time (one
 // atomic per flush).
-constexpr uint32_t kHitStage = 512;
+constexpr uint32_t kHitStage = 256;
 
 __device__ __forceinline__ void hit_flush(unsigned long long *stage, uint32_t &n, unsigned long long *hits,
                                           uint32_t *nhits, uint32_t hit_cap, unsigned long long *overflow) {
@@ -462,17 +462,18 @@ __global__ void __launch_bounds__(kSweepBlock) k_msweep(GraphView g, const unsig
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
     SweepWave &sh = s_wave[w];
     uint32_t nst = 0;  // wave-uniform fill of this wave's hit stage
-    unsigned long long n_edges = 0;
+    uint32_t n_edges = 0;
     load_filter(s_filter, coarse, coarse_words(g.nv, cshift));
-    const unsigned long long tend = min<unsigned long long>(ntiles, tiles.cap - 1);
-    const unsigned long long nw = (unsigned long long)gridDim.x * kSweepWaves;
-    unsigned long long t = (unsigned long long)blockIdx.x * kSweepWaves + w;
-    unsigned long long head = t < tend ? tile_head_load(tiles, t) : 0;
-    for (; t < tend; t += nw) {
-        const unsigned long long cur = head;
-        if (t + nw < tend) head = tile_head_load(tiles, t + nw);
+    TilePipe tp;
+    tp.tend = min<unsigned long long>(ntiles, tiles.cap - 1);
+    tp.nw = (unsigned long long)gridDim.x * kSweepWaves;
+    tp.t = (unsigned long long)blockIdx.x * kSweepWaves + w;
+    tp.start(g, dpre, tiles);
+    while (tp.t < tp.tend) {
         uint32_t vv[kSweepPerLane];
-        const uint32_t v0 = sweep_tile(g, dpre, cur, g.col, 0, 0, g.nv, sh, vv, n_edges);
+        const uint32_t v0 = tp.h.v0;
+        sweep_tile(g, dpre, tp.h, tp.pre, g.col, 0, 0, g.nv, sh, vv, n_edges);
+        tp.advance(g, dpre, tiles);
         bool pass[kSweepPerLane];
 #pragma unroll
         for (int j = 0; j < kSweepPerLane; ++j) pass[j] = vv[j] != kGap && coarse_test(s_filter, cshift, vv[j]);

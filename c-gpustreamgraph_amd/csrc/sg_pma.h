@@ -513,40 +513,75 @@ __device__ __forceinline__ unsigned long long tile_head_load(const SweepTiles &t
     return x;
 }
 
+struct TileHead {
+    uint32_t v0, span, ne;  // first source, sources (v0 + span <= nv), edges
+    unsigned long long e0;  // first live-edge index
+};
+
+__device__ __forceinline__ unsigned long long read_lane_u64(unsigned long long x, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), lane);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// The bounds as wave-uniform (scalar) values.
+__device__ __forceinline__ TileHead tile_head(unsigned long long raw) {
+    TileHead h;
+    h.v0 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)raw, 0);
+    const uint32_t v1 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)raw, 1);
+    h.e0 = read_lane_u64(raw, 2);
+    const unsigned long long e1 = read_lane_u64(raw, 3);
+    h.ne = (uint32_t)min<unsigned long long>(e1 - h.e0, kSweepTile);
+    h.span = min<uint32_t>(v1 - h.v0 + 1, kSweepTile + 1);
+    return h;
+}
+
+// Per-source words of the tile's first kWave sources (lane i: source v0 + i,
+// off and dpre; position span is the vertex after the last source: its off
+// is the width check and its dpre the last source's end). Loaded one tile
+// ahead.
+struct TilePre {
+    uint64_t ov;
+    unsigned long long a;
+};
+
+__device__ __forceinline__ TilePre tile_pre(const GraphView &g, const unsigned long long *dpre, const TileHead &h,
+                                            uint32_t i) {
+    TilePre p{0, 0};
+    if (i <= h.span) {
+        p.ov = g.off[h.v0 + i];
+        p.a = dpre[h.v0 + i];
+    }
+    return p;
+}
+
 // colp / slot_lo: where slot s of the tile's sources is read (colp[s - slot_lo]:
 // the edge array itself, or one streamed segment of it); only sources in
-// [src_lo, src_hi) are walked. head = tile_head_load() of the tile. vv[j] =
-// target at tile position j * 64 + lane (kGap when none); the owner index of a
-// position stays in sh.own.
-__device__ __forceinline__ uint32_t sweep_tile(const GraphView &g, const unsigned long long *dpre,
-                                               unsigned long long head, const uint32_t *colp, uint64_t slot_lo,
-                                               uint32_t src_lo, uint32_t src_hi, SweepWave &sh,
-                                               uint32_t vv[kSweepPerLane], unsigned long long &n_edges) {
+// [src_lo, src_hi) are walked. pre = tile_pre(h, lane). vv[j] = target at tile
+// position j * 64 + lane (kGap when none); the owner index of a position stays
+// in sh.own.
+__device__ __forceinline__ void sweep_tile(const GraphView &g, const unsigned long long *dpre, const TileHead &h,
+                                           const TilePre &pre, const uint32_t *colp, uint64_t slot_lo, uint32_t src_lo,
+                                           uint32_t src_hi, SweepWave &sh, uint32_t vv[kSweepPerLane],
+                                           uint32_t &n_edges) {
     const uint32_t lane = lane_id();
-    const uint32_t v0 = (uint32_t)__shfl(head, 0, kWave), v1 = (uint32_t)__shfl(head, 1, kWave);
-    const unsigned long long e0 = __shfl(head, 2, kWave), e1 = __shfl(head, 3, kWave);
-    const uint32_t ne = (uint32_t)min<unsigned long long>(e1 - e0, kSweepTile);
-    const uint32_t span = min<uint32_t>(v1 - v0 + 1, kSweepTile + 1);  // v0 + span <= v1 + 1 <= nv
+    const uint32_t v0 = h.v0, span = h.span, ne = h.ne;
+    const unsigned long long e0 = h.e0;
 #pragma unroll
     for (int j = 0; j < kSweepPerLane; ++j) sh.own[j * kWave + lane] = 0;
     __builtin_amdgcn_wave_barrier();
-    // sources v0 .. v0 + span - 1; position span only reads off (the width check)
-    uint64_t off0 = 0;
+    const uint64_t off0 = read_lane_u64(pre.ov, 0);
     bool far = false;
     for (uint32_t i0 = 0; i0 <= span; i0 += kWave) {
         const uint32_t i = i0 + lane;
-        uint64_t ov = 0;
-        unsigned long long a = 0, b = 0;
-        if (i <= span) ov = g.off[v0 + i];
+        const TilePre p = i0 == 0 ? pre : tile_pre(g, dpre, h, i);
+        far |= i <= span && p.ov - off0 >= 0x40000000ull;
+        // end of source i = start of source i + 1 (the next lane, or one more load on the last lane)
+        unsigned long long b = __shfl_down(p.a, 1, kWave);
+        if (lane == kWave - 1 && i < span) b = dpre[v0 + i + 1];
         if (i < span) {
-            a = dpre[v0 + i];
-            b = dpre[v0 + i + 1];
-        }
-        if (i0 == 0) off0 = __shfl(ov, 0, kWave);
-        far |= i <= span && ov - off0 >= 0x40000000ull;
-        if (i < span) {
-            sh.base[i] = (int32_t)((long long)(ov - off0) - (long long)(a - e0));
-            if (b > a && a >= e0 && a < e0 + ne) sh.own[a - e0] = (uint16_t)i;
+            sh.base[i] = (int32_t)((long long)(p.ov - off0) - (long long)(p.a - e0));
+            if (b > p.a && p.a >= e0 && p.a < e0 + ne) sh.own[p.a - e0] = (uint16_t)i;
         }
     }
     const bool wide = __ballot(far) != 0;
@@ -589,8 +624,34 @@ __device__ __forceinline__ uint32_t sweep_tile(const GraphView &g, const unsigne
             }
         }
     }
-    return v0;
 }
+
+// Tile pipeline of one wave over tiles t, t + nw, ... < tend: the bounds are
+// loaded two tiles ahead and the first sources' words one tile ahead, so they
+// are in flight while the current tile's edges are loaded and tested.
+struct TilePipe {
+    unsigned long long t, nw, tend;
+    unsigned long long raw_next;  // tile_head_load(t + nw)
+    TileHead h;                   // tile t
+    TilePre pre;                  // tile_pre of tile t
+    __device__ __forceinline__ void start(const GraphView &g, const unsigned long long *dpre, const SweepTiles &tiles) {
+        raw_next = t + nw < tend ? tile_head_load(tiles, t + nw) : 0;
+        if (t < tend) {
+            h = tile_head(tile_head_load(tiles, t));
+            pre = tile_pre(g, dpre, h, lane_id());
+        }
+    }
+    // after the current tile's edges are loaded: start the next tile's loads
+    __device__ __forceinline__ void advance(const GraphView &g, const unsigned long long *dpre, const SweepTiles &tiles) {
+        const unsigned long long raw2 = t + 2 * nw < tend ? tile_head_load(tiles, t + 2 * nw) : 0;
+        if (t + nw < tend) {
+            h = tile_head(raw_next);
+            pre = tile_pre(g, dpre, h, lane_id());
+        }
+        raw_next = raw2;
+        t += nw;
+    }
+};
 
 // Block-wide sum of a per-lane counter, one atomic per block of kWaves waves.
 template <int kWaves>
@@ -626,38 +687,61 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_edges(GraphView g, const 
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
     SweepWave &sh = s_wave[w];
     OutStage st{sh.out, 0, kSweepOutStage};
-    unsigned long long n_edges = 0;
+    uint32_t n_edges = 0;
     const uint32_t cs = efn.cshift;
     load_filter(s_filter, efn.coarse, coarse_words(g.nv, cs));
-    const unsigned long long tend = min<unsigned long long>(min<unsigned long long>(t_end, ntiles), tiles.cap - 1);
-    const unsigned long long nw = (unsigned long long)gridDim.x * kSweepWaves;
-    unsigned long long t = t_begin + (unsigned long long)blockIdx.x * kSweepWaves + w;
-    unsigned long long head = t < tend ? tile_head_load(tiles, t) : 0;
-    for (; t < tend; t += nw) {
-        const unsigned long long cur = head;
-        if (t + nw < tend) head = tile_head_load(tiles, t + nw);
+    TilePipe tp;
+    tp.tend = min<unsigned long long>(min<unsigned long long>(t_end, ntiles), tiles.cap - 1);
+    tp.nw = (unsigned long long)gridDim.x * kSweepWaves;
+    tp.t = t_begin + (unsigned long long)blockIdx.x * kSweepWaves + w;
+    tp.start(g, dpre, tiles);
+    while (tp.t < tp.tend) {
         uint32_t vv[kSweepPerLane];
-        const uint32_t v0 = sweep_tile(g, dpre, cur, colp, slot_lo, src_lo, src_hi, sh, vv, n_edges);
+        const uint32_t v0 = tp.h.v0;
+        sweep_tile(g, dpre, tp.h, tp.pre, colp, slot_lo, src_lo, src_hi, sh, vv, n_edges);
+        tp.advance(g, dpre, tiles);
         bool pass[kSweepPerLane];
 #pragma unroll
         for (int j = 0; j < kSweepPerLane; ++j) pass[j] = vv[j] != kGap && coarse_test(s_filter, cs, vv[j]);
-        uint64_t peek[kSweepPerLane];
+        decltype(efn.load(0u)) peek[kSweepPerLane];
 #pragma unroll
         for (int j = 0; j < kSweepPerLane; ++j) peek[j] = pass[j] ? efn.load(vv[j]) : 0;
+        // compact the positions that pass the functor's word test in place:
+        // the k-th hit (position p >= k) moves its owner to own[k] and its
+        // target to the (now unused) base[k]; then one loop applies them
+        uint32_t nh = 0;  // wave-uniform
+        uint32_t *const hv = reinterpret_cast<uint32_t *>(sh.base);
 #pragma unroll
         for (int j = 0; j < kSweepPerLane; ++j) {
-            const uint32_t v = vv[j];
+            const bool h = pass[j] && efn.test(vv[j], peek[j]);
+            const uint64_t m = __ballot(h);
+            if (m) {
+                const uint32_t o = sh.own[j * kWave + lane];
+                __builtin_amdgcn_wave_barrier();
+                if (h) {
+                    const uint32_t k = nh + (uint32_t)__popcll(m & lanemask_lt());
+                    sh.own[k] = (uint16_t)o;
+                    hv[k] = vv[j];
+                }
+                nh += (uint32_t)__popcll(m);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i0 = 0; i0 < nh; i0 += kWave) {
+            const uint32_t i = i0 + lane;
             bool enq = false;
-            if (pass[j] && ((peek[j] >> (v & 31)) & 1u)) {
-                const uint32_t src = v0 + sh.own[j * kWave + lane];
-                enq = efn.apply(src, v, op.PushValue(ld_peek(&buffer[src])), peek[j]);
+            uint32_t v = 0;
+            if (i < nh) {
+                v = hv[i];
+                const uint32_t src = v0 + sh.own[i];
+                enq = efn.apply(src, v, op.PushValue(ld_peek(&buffer[src])));
             }
             stage_push(enq, v, st, io);
         }
         __builtin_amdgcn_wave_barrier();
     }
     stage_flush(st, io);
-    block_count<kSweepWaves>(n_edges, s_red, io.stats + 0);
+    block_count<kSweepWaves>((unsigned long long)n_edges, s_red, io.stats + 0);
 }
 
 // Invalidated vertices that are active on their own (CC: own label) join the
