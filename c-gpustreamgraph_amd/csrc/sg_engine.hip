@@ -710,6 +710,8 @@ public:
         hallocs_.clear();
         host_bytes_ = 0;
         for (auto &ev : seg_ev_) if (ev) (void)hipEventDestroy(ev);
+        if (sync_ev_) (void)hipEventDestroy(sync_ev_);
+        sync_ev_ = nullptr;
         if (cstream_) (void)hipStreamDestroy(cstream_);
         cstream_ = nullptr;
         for (auto &ev : events_) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
@@ -804,6 +806,7 @@ public:
     bool seg_dirty_ = true;
     hipStream_t cstream_ = nullptr;           // copy stream of the streamed segments
     hipEvent_t seg_ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // copied[2], used[2], start
+    hipEvent_t sync_ev_ = nullptr;                                            // polled by sync()
     uint32_t *stage_[2] = {nullptr, nullptr};
     size_t stage_cap_[2] = {0, 0};
     // HBM edge cache (host edges only)
@@ -842,13 +845,25 @@ public:
         if (prop.warpSize != 64) fail(SG_EDEVICE, "wave64 device required");
         num_cus_ = prop.multiProcessorCount;
         SG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        SG_HIP(hipEventCreateWithFlags(&sync_ev_, hipEventDisableTiming));
         SG_HIP(hipHostMalloc(&pinned_, 65536, hipHostMallocDefault));
         h_ctl_ = (Ctl *)pinned_;
         h_blk_ = (uint32_t *)((char *)pinned_ + 4096);
         h_u64_ = (unsigned long long *)((char *)pinned_ + 16384);
     }
 
-    void sync() { SG_HIP(hipStreamSynchronize(stream_)); }
+    // Wait for the stream by polling an event: the host resumes as soon as the
+    // work ends instead of after a blocking wake-up (the engine syncs a few
+    // times per snapshot, and the GPU idles until the next launch).
+    void sync() {
+        SG_HIP(hipEventRecord(sync_ev_, stream_));
+        for (;;) {
+            const hipError_t e = hipEventQuery(sync_ev_);
+            if (e == hipSuccess) return;
+            if (e != hipErrorNotReady) SG_HIP(e);
+            __builtin_ia32_pause();
+        }
+    }
 
     template <class T>
     void fill(T *p, uint64_t n, T v) {
