@@ -783,7 +783,7 @@ public:
     size_t chunks_cap_ = 0;
     unsigned long long *ccount_ = nullptr, *cpos_ = nullptr;
     size_t ccount_cap_ = 0, cpos_cap_ = 0;
-    uint64_t merged_total_ = 0, chunk_bound_ = 0;
+    uint64_t chunk_bound_ = 0;
     uint32_t *tmp2_ = nullptr;
     size_t tmp2_cap_ = 0;
     Ctl *ctl_ = nullptr;
@@ -981,6 +981,8 @@ public:
         fill<uint8_t>(chosen_, 2 * nleaves_, 0);
         fill<uint8_t>(maximal_, 2 * nleaves_, 0);
         fill<uint32_t>(mark_, ((uint64_t)nv + 31) / 32, 0u);
+        fill<uint32_t>(inval_, ((uint64_t)nv + 31) / 32, 0u);
+        fill<uint32_t>(coarse_, coarse_words(nv, cshift_), 0u);
         fill<uint32_t>(br_.ins_b, nv, 0u);
         fill<uint32_t>(br_.ins_e, nv, 0u);
         fill<uint32_t>(br_.del_b, nv, 0u);
@@ -1351,10 +1353,8 @@ public:
             SG_LAUNCHED();
         }
         SG_HIP(hipMemcpyAsync(h_ctl_, ctl_, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
-        SG_HIP(hipMemcpyAsync(h_u64_, cpos_ + chunk_bound, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
         sync();
         if (h_ctl_->pad[0] || h_ctl_->pad[1]) fail(SG_EDEVICE, "internal touched/chunk list overflow");
-        merged_total_ = h_u64_[0];
         st_.dropped_updates += h_ctl_->dropped;
         return h_ctl_->overflow != 0;
     }
@@ -1396,9 +1396,8 @@ public:
         }
         cache_rebuild();
         if (nins_ + ndel_ == 0) return;
-        hipLaunchKernelGGL(k_clear_ranges, dim3(1024), dim3(256), 0, stream_, tinfo_, &ctl_->tcount, br_);
+        hipLaunchKernelGGL(k_clear_ranges, dim3(1024), dim3(256), 0, stream_, tinfo_, &ctl_->tcount, br_, mark_);
         SG_LAUNCHED();
-        fill<uint32_t>(mark_, ((uint64_t)nv_ + 31) / 32, 0u);
     }
 
     void rebalance(uint64_t ne_new) {
@@ -1716,10 +1715,8 @@ public:
         auto t0 = std::chrono::steady_clock::now();
         const bool overflow = prepare(d_ins, nins, d_del, ndel);
         commit(overflow);
-        // 1. invalidate the subtrees hanging from deleted tree edges (fixed point)
-        const uint32_t nwords = (nv_ + 31) / 32;
-        fill<uint32_t>(inval_, nwords, 0u);
-        fill<uint32_t>(coarse_, coarse_words(nv_, cshift_), 0u);
+        // 1. invalidate the subtrees hanging from deleted tree edges (fixed point);
+        //    the bitmaps are zero here (cleared by the last batch's activation pass)
         frontier_reset();
         if (ndel_) {
             hipLaunchKernelGGL(k_inval_seed<Op>, dim3(grid_for(ndel_, kBlock, 8192)), dim3(kBlock), 0, stream_, op_,
@@ -1745,7 +1742,8 @@ public:
             sweep_all(tl, op_, (const typename Op::buffer_t *)buffer_, ef, io);
             // invalidated vertices active on their own (CC: own label)
             hipLaunchKernelGGL(k_inval_activate<Op>, dim3(grid_for((nv_ + 31) / 32, 256, 4096)), dim3(256), 0, stream_, nv_,
-                               op_, inval_, value_, buffer_, stamp_, R, q_[R & 1], cnt_ + (R % 3), nv_, stats_ + 3);
+                               op_, inval_, coarse_, cshift_, value_, buffer_, stamp_, R, q_[R & 1], cnt_ + (R % 3), nv_,
+                               stats_ + 3);
             SG_LAUNCHED();
         }
         if (nins_) {
@@ -2086,10 +2084,8 @@ public:
             // every live edge is at most one hit: the list length is bounded by ne
             mpairs<2>(std::min<uint64_t>(ne_, hits_cap_), R, nhits);
             hipLaunchKernelGGL(k_mclear_inv, dim3(grid_for((nv_ + 31) / 32, 256, 4096)), dim3(256), 0, stream_, nv_, G_,
-                               (const uint32_t *)inval_, minv_);
+                               inval_, coarse_, cshift_, minv_);
             SG_LAUNCHED();
-            fill<uint32_t>(inval_, ((uint64_t)nv_ + 31) / 32, 0u);
-            fill<uint32_t>(coarse_, coarse_words(nv_, cshift_), 0u);
         }
         mpairs<0>(nins_, R, nullptr);
         // 3. converge

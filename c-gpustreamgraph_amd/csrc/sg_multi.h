@@ -518,10 +518,14 @@ __global__ void __launch_bounds__(kSweepBlock) k_msweep(GraphView g, const unsig
 }
 
 // Clear the masks of the vertices with a reset slot (one thread per bitmap word).
-__global__ void k_mclear_inv(uint32_t nv, uint32_t G, const uint32_t *invbit, unsigned long long *inv) {
+// The vertex bitmap and its coarse filter are cleared on the way.
+__global__ void k_mclear_inv(uint32_t nv, uint32_t G, uint32_t *invbit, uint32_t *coarse, uint32_t cshift,
+                             unsigned long long *inv) {
     const uint32_t nw = (nv + 31) / 32;
     for (uint64_t wd = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; wd < nw; wd += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t bits = invbit[wd];
+        if (bits) invbit[wd] = 0;
+        if ((wd & ((1ull << cshift) - 1)) == 0) coarse[wd >> cshift] = 0;
         while (bits) {
             const uint32_t b = __builtin_ctz(bits);
             bits &= bits - 1;

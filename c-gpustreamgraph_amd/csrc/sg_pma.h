@@ -208,11 +208,14 @@ __global__ void __launch_bounds__(kBlock) k_touch_info(GraphView g, BatchRanges 
 }
 
 // Zero the batch ranges of the touched vertices (after the batch is applied).
-__global__ void k_clear_ranges(const TInfo *tinfo, const uint32_t *tcount, BatchRanges br) {
+// Also clears the touched-vertex bitmap: every set bit belongs to a touched
+// vertex, so each touched vertex may zero its whole word.
+__global__ void k_clear_ranges(const TInfo *tinfo, const uint32_t *tcount, BatchRanges br, uint32_t *mark) {
     const uint32_t n = *tcount;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t u = tinfo[t].u;
         br.ins_b[u] = br.ins_e[u] = br.del_b[u] = br.del_e[u] = 0;
+        mark[u >> 5] = 0;
     }
 }
 
@@ -745,14 +748,18 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_edges(GraphView g, const 
 }
 
 // Invalidated vertices that are active on their own (CC: own label) join the
-// frontier: one thread per bitmap word.
+// frontier: one thread per bitmap word. The sweep has read the bitmap and its
+// coarse filter for the last time: both are cleared here for the next batch.
 template <class Op>
-__global__ void k_inval_activate(uint32_t nv, Op op, const uint32_t *inval, const typename Op::value_t *value,
-                                 const typename Op::buffer_t *buffer, uint32_t *stamp, uint32_t stamp_val, uint32_t *list,
-                                 uint32_t *cnt, uint32_t cap, unsigned long long *overflow) {
+__global__ void k_inval_activate(uint32_t nv, Op op, uint32_t *inval, uint32_t *coarse, uint32_t cshift,
+                                 const typename Op::value_t *value, const typename Op::buffer_t *buffer, uint32_t *stamp,
+                                 uint32_t stamp_val, uint32_t *list, uint32_t *cnt, uint32_t cap,
+                                 unsigned long long *overflow) {
     const uint32_t nw = (nv + 31) / 32;
     for (uint64_t wd = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; wd < nw; wd += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t bits = inval[wd];
+        if (bits) inval[wd] = 0;
+        if ((wd & ((1ull << cshift) - 1)) == 0) coarse[wd >> cshift] = 0;  // coarse word c covers words c << cshift ..
         while (bits) {
             const uint32_t b = __builtin_ctz(bits);
             bits &= bits - 1;
