@@ -18,7 +18,7 @@ import sgraph
 RESIDENT_SPACE = {
     "wave_budget": [512, 1024, 2048],
     "heavy_degree": [2048, 8192],
-    "rounds_per_sync": [2, 4, 8],
+    "rounds_per_sync": [0, 2, 4, 8],
     "round_blocks": [1024, 2048],
 }
 # knobs of the out-of-memory path

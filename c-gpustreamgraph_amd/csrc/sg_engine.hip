@@ -755,6 +755,8 @@ public:
     uint32_t *inval_ = nullptr;   // bitmap of invalidated vertices
     uint32_t *coarse_ = nullptr;  // coarse filter of inval_: one bit per 2^cshift_ vertices
     uint32_t cshift_ = 6;
+    static constexpr int kPhaseInval = 0, kPhaseRelax = 1;
+    uint32_t last_rounds_[2] = {0, 0};  // rounds the phase took at the last snapshot
     std::unordered_map<const void *, int> occ_;  // resident blocks per CU of a kernel
     int num_cus_ = 0;
     uint32_t R_ = 1;             // global round counter (stamps)
@@ -1193,19 +1195,24 @@ public:
     // last busy one find empty inputs and return at once. make_fns(stamp_for_next)
     // returns {vfn, efn}. Timed runs count into stats [8..11]. Returns the rounds run.
     template <class MakeFns>
-    uint32_t run_rounds(MakeFns make_fns, bool timed) {
+    uint32_t run_rounds(MakeFns make_fns, bool timed, int phase = -1) {
         const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
         return drive_rounds([&](const PassIO &io) {
             auto fns = make_fns(io.round + 1);
             launch_pass(io, fns.first, fns.second, do_time);
-        }, timed);
+        }, timed, phase);
     }
 
     // launch(io) enqueues round io.round (timed with begin_timed / end_timed
-    // when the run is timed and SG_FLAG_KERNEL_TIMING is set).
+    // when the run is timed and SG_FLAG_KERNEL_TIMING is set). phase (kPhase*,
+    // or -1) names a per-snapshot phase: with rounds_per_sync = 0 its first
+    // batch is the rounds it took last time plus one, so a phase whose round
+    // count is steady costs one wait and at most one empty round.
     template <class Launch>
-    uint32_t drive_rounds(Launch launch, bool timed) {
-        uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(cfg_.rounds_per_sync ? cfg_.rounds_per_sync : 4, kRoundLog));
+    uint32_t drive_rounds(Launch launch, bool timed, int phase = -1) {
+        uint32_t k0 = cfg_.rounds_per_sync ? cfg_.rounds_per_sync : 4;
+        if (!cfg_.rounds_per_sync && phase >= 0 && last_rounds_[phase]) k0 = last_rounds_[phase] + 1;
+        uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(k0, kRoundLog));
         const uint32_t maxr = cfg_.max_rounds ? cfg_.max_rounds : 1000000u;
         const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
         if (timed) fill<unsigned long long>(stats_ + 8, 8, 0ull);
@@ -1242,6 +1249,7 @@ public:
                 last_appended_ += front;
                 if (front == 0 && spill == 0) {
                     finish_rounds(timed);
+                    if (phase >= 0) last_rounds_[phase] = rounds;
                     return rounds;
                 }
             }
@@ -1725,7 +1733,7 @@ public:
             SG_LAUNCHED();
             run_rounds([this](uint32_t) {
                 return std::make_pair(VAll{}, EInvalidate<Op>{op_, value_, buffer_, inval_, coarse_, cshift_});
-            }, false);
+            }, false, kPhaseInval);
         }
         // the invalidation rounds' last read of the control block holds the count
         // every vertex the invalidation seeds and rounds appended was reset once
@@ -1753,7 +1761,7 @@ public:
             SG_LAUNCHED();
         }
         // 3. converge
-        st_.rounds = run_rounds([this](uint32_t sv) { return relax_fns(sv); }, true);
+        st_.rounds = run_rounds([this](uint32_t sv) { return relax_fns(sv); }, true, kPhaseRelax);
         finish_batch();
         st_.snapshots += 1;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1861,7 +1869,7 @@ public:
             launch_pass(io, VAmend<api::PageRank>{op_, +1, value_, buffer_},
                         ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0}, false);
         }
-        st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, true);
+        st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, true, kPhaseRelax);
         finish_batch();
         st_.snapshots += 1;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1975,9 +1983,9 @@ public:
     }
 
     template <int kMode>
-    uint32_t mrounds(bool timed) {
+    uint32_t mrounds(bool timed, int phase = -1) {
         const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
-        return drive_rounds([&](const PassIO &io) { mround<kMode>(io, do_time); }, timed);
+        return drive_rounds([&](const PassIO &io) { mround<kMode>(io, do_time); }, timed, phase);
     }
 
     template <int kMode>
@@ -2064,7 +2072,7 @@ public:
         uint64_t ninval = 0;
         if (ndel_) {
             mpairs<1>(ndel_, R_, nullptr);
-            mrounds<1>(false);
+            mrounds<1>(false, kPhaseInval);
             ninval = last_appended_;  // vertices put on the invalidation frontiers (> 0: some slot reset)
         }
         st_.invalidated = ninval;
@@ -2089,7 +2097,7 @@ public:
         }
         mpairs<0>(nins_, R, nullptr);
         // 3. converge
-        st_.rounds = mrounds<0>(true);
+        st_.rounds = mrounds<0>(true, kPhaseRelax);
         finish_batch();
         st_.snapshots += 1;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2172,7 +2180,7 @@ void sg_config_default(sg_config *c) {
     c->n_stream = 3;
     c->block_size = 512;
     c->max_rounds = 0;
-    c->rounds_per_sync = 4;
+    c->rounds_per_sync = 0;  // adaptive
     c->flags = 0;
     c->device_budget_gb = 0;
     c->device = 0;

@@ -63,7 +63,9 @@ typedef struct sg_config {
     uint32_t n_stream;        /* -n_stream streams, app_skeleton.h:89                     */
     uint32_t block_size;      /* threads per relax block (multiple of 64, <= 1024)        */
     uint32_t max_rounds;      /* cap on convergence rounds per phase (0 = 1000000)        */
-    uint32_t rounds_per_sync; /* rounds enqueued between host convergence checks (0 = 4)  */
+    uint32_t rounds_per_sync; /* rounds enqueued between host convergence checks; 0 =
+                                 adaptive: a snapshot phase first enqueues the rounds it
+                                 took last time + 1 (4 at first), then doubles         */
     uint32_t flags;           /* SG_FLAG_*                                                */
     double device_budget_gb;  /* cap on engine device memory in GiB (0 = no cap)          */
     int32_t device;           /* HIP device ordinal                                       */
