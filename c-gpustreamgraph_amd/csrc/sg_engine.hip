@@ -418,7 +418,9 @@ __global__ void __launch_bounds__(kBlock) k_win_gather(GraphView g, BatchView bv
         uint64_t ntail = 0;
         if (bit_test(mark, v)) {
             const TInfo ti = tinfo[tidx[v]];
-            if (ti.de > ti.db) {  // survivors of old list + inserts, staged by the filter chunk by chunk
+            if (ti.de > ti.db && chunks[ti.cbase].ibase == kInPlace) {  // compacted where it lies
+                keep_old = (uint32_t)(cpos[ti.cbase + 1] - cpos[ti.cbase]);
+            } else if (ti.de > ti.db) {  // survivors of old list + inserts, staged by the filter chunk by chunk
                 for (uint32_t c = ti.cbase; c < ti.cbase + ti.nch; ++c) {
                     const uint32_t *src = merged + chunks[c].ibase;
                     const uint64_t at = cpos[c] - cpos[ti.cbase], cnt = cpos[c + 1] - cpos[c];
@@ -1076,6 +1078,10 @@ public:
         const uint64_t resident = (uint64_t)per_cu * (uint64_t)std::max(num_cus_, 1);
         return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((ntiles + kSweepWaves - 1) / kSweepWaves, resident));
     }
+    // An engine that walks the old out-lists between prepare and commit
+    // (PageRank's cancellation) keeps them intact: no chunk is compacted in
+    // place during prepare.
+    virtual bool reads_old_lists() const { return false; }
     uint32_t heavy_deg() const { return std::max<uint32_t>(kChunk, cfg_.heavy_degree ? cfg_.heavy_degree : kHeavyDeg); }
 
     // ---- passes and rounds ---------------------------------------------------------
@@ -1345,11 +1351,12 @@ public:
             SG_LAUNCHED();
             const uint32_t gt = grid_for((uint64_t)nins + ndel, kBlock, 8192);
             hipLaunchKernelGGL(k_touch_info, dim3(gt), dim3(kBlock), 0, stream_, gview(), br_, touched_, &ctl_->tcount, tinfo_,
-                               tidx_, chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, &ctl_->seqcur, &ctl_->pad[1]);
+                               tidx_, chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, &ctl_->seqcur,
+                               reads_old_lists() ? 0 : 1, &ctl_->pad[1]);
             SG_LAUNCHED();
             begin_timed(timing(), SG_KCLASS_MERGE);
             hipLaunchKernelGGL(k_chunk_filter, dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_,
-                               (const MChunk *)chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, ccount_, tmp_);
+                               (const MChunk *)chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, ccount_, tmp_, col_);
             SG_LAUNCHED();
             end_timed(timing());
         }
@@ -1800,6 +1807,7 @@ public:
 class PageRankEngine final : public Engine {
 public:
     using Engine::Engine;
+    bool reads_old_lists() const override { return true; }
     api::PageRank op_{};
     float *value_ = nullptr, *buffer_ = nullptr;
     double *d_sum_ = nullptr;

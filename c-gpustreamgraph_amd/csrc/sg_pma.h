@@ -61,6 +61,11 @@ struct MChunk {
     uint32_t first;   // the vertex's first chunk (TInfo::cbase)
     uint64_t ibase;   // where the filter writes this chunk's survivors (vertices with deletes)
 };
+// ibase of a chunk filtered in place: the vertex has deletes, its whole
+// sequence is this one chunk and fits the vertex's slot range, so one wave
+// holds all of it in registers and compacts it where it lies (no staging).
+constexpr uint64_t kInPlace = ~0ull;
+
 __device__ __forceinline__ TInfo chunk_info(const MChunk &m) { return TInfo{m.u, m.d0, 0, m.ib, m.db, m.de, m.first, 0}; }
 __device__ __forceinline__ MChunk load_chunk(const MChunk *chunks, uint64_t c, uint64_t n) {
     if (c < n) return chunks[c];
@@ -131,7 +136,8 @@ __device__ __forceinline__ bool chunked(const TInfo &t) { return has_del(t) || t
 __global__ void __launch_bounds__(kBlock) k_touch_info(GraphView g, BatchRanges br, const uint32_t *touched,
                                                        const uint32_t *tcount, TInfo *tinfo, uint32_t *tidx,
                                                        MChunk *chunks, uint32_t *nchunks, uint32_t chunk_cap,
-                                                       unsigned long long *seqcur, unsigned long long *overflow) {
+                                                       unsigned long long *seqcur, int allow_inplace,
+                                                       unsigned long long *overflow) {
     __shared__ uint32_t s_c[kWavesPerBlock], s_cb;
     __shared__ unsigned long long s_s[kWavesPerBlock], s_sb;
     const uint32_t n = *tcount;
@@ -150,10 +156,12 @@ __global__ void __launch_bounds__(kBlock) k_touch_info(GraphView g, BatchRanges 
             nch = chunked(ti) ? (seq_len(ti) + kMergeChunk - 1) / kMergeChunk : 0;
             ti.nch = nch;
         }
+        const bool inplace = allow_inplace && t < n && has_del(ti) && nch == 1 &&
+                             g.off[ti.u + 1] - g.off[ti.u] >= seq_len(ti);
         // chunk indexes and staging ranges: wave scans, then one atomic per
         // counter and block
         const uint32_t incl = wave_incl_scan(nch);
-        const unsigned long long sl = (t < n && has_del(ti)) ? seq_len(ti) : 0ull;
+        const unsigned long long sl = (t < n && has_del(ti) && !inplace) ? seq_len(ti) : 0ull;
         const unsigned long long sincl = wave_incl_scan(sl);
         const uint32_t w = threadIdx.x / kWave;
         if (lane_id() == kWave - 1) {
@@ -200,7 +208,7 @@ __global__ void __launch_bounds__(kBlock) k_touch_info(GraphView g, BatchRanges 
                 m.db = ti.db;
                 m.de = ti.de;
                 m.first = ti.cbase;
-                m.ibase = sb + m.lo;
+                m.ibase = inplace ? kInPlace : sb + m.lo;
                 chunks[ti.cbase + c] = m;
             }
         }
@@ -253,12 +261,15 @@ __device__ __forceinline__ bool survives(uint32_t x, const BatchView &bv, const 
 }
 
 // One wave per chunk of a vertex with deletes: write the survivors, in
-// order, to out[ibase ...] and count them. Chunks of insert-only vertices
-// keep everything: counted without reads (they are placed straight from the
-// sorted batch).
+// order, to out[ibase ...] and count them; an in-place chunk (kInPlace)
+// compacts its survivors into the vertex's own slots instead (every load is
+// in registers before the first store, a survivor only moves left, and slots
+// already holding their survivor are not rewritten) and gap-fills the freed
+// tail. Chunks of insert-only vertices keep everything: counted without reads
+// (they are placed straight from the sorted batch).
 __global__ void __launch_bounds__(kBlock) k_chunk_filter(GraphView g, BatchView bv, const MChunk *chunks,
                                                          const uint32_t *nchunks, uint32_t chunk_cap,
-                                                         unsigned long long *ccount, uint32_t *out) {
+                                                         unsigned long long *ccount, uint32_t *out, uint32_t *col) {
     __shared__ uint32_t s_del[kWavesPerBlock][kDelStage];
     __shared__ uint32_t s_mask[kWavesPerBlock][kDelMaskWords];
     const uint32_t n = min(*nchunks, chunk_cap);
@@ -285,15 +296,19 @@ __global__ void __launch_bounds__(kBlock) k_chunk_filter(GraphView g, BatchView 
             x[j] = i < m.hi ? (i < ti.d0 ? old[i] : ins[i - ti.d0]) : 0;
         }
         uint32_t cnt = 0;
-        uint32_t *dst = out + m.ibase;
+        const bool inpl = m.ibase == kInPlace;
+        uint32_t *dst = inpl ? col + m.base : out + m.ibase;
 #pragma unroll
         for (int j = 0; j < kMergePerLane; ++j) {  // survivors stay in sequence order
             const uint32_t i = m.lo + j * kWave + lane;
             const bool keep = i < m.hi && survives(x[j], bv, ti, s_del[w], s_mask[w], ns);
             const uint64_t b = __ballot(keep);
-            if (keep) dst[cnt + __popcll(b & lanemask_lt())] = x[j];
+            const uint32_t pos = cnt + (uint32_t)__popcll(b & lanemask_lt());
+            if (keep && (!inpl || pos != i)) dst[pos] = x[j];
             cnt += (uint32_t)__popcll(b);
         }
+        if (inpl)
+            for (uint32_t k = cnt + lane; k < ti.d0; k += kWave) dst[k] = kGap;
         if (lane == 0) ccount[c] = cnt;
         __builtin_amdgcn_wave_barrier();
     }
@@ -324,6 +339,7 @@ __global__ void __launch_bounds__(kBlock) k_chunk_place(GraphView g, BatchView b
         const MChunk m = mn;
         mn = load_chunk(chunks, c + nw, n);
         const TInfo ti = chunk_info(m);
+        if (m.ibase == kInPlace) continue;  // compacted by the filter
         if (skip_covered && covered(maximal, nleaves, L, ti.u)) continue;
         uint32_t *lst = col + m.base;
         const bool hd = has_del(ti);

@@ -156,6 +156,18 @@ def test_dense_pma_forces_rebalance_and_resize(algo):
     assert st["rebalances"] > 0 and st["resizes"] > 0
 
 
+@pytest.mark.parametrize("algo", ["bfs", "cc"])
+def test_compaction_in_place_with_rebalance(algo):
+    # many deletes next to hub-biased inserts in a dense PMA: short lists with
+    # deletes are compacted where they lie during the batch's preparation, long
+    # or overflowing ones are staged, and window re-spreads move both kinds
+    nv, row, col = sgraph.gen_rmat(11, 8, seed=43)
+    ins, dele = sgraph.gen_updates(row, col, 11, 4, 2500, 2000, seed=44, hub_frac=0.8, hub_top=0.02)
+    st = run_stream(algo, row, col, list(zip(ins, dele)), pma_density=0.85)
+    say(f"[gpu] rebalances={st['rebalances']} resizes={st['resizes']}")
+    assert st["rebalances"] > 0
+
+
 @pytest.mark.parametrize("algo", ALGOS)
 def test_edge_cases(algo):
     # isolated source, a deleted-then-reinserted edge, an edge inserted and
