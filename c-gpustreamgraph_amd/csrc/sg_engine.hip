@@ -482,7 +482,7 @@ __global__ void k_init_state(uint32_t nv, Op op, typename Op::value_t *value, ty
 }
 
 // RebuildWorklist (driver.cuh:654-674): every active vertex joins the frontier.
-template <class Op>
+template <class Op, bool kBothSigns = false>
 __global__ void __launch_bounds__(kBlock) k_seed_active(uint32_t nv, Op op, const typename Op::value_t *value,
                                                         const typename Op::buffer_t *buffer, uint32_t *stamp,
                                                         uint32_t stamp_val, uint32_t *list, uint32_t *cnt,
@@ -494,6 +494,7 @@ __global__ void __launch_bounds__(kBlock) k_seed_active(uint32_t nv, Op op, cons
         bool p = false;
         if (v < nv) {
             p = op.IsActiveNode((uint32_t)v, buffer[v], value[v]);
+            if constexpr (kBothSigns) p = p || op.IsActiveNode_INC((uint32_t)v, buffer[v], value[v]);
             if (p) stamp[v] = stamp_val;
         }
         block_append(p, (uint32_t)v, list, cnt, nv, overflow, sh);
@@ -1641,6 +1642,8 @@ public:
         fail(SG_EINVAL, "query not available pointwise");
     }
     virtual void topk(uint32_t, uint32_t *, float *) { fail(SG_EINVAL, "sg_pr_topk needs a PageRank engine"); }
+    // Bring deferred snapshots to their fixed point (SG_FLAG_DEFER_CONVERGE).
+    virtual void converge() {}
 
     uint32_t *gat_v_ = nullptr, *gat_o_ = nullptr;
     size_t gat_v_cap_ = 0, gat_o_cap_ = 0;
@@ -1877,11 +1880,38 @@ public:
             launch_pass(io, VAmend<api::PageRank>{op_, +1, value_, buffer_},
                         ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0}, false);
         }
-        st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, true, kPhaseRelax);
+        if (cfg_.flags & SG_FLAG_DEFER_CONVERGE) {
+            // the long new lists were cut into chunks for the first round: walk
+            // them now. The residual then holds alpha * (A' - A) * value on top
+            // of the old one, exact for any committed value: later snapshots add
+            // theirs and one convergence serves them all
+            drain_seed_spill(R, ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0});
+            pending_ = true;
+            st_.rounds = 0;
+        } else {
+            st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, true, kPhaseRelax);
+        }
         finish_batch();
         st_.snapshots += 1;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (!pending_) residual();
+    }
+
+    // Deferred snapshots: every vertex whose residual is beyond +-epsilon
+    // starts the rounds (the frontiers of the deferred passes were not kept;
+    // cancellations leave negative residuals).
+    bool pending_ = false;
+    void converge() override {
+        if (!pending_) return;
+        auto t0 = std::chrono::steady_clock::now();
+        frontier_reset();
+        hipLaunchKernelGGL((k_seed_active<api::PageRank, true>), dim3(grid_for(nv_, kBlock, 16384)), dim3(kBlock), 0, stream_, nv_, op_,
+                           value_, buffer_, stamp_, R_, q_[R_ & 1], cnt_ + (R_ % 3), stats_ + 3);
+        SG_LAUNCHED();
+        st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, true, kPhaseRelax);
+        pending_ = false;
         residual();
+        st_.last_phase_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 
     void word_source(uint32_t q, uint32_t qi, const uint32_t *&base, uint64_t &stride, uint32_t &word) override {
@@ -2275,14 +2305,20 @@ sg_status sg_gather(sg_engine *eng, uint32_t query, void *out, size_t bytes) {
     return guarded(eng, [&](sg::Engine &e) {
         if (!out) sg::fail(SG_EINVAL, "null output");
         if (!e.loaded_) sg::fail(SG_EINVAL, "no graph loaded");
+        e.converge();
         e.gather(query, out, bytes);
     });
+}
+
+sg_status sg_converge(sg_engine *eng) {
+    return guarded(eng, [&](sg::Engine &e) { e.converge(); });
 }
 
 sg_status sg_gather_at(sg_engine *eng, uint32_t query, uint32_t qi, const uint32_t *vertices, uint64_t n, uint32_t *out) {
     return guarded(eng, [&](sg::Engine &e) {
         if (n && (!vertices || !out)) sg::fail(SG_EINVAL, "null vertex list or output");
         if (!e.loaded_) sg::fail(SG_EINVAL, "no graph loaded");
+        e.converge();
         e.gather_at(query, qi, vertices, n, out);
     });
 }
@@ -2304,6 +2340,7 @@ sg_status sg_device_memory(int32_t device, uint64_t *free_bytes, uint64_t *total
 sg_status sg_pr_topk(sg_engine *eng, uint32_t k, uint32_t *ids, float *ranks) {
     return guarded(eng, [&](sg::Engine &e) {
         if (k && !ids) sg::fail(SG_EINVAL, "null id output");
+        e.converge();
         e.topk(k, ids, ranks);
     });
 }

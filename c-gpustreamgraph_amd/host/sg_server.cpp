@@ -1,8 +1,9 @@
 // Request loop over one dynamic graph (declared in sgraph_serve.h). No
 // counterpart in the reference, whose drivers run a fixed snapshot list with
 // one source per process (samples/hybrid_*/main.cu). The worker thread owns
-// every engine handle; submitters only touch the request table under the
-// mutex.
+// the batched-query and CC engines; a second thread owns the PageRank engine,
+// so its long convergences overlap the queries on the device instead of
+// delaying them. Submitters only touch the request tables under the mutex.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -44,6 +45,12 @@ double pct(std::vector<double> v, double p) {
 
 }  // namespace
 
+// An update batch the worker has applied, on its way to the PageRank thread.
+struct PrUpdate {
+    uint64_t snapshot = 0;
+    std::vector<uint32_t> ins, del;
+};
+
 struct sg_server {
     sg_server_config cfg{};
     uint32_t nv = 0;
@@ -59,6 +66,17 @@ struct sg_server {
     bool stop = false;
     std::thread worker;
     std::string error;
+
+    // PageRank thread: applies the worker's snapshots in order with deferred
+    // convergence and answers top-k requests on the snapshot it has reached
+    // (converging once for all snapshots since the last answer).
+    std::deque<PrUpdate> q_pr;
+    std::deque<uint64_t> q_topk;
+    std::condition_variable cv_pr;
+    std::thread pr_worker;
+    bool pr_stop = false;
+    uint64_t pr_snapshot = 0;
+    std::string pr_error;
 
     uint64_t snapshot = 0, step = 0;
     std::vector<double> lat_all, lat_q, lat_u;
@@ -120,7 +138,7 @@ struct sg_server {
         for (uint64_t id : U) {
             Request *r = req(id);
             const uint64_t ni = r->ins.size() / 2, nd = r->del.size() / 2;
-            for (sg_engine *e : {multi, cc, pr}) {
+            for (sg_engine *e : {multi, cc}) {
                 if (!e) continue;
                 const sg_status s = sg_apply_snapshot(e, r->ins.data(), ni, r->del.data(), nd);
                 if (s != SG_OK) { fail_request(r, s, engine_error(e)); break; }
@@ -128,6 +146,15 @@ struct sg_server {
             ++snapshot;
             st.edge_updates += ni + nd;
             r->info.snapshot = snapshot;
+            if (pr) {
+                PrUpdate pu;
+                pu.snapshot = snapshot;
+                pu.ins = std::move(r->ins);
+                pu.del = std::move(r->del);
+                std::lock_guard<std::mutex> lk(mu);
+                q_pr.push_back(std::move(pu));
+                cv_pr.notify_one();
+            }
             r->ins.clear();
             r->ins.shrink_to_fit();
             r->del.clear();
@@ -138,19 +165,14 @@ struct sg_server {
         sg_stats es{};
         uint64_t ne = 0;
         if (sg_get_stats(multi, &es) == SG_OK) ne = es.ne;
-        // 2. lookups on the current snapshot
+        // 2. CC label lookups on the current snapshot
         std::vector<uint32_t> verts;
         std::vector<Request *> ccr;
-        uint32_t kmax = 0;
         for (uint64_t id : L) {
             Request *r = req(id);
             r->info.snapshot = snapshot;
-            if (r->kind == SG_REQ_CC) {
-                verts.push_back(r->arg);
-                ccr.push_back(r);
-            } else {
-                kmax = std::max(kmax, r->arg);
-            }
+            verts.push_back(r->arg);
+            ccr.push_back(r);
             done.push_back({id, r});
         }
         if (!ccr.empty()) {
@@ -162,21 +184,10 @@ struct sg_server {
                 std::memcpy(ccr[i]->result.data(), &out[i], 4);
             }
         }
-        if (kmax) {
-            std::vector<uint32_t> ids(kmax);
-            std::vector<float> rk(kmax);
-            const sg_status s = sg_pr_topk(pr, kmax, ids.data(), rk.data());
-            for (uint64_t id : L) {
-                Request *r = req(id);
-                if (r->kind != SG_REQ_PR_TOPK) continue;
-                if (s != SG_OK) { fail_request(r, s, engine_error(pr)); continue; }
-                const uint32_t k = std::min<uint32_t>(r->arg, nv);
-                r->result.resize((size_t)k * 8);
-                std::memcpy(r->result.data(), ids.data(), (size_t)k * 4);
-                std::memcpy(r->result.data() + (size_t)k * 4, rk.data(), (size_t)k * 4);
-            }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            st.lookups += L.size();
         }
-        st.lookups += L.size();
         // 3. queries admitted this step, converged together
         if (!Q.empty()) {
             std::vector<uint32_t> src, kinds;
@@ -209,6 +220,60 @@ struct sg_server {
             st.max_batch = std::max<uint64_t>(st.max_batch, Q.size());
         }
         complete(done);
+    }
+
+    void pr_loop() {
+        for (;;) {
+            std::vector<PrUpdate> U;
+            std::vector<uint64_t> K;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_pr.wait(lk, [&] { return pr_stop || !q_pr.empty() || !q_topk.empty(); });
+                if (pr_stop && q_pr.empty() && q_topk.empty()) break;
+                for (auto &u : q_pr) U.push_back(std::move(u));
+                q_pr.clear();
+                K.assign(q_topk.begin(), q_topk.end());
+                q_topk.clear();
+            }
+            for (auto &u : U) {
+                if (!pr_error.empty()) break;
+                const sg_status s = sg_apply_snapshot(pr, u.ins.data(), u.ins.size() / 2, u.del.data(), u.del.size() / 2);
+                if (s != SG_OK) pr_error = engine_error(pr);
+                else pr_snapshot = u.snapshot;
+            }
+            if (K.empty()) continue;
+            std::vector<std::pair<uint64_t, Request *>> done;
+            uint32_t kmax = 0;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                for (uint64_t id : K) {
+                    Request *r = reqs[id].get();
+                    kmax = std::max(kmax, std::min<uint32_t>(r->arg, nv));
+                    done.push_back({id, r});
+                }
+            }
+            std::vector<uint32_t> ids(kmax);
+            std::vector<float> rk(kmax);
+            sg_status s = SG_EDEVICE;
+            if (pr_error.empty()) {
+                s = sg_pr_topk(pr, kmax, ids.data(), rk.data());  // converges the deferred snapshots
+                if (s != SG_OK) pr_error = engine_error(pr);
+            }
+            for (auto &d : done) {
+                Request *r = d.second;
+                r->info.snapshot = pr_snapshot;
+                if (s != SG_OK) { fail_request(r, s, pr_error); continue; }
+                const uint32_t k = std::min<uint32_t>(r->arg, nv);
+                r->result.resize((size_t)k * 8);
+                std::memcpy(r->result.data(), ids.data(), (size_t)k * 4);
+                std::memcpy(r->result.data() + (size_t)k * 4, rk.data(), (size_t)k * 4);
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                st.lookups += done.size();
+            }
+            complete(done);
+        }
     }
 
     void loop() {
@@ -279,7 +344,11 @@ sg_status sg_server_create(const sg_server_config *cfg, uint32_t nv, uint64_t ne
     std::string err;
     sg_status st = make_engine(cfg, SG_ALGO_MULTI, nv, ne, row, col, &s->multi, err);
     if (st == SG_OK && (cfg->standing & SG_SERVE_CC)) st = make_engine(cfg, SG_ALGO_CC, nv, ne, row, col, &s->cc, err);
-    if (st == SG_OK && (cfg->standing & SG_SERVE_PR)) st = make_engine(cfg, SG_ALGO_PR, nv, ne, row, col, &s->pr, err);
+    if (st == SG_OK && (cfg->standing & SG_SERVE_PR)) {
+        sg_server_config pc = *cfg;
+        pc.engine.flags |= SG_FLAG_DEFER_CONVERGE;  // one convergence per top-k answer
+        st = make_engine(&pc, SG_ALGO_PR, nv, ne, row, col, &s->pr, err);
+    }
     if (st != SG_OK) {
         sg_destroy(s->multi);
         sg_destroy(s->cc);
@@ -289,8 +358,17 @@ sg_status sg_server_create(const sg_server_config *cfg, uint32_t nv, uint64_t ne
     s->t_start = Clock::now();
     sg_server *raw = s.get();
     try {
+        if (s->pr) s->pr_worker = std::thread([raw] { raw->pr_loop(); });
         s->worker = std::thread([raw] { raw->loop(); });
     } catch (...) {
+        if (s->pr_worker.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->pr_stop = true;
+            }
+            s->cv_pr.notify_all();
+            s->pr_worker.join();
+        }
         sg_destroy(s->multi);
         sg_destroy(s->cc);
         sg_destroy(s->pr);
@@ -339,10 +417,15 @@ sg_status sg_server_submit(sg_server *s, uint32_t kind, uint32_t arg, uint32_t t
     if (s->stop) { s->error = "server is stopping"; return SG_EINVAL; }
     const uint64_t id = s->next_id++;
     s->reqs[id] = std::move(r);
+    *ticket = id;
+    if (kind == SG_REQ_PR_TOPK) {
+        s->q_topk.push_back(id);
+        s->cv_pr.notify_one();
+        return SG_OK;
+    }
     if (kind == SG_REQ_UPDATE) s->q_updates.push_back(id);
     else if (kind == SG_REQ_BFS || kind == SG_REQ_SSSP) s->q_queries.push_back(id);
     else s->q_lookups.push_back(id);
-    *ticket = id;
     s->cv_work.notify_one();
     return SG_OK;
 }
@@ -392,7 +475,13 @@ void sg_server_destroy(sg_server *s) {
         s->stop = true;
     }
     s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    if (s->worker.joinable()) s->worker.join();  // may still hand snapshots to the PageRank thread
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->pr_stop = true;
+    }
+    s->cv_pr.notify_all();
+    if (s->pr_worker.joinable()) s->pr_worker.join();
     sg_destroy(s->multi);
     sg_destroy(s->cc);
     sg_destroy(s->pr);

@@ -22,6 +22,7 @@ WEIGHT = {"hash128": 0, "seeded": 1, "one": 2}
 Q_VALUE, Q_BUFFER, Q_PARENT, Q_DEGREE, Q_OFFSETS, Q_SLOTS, Q_SEGMENTS = range(7)
 FLAG_KERNEL_TIMING = 0x1
 FLAG_HOST_EDGES = 0x2
+FLAG_DEFER_CONVERGE = 0x4
 KCLASS = ("relax", "sweep", "merge")
 INF = 0xFFFFFFFF
 DEFAULT_SEED = 27491095  # kRandSeed of the reference (groute/graphs/util.h:68)
@@ -105,6 +106,7 @@ SIGNATURES = [
     ("sg_init_queries", C.c_int, [_P, _P, _P, C.c_uint32]),
     ("sg_gather_at", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint64, _P]),
     ("sg_pr_topk", C.c_int, [_P, C.c_uint32, _P, _P]),
+    ("sg_converge", C.c_int, [_P]),
     ("sg_device_memory", C.c_int, [C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("sg_server_config_default", None, [C.POINTER(ServerConfig)]),
     ("sg_server_create", C.c_int, [C.POINTER(ServerConfig), C.c_uint32, C.c_uint64, _P, _P, C.POINTER(_P)]),
@@ -298,6 +300,10 @@ class Engine:
         out = np.empty(len(v), dtype=np.uint32)
         _check(lib().sg_gather_at(self.h, Q_VALUE if q is None else q, qi, _ptr(v), len(v), _ptr(out)), self.h)
         return out
+
+    def converge(self):
+        """Converge snapshots deferred by FLAG_DEFER_CONVERGE (PageRank)."""
+        _check(lib().sg_converge(self.h), self.h)
 
     def topk(self, k):
         ids = np.empty(k, dtype=np.uint32)

@@ -86,6 +86,12 @@ typedef struct sg_config {
 /* Keep the edge array in pinned host memory whatever its size (the
  * out-of-memory path of -cache / -SEGMENT, framework.cuh:310-384). */
 #define SG_FLAG_HOST_EDGES 0x2u
+/* PageRank only: sg_apply_snapshot updates the graph and the residuals
+ * (cancel over the old out-lists, compensate over the new ones) but runs no
+ * convergence rounds; sg_converge, or the next sg_gather / sg_gather_at /
+ * sg_pr_topk, converges once for all deferred snapshots. The fixed point is
+ * the same as converging after each one. Other algorithms ignore the flag. */
+#define SG_FLAG_DEFER_CONVERGE 0x4u
 
 typedef struct sg_stats {
     uint32_t nv;                  /* vertices                                          */
@@ -217,6 +223,11 @@ sg_status sg_gather_at(sg_engine *eng, uint32_t query, uint32_t qi, const uint32
  * ranks on the host after a full copy (hybrid_pr_host.cu:161-229, -top_ranks);
  * here the ranking is a device sort and only k entries cross the link. */
 sg_status sg_pr_topk(sg_engine *eng, uint32_t k, uint32_t *ids, float *ranks);
+
+/* Converge the snapshots deferred by SG_FLAG_DEFER_CONVERGE (a no-op
+ * otherwise). No counterpart in the reference, which converges (or stops
+ * after 100 rounds) inside every snapshot. */
+sg_status sg_converge(sg_engine *eng);
 
 /* Kernel classes timed with HIP events when SG_FLAG_KERNEL_TIMING is set
  * (events recorded on the engine's stream around each launch). */

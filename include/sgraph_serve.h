@@ -16,11 +16,17 @@
  *                    the query slots and the free HBM allow;
  *   CC label lookup  label of one vertex from a standing CC computation that
  *                    every update keeps at its fixed point;
- *   PR top-k         the k highest ranks of a standing PageRank.
+ *   PR top-k         the k highest ranks of a standing PageRank, answered on
+ *                    the latest snapshot its own thread has reached (the
+ *                    snapshot is reported with the answer).
  *
  * A worker thread runs steps: apply pending updates, answer lookups, admit
  * queries, converge them in one launch per round, complete their requests.
- * Submission and waiting are thread-safe; every call returns an sg_status.
+ * A second thread applies the same snapshots, in order, to the PageRank
+ * engine with deferred convergence (SG_FLAG_DEFER_CONVERGE) and converges
+ * once per batch of top-k requests, on its own HIP stream, so PageRank work
+ * overlaps the queries on the device. Submission and waiting are
+ * thread-safe; every call returns an sg_status.
  */
 #ifndef SGRAPH_SERVE_H
 #define SGRAPH_SERVE_H

@@ -119,3 +119,27 @@ def test_pointwise_gather_and_topk_on_engines():
     assert np.array_equal(ranks, vals[ids])
     assert ranks.min() >= np.sort(vals)[::-1][9]
     p.close()
+
+
+def test_deferred_pagerank_converges_once_for_many_snapshots():
+    nv, row, col = sgraph.gen_rmat(11, 8, seed=17)
+    ins, dele = sgraph.gen_updates(row, col, 11, 3, 300, 300, seed=18)
+    lazy = sgraph.Engine("pr", flags=sgraph.FLAG_DEFER_CONVERGE)
+    eager = sgraph.Engine("pr")
+    for e in (lazy, eager):
+        e.load_csr(row, col)
+        e.init()
+    r, c = row, col
+    for t in range(3):
+        lazy.apply(ins[t], dele[t])
+        eager.apply(ins[t], dele[t])
+        assert lazy.stats()["rounds"] == 0  # no rounds inside a deferred snapshot
+        r, c = O.apply_updates(r, c, ins[t], dele[t])
+    got = lazy.values()  # converges the three deferred snapshots
+    exp = O.pagerank(r, c)
+    assert np.all(np.abs(got - exp) <= 5e-5 * exp)
+    assert np.all(np.abs(got - eager.values()) <= 5e-5 * exp)
+    lazy.converge()  # nothing pending: a no-op
+    assert np.array_equal(lazy.values(), got)
+    lazy.close()
+    eager.close()
