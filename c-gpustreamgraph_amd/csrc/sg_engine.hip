@@ -1977,8 +1977,6 @@ public:
     uint32_t *src_ = nullptr;
     MDesc *mspill_[3] = {nullptr, nullptr, nullptr};
     size_t mspill_cap_ = 0;
-    unsigned long long *hits_ = nullptr;
-    size_t hits_cap_ = 0;
     size_t state_nv_ = 0, state_kp_ = 0;
 
     MultiState mstate() const {
@@ -2027,13 +2025,12 @@ public:
     }
 
     template <int kMode>
-    void mpairs(uint64_t n, uint32_t R, const uint32_t *nhits) {
+    void mpairs(uint64_t n, uint32_t R) {
         if (n == 0) return;
         PassIO io = seed_io(R, nullptr, nullptr, 0);
         const uint64_t per = kWave / QS_;
         const uint32_t grid = grid_for((n + per - 1) / per, kWavesPerBlock, 8192);
-        hipLaunchKernelGGL((k_mpairs<kMode>), dim3(grid), dim3(kBlock), 0, stream_, mstate(), bv_, br_,
-                           (const unsigned long long *)hits_, nhits, (uint64_t)hits_cap_, io, stamp_, R);
+        hipLaunchKernelGGL((k_mpairs<kMode>), dim3(grid), dim3(kBlock), 0, stream_, mstate(), bv_, br_, io, stamp_, R);
         SG_LAUNCHED();
     }
 
@@ -2109,31 +2106,27 @@ public:
         frontier_reset();
         uint64_t ninval = 0;
         if (ndel_) {
-            mpairs<1>(ndel_, R_, nullptr);
+            mpairs<1>(ndel_, R_);
             mrounds<1>(false, kPhaseInval);
             ninval = last_appended_;  // vertices put on the invalidation frontiers (> 0: some slot reset)
         }
         st_.invalidated = ninval;
         // 2. seed the recompute: live edges into reset slots (one sweep for all queries) + inserted edges
-        frontier_reset();  // also zeroes the hit counter (word 6)
+        frontier_reset();
         const uint32_t R = R_;
-        uint32_t *nhits = cnt_ + 6;
         if (ninval) {
             const SweepTiles tl = sweep_tiles();
-            ensure(hits_, hits_cap_, ne_ + 1);
             const uint64_t ntiles = ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile;
             const uint32_t grid = sweep_grid(k_msweep, ntiles);
-            hipLaunchKernelGGL(k_msweep, dim3(grid), dim3(kSweepBlock), 0, stream_, gview(), dpre_, tl, (const uint32_t *)inval_,
-                               (const uint32_t *)coarse_, cshift_, hits_, nhits, (uint32_t)std::min<size_t>(hits_cap_, 0xFFFFFFFFu),
-                               stats_ + 3);
+            PassIO io = seed_io(R, nullptr, nullptr, 0);
+            hipLaunchKernelGGL(k_msweep, dim3(grid), dim3(kSweepBlock), 0, stream_, gview(), dpre_, tl, mstate(), io, stamp_,
+                               R);
             SG_LAUNCHED();
-            // every live edge is at most one hit: the list length is bounded by ne
-            mpairs<2>(std::min<uint64_t>(ne_, hits_cap_), R, nhits);
             hipLaunchKernelGGL(k_mclear_inv, dim3(grid_for((nv_ + 31) / 32, 256, 4096)), dim3(256), 0, stream_, nv_, G_,
                                inval_, coarse_, cshift_, minv_);
             SG_LAUNCHED();
         }
-        mpairs<0>(nins_, R, nullptr);
+        mpairs<0>(nins_, R);
         // 3. converge
         st_.rounds = mrounds<0>(true, kPhaseRelax);
         finish_batch();

@@ -374,17 +374,15 @@ __global__ void __launch_bounds__(kBlock) k_mround(GraphView g, MultiState ms, P
 
 // Pairs (u, v) processed per wave step (one per item, QS slots each):
 // kMode 0 relaxes all slots of inserted edges that survive the batch;
-// kMode 1 resets the slots of v whose parent is u for deleted edges;
-// kMode 2 relaxes the reset slots of v (sweep hits, pairs packed u << 32 | v).
+// kMode 1 resets the slots of v whose parent is u for deleted edges.
 template <int kMode>
-__global__ void __launch_bounds__(kBlock) k_mpairs(MultiState ms, BatchView bv, BatchRanges br,
-                                                   const unsigned long long *hits, const uint32_t *nhits, uint64_t hit_cap,
-                                                   PassIO io, uint32_t *stamp, uint32_t stamp_val) {
+__global__ void __launch_bounds__(kBlock) k_mpairs(MultiState ms, BatchView bv, BatchRanges br, PassIO io,
+                                                   uint32_t *stamp, uint32_t stamp_val) {
     __shared__ uint32_t s_out[kWavesPerBlock][kOutStage];
     const uint32_t w = threadIdx.x / kWave;
     const QLane q = qlane(ms);
     OutStage st{s_out[w], 0};
-    const uint64_t n = kMode == 0 ? bv.nins : kMode == 1 ? bv.ndel : min((uint64_t)*nhits, hit_cap);
+    const uint64_t n = kMode == 0 ? bv.nins : bv.ndel;
     const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerBlock + w, nw = (uint64_t)gridDim.x * kWavesPerBlock;
     for (uint64_t b0 = gw * q.per; b0 < n; b0 += nw * q.per) {
         const uint64_t i = b0 + q.sub;
@@ -399,15 +397,11 @@ __global__ void __launch_bounds__(kBlock) k_mpairs(MultiState ms, BatchView bv, 
                     const uint32_t db = br.del_b[u], de = br.del_e[u];
                     ok = db == de || !in_sorted_u64(bv.del_key, db, de, ((uint64_t)u << 32) | v);
                 }
-            } else if (kMode == 1) {
+            } else {
                 const uint64_t k = bv.del_key[i];
                 ok = k != ~0ull;
                 u = (uint32_t)(k >> 32);
                 v = (uint32_t)k;
-            } else {
-                const unsigned long long h = hits[i];
-                u = (uint32_t)(h >> 32);
-                v = (uint32_t)h;
             }
         }
         for (uint32_t grp = 0; grp < ms.G; ++grp) {
@@ -419,10 +413,8 @@ __global__ void __launch_bounds__(kBlock) k_mpairs(MultiState ms, BatchView bv, 
                 const bool child = on && lo32(cur) == u && hi32(cur) != kInf;
                 reset_children(ms, io, st, q, grp, v, child, stamp, stamp_val);
             } else {
-                bool want = on;
-                if (kMode == 2 && on) want = (ms.inv[(uint64_t)v * ms.G + grp] >> q.slot) & 1u;
-                const uint32_t du = want ? hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + qi])) : kInf;
-                const bool hit = want && du != kInf && mrelax(ms, u, v, grp, q.slot, du, kind);
+                const uint32_t du = on ? hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + qi])) : kInf;
+                const bool hit = on && du != kInf && mrelax(ms, u, v, grp, q.slot, du, kind);
                 mark_items(ms, io, st, q, ms.act, grp, v, hit, stamp, stamp_val);
             }
         }
@@ -431,39 +423,23 @@ __global__ void __launch_bounds__(kBlock) k_mpairs(MultiState ms, BatchView bv, 
 }
 
 // Edge-centric sweep for the batch: every live edge (u, v) whose target has a
-// reset slot becomes a hit (u << 32 | v). Same wave tiles as k_sweep_edges;
-// hits are gathered per wave in LDS and appended kHitStage at a time (one
-// atomic per flush).
-constexpr uint32_t kHitStage = 256;
-
-__device__ __forceinline__ void hit_flush(unsigned long long *stage, uint32_t &n, unsigned long long *hits,
-                                          uint32_t *nhits, uint32_t hit_cap, unsigned long long *overflow) {
-    if (n == 0) return;
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(nhits, n);
-    base = __shfl(base, 0, kWave);
-    for (uint32_t i = lane_id(); i < n; i += kWave) {
-        if (base + i < hit_cap) hits[base + i] = stage[i];
-        else atomicOr(overflow, 8ull);
-    }
-    n = 0;
-}
-
+// reset slot relaxes v's reset slots from u (the sweep hits). Same wave tiles
+// as k_sweep_edges: the tile's hits are compacted in place in the wave's LDS
+// (owner into own[k], target into the dead base words) and relaxed 64 / QS at
+// a time, one lane per (hit, slot), as the pair pass does for inserts.
 __global__ void __launch_bounds__(kSweepBlock) k_msweep(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
-                                                        const uint32_t *invbit, const uint32_t *coarse, uint32_t cshift,
-                                                        unsigned long long *hits, uint32_t *nhits, uint32_t hit_cap,
-                                                        unsigned long long *overflow) {
+                                                        MultiState ms, PassIO io, uint32_t *stamp, uint32_t stamp_val) {
     __shared__ SweepWave s_wave[kSweepWaves];
-    __shared__ unsigned long long s_hits[kSweepWaves][kHitStage];
     __shared__ uint32_t s_filter[kFilterWords];
-    __shared__ uint32_t s_cnt[kSweepWaves], s_base;
     const unsigned long long E = dpre[g.nv];
     const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
     SweepWave &sh = s_wave[w];
-    uint32_t nst = 0;  // wave-uniform fill of this wave's hit stage
+    OutStage st{sh.out, 0, kSweepOutStage};
+    const QLane q = qlane(ms);
     uint32_t n_edges = 0;
-    load_filter(s_filter, coarse, coarse_words(g.nv, cshift));
+    const uint32_t cs = ms.cshift;
+    load_filter(s_filter, ms.coarse, coarse_words(g.nv, cs));
     TilePipe tp;
     tp.tend = min<unsigned long long>(ntiles, tiles.cap - 1);
     tp.nw = (unsigned long long)gridDim.x * kSweepWaves;
@@ -476,45 +452,48 @@ __global__ void __launch_bounds__(kSweepBlock) k_msweep(GraphView g, const unsig
         tp.advance(g, dpre, tiles);
         bool pass[kSweepPerLane];
 #pragma unroll
-        for (int j = 0; j < kSweepPerLane; ++j) pass[j] = vv[j] != kGap && coarse_test(s_filter, cshift, vv[j]);
+        for (int j = 0; j < kSweepPerLane; ++j) pass[j] = vv[j] != kGap && coarse_test(s_filter, cs, vv[j]);
         uint32_t word[kSweepPerLane];
 #pragma unroll
-        for (int j = 0; j < kSweepPerLane; ++j) word[j] = pass[j] ? invbit[vv[j] >> 5] : 0u;
+        for (int j = 0; j < kSweepPerLane; ++j) word[j] = pass[j] ? ms.invbit[vv[j] >> 5] : 0u;
+        uint32_t nh = 0;  // wave-uniform
+        uint32_t *const hv = reinterpret_cast<uint32_t *>(sh.base);
 #pragma unroll
         for (int j = 0; j < kSweepPerLane; ++j) {
-            const uint32_t v = vv[j];
-            const bool hit = pass[j] && ((word[j] >> (v & 31)) & 1u);
-            const uint64_t bm = __ballot(hit);
-            if (bm) {
-                const uint32_t c = (uint32_t)__popcll(bm);
-                if (nst + c > kHitStage) hit_flush(s_hits[w], nst, hits, nhits, hit_cap, overflow);
-                if (hit) {
-                    const uint32_t src = v0 + sh.own[j * kWave + lane];
-                    s_hits[w][nst + (uint32_t)__popcll(bm & lanemask_lt())] = ((unsigned long long)src << 32) | v;
+            const bool h = pass[j] && ((word[j] >> (vv[j] & 31)) & 1u);
+            const uint64_t m = __ballot(h);
+            if (m) {
+                const uint32_t o = sh.own[j * kWave + lane];
+                __builtin_amdgcn_wave_barrier();
+                if (h) {
+                    const uint32_t k = nh + (uint32_t)__popcll(m & lanemask_lt());
+                    sh.own[k] = (uint16_t)o;
+                    hv[k] = vv[j];
                 }
-                nst += c;
+                nh += (uint32_t)__popcll(m);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i0 = 0; i0 < nh; i0 += q.per) {
+            const uint32_t i = i0 + q.sub;
+            const bool ok = i < nh;
+            uint32_t u = 0, v = 0;
+            if (ok) {
+                v = hv[i];
+                u = v0 + sh.own[i];
+            }
+            for (uint32_t grp = 0; grp < ms.G; ++grp) {
+                const uint32_t qi = grp * ms.QS + q.slot;
+                const uint8_t kind = ms.kind[qi];
+                const bool want = ok && kind != kQNone && ((ms.inv[(uint64_t)v * ms.G + grp] >> q.slot) & 1u);
+                const uint32_t du = want ? hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + qi])) : kInf;
+                const bool hit = want && du != kInf && mrelax(ms, u, v, grp, q.slot, du, kind);
+                mark_items(ms, io, st, q, ms.act, grp, v, hit, stamp, stamp_val);
             }
         }
         __builtin_amdgcn_wave_barrier();
     }
-    // the partly filled stages of the block's waves: one atomic for all of them
-    if (lane == 0) s_cnt[w] = nst;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tot = 0;
-        for (int i = 0; i < kSweepWaves; ++i) {
-            const uint32_t c = s_cnt[i];
-            s_cnt[i] = tot;
-            tot += c;
-        }
-        s_base = tot ? atomicAdd(nhits, tot) : 0u;
-    }
-    __syncthreads();
-    const uint32_t b0 = s_base + s_cnt[w];
-    for (uint32_t i = lane; i < nst; i += kWave) {
-        if (b0 + i < hit_cap) hits[b0 + i] = s_hits[w][i];
-        else atomicOr(overflow, 8ull);
-    }
+    stage_flush(st, io);
 }
 
 // Clear the masks of the vertices with a reset slot (one thread per bitmap word).
