@@ -1214,11 +1214,13 @@ public:
     // when the run is timed and SG_FLAG_KERNEL_TIMING is set). phase (kPhase*,
     // or -1) names a per-snapshot phase: with rounds_per_sync = 0 its first
     // batch is the rounds it took last time plus one, so a phase whose round
-    // count is steady costs one wait and at most one empty round.
+    // count is steady costs one wait and at most one empty round; a phase that
+    // runs longer continues with batches of 4, 8, ...
     template <class Launch>
     uint32_t drive_rounds(Launch launch, bool timed, int phase = -1) {
         uint32_t k0 = cfg_.rounds_per_sync ? cfg_.rounds_per_sync : 4;
-        if (!cfg_.rounds_per_sync && phase >= 0 && last_rounds_[phase]) k0 = last_rounds_[phase] + 1;
+        const bool adaptive = !cfg_.rounds_per_sync && phase >= 0 && last_rounds_[phase];
+        if (adaptive) k0 = last_rounds_[phase] + 1;
         uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(k0, kRoundLog));
         const uint32_t maxr = cfg_.max_rounds ? cfg_.max_rounds : 1000000u;
         const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
@@ -1264,7 +1266,8 @@ public:
                 finish_rounds(timed);
                 fail(SG_EDEVICE, "no convergence within " + std::to_string(maxr) + " rounds");
             }
-            k = std::min<uint32_t>(2 * k, 64);
+            // past the expected count the remaining rounds are few: restart small
+            k = adaptive && rounds == k ? 4u : std::min<uint32_t>(2 * k, 64);
         }
     }
 
