@@ -217,7 +217,7 @@ __device__ __forceinline__ uint64_t ld_peek(const unsigned long long *p) { retur
 // one per wave-iteration, and coalesced stores.
 constexpr uint32_t kOutStage = 512;
 constexpr uint32_t kWaveBudget = 1024;  // default light-tier edges per 64-item batch of a wave
-constexpr int kIlp = 4;                 // edges per lane per iteration
+constexpr int kIlp = 8;                 // edges per lane per iteration
 
 struct OutStage {
     uint32_t *buf;              // this wave's LDS slice [cap]
