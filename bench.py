@@ -116,7 +116,8 @@ def run_oom(args):
     log(f"[bench] oom graph nv={nv} ne={len(col)} gen {time.perf_counter() - t:.1f}s")
     res = {}
     for a in args.algos.split(","):
-        e = sgraph.Engine(a, edge_budget_gb=args.edge_budget_gb, cache_level=args.cache, segments=args.segments)
+        kw = {"pma_density": args.pma_density} if args.pma_density else {}
+        e = sgraph.Engine(a, edge_budget_gb=args.edge_budget_gb, cache_level=args.cache, segments=args.segments, **kw)
         t = time.perf_counter()
         e.load_csr(row, col)
         e.init()
@@ -152,7 +153,8 @@ def run_oom(args):
         "higher_is_better": True, "dtype": "u32/f32", "data": "synthetic (seeded R-MAT, hub-biased update stream)",
         "config": {"workload": f"PR+CC on R-MAT scale {args.scale} EF{args.edge_factor}, edge budget "
                                f"{args.edge_budget_gb} GiB, -cache {args.cache}, SEGMENT {args.segments}, "
-                               f"{args.batch}+{args.batch} updates/snapshot (80% of inserts from the top-1% sources)",
+                               f"{args.batch}+{args.batch} updates/snapshot (80% of inserts from the top-1% sources)"
+                               + (f", PMA density {args.pma_density}" if args.pma_density else ""),
                    "nv": nv, "ne0": len(col)},
         "latency_p50_ms": round(float(np.percentile(all_lat, 50)), 3),
         "latency_p99_ms": round(float(np.percentile(all_lat, 99)), 3),
@@ -323,6 +325,8 @@ def main():
     ap.add_argument("--edge-budget-gb", type=float, default=8.0, help="oom: device budget of the edge array")
     ap.add_argument("--cache", type=int, default=4, help="oom: -cache level")
     ap.add_argument("--segments", type=int, default=512, help="oom: SEGMENT count")
+    ap.add_argument("--pma-density", type=float, default=0.0,
+                    help="oom: PMA fill density at load (0: engine default); the host edge array is capacity x 4 B")
     ap.add_argument("--duration", type=float, default=20.0, help="serve: seconds of arrivals")
     ap.add_argument("--update-rate", type=float, default=200.0)
     ap.add_argument("--query-rate", type=float, default=500.0)

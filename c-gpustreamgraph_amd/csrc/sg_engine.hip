@@ -1324,7 +1324,8 @@ public:
         chunk_bound_ = chunk_bound;
         ensure(tinfo_, tinfo_cap_, (size_t)nins + ndel + 1);
         // filtered sequences are staged at their own ranges: at most every live edge plus the inserts
-        ensure(tmp_, tmp_cap_, std::max<uint64_t>(ne_ + nins, 1));
+        // (with the edges in host memory the staging is sized to the batch instead, below)
+        if (!host_edges_) ensure(tmp_, tmp_cap_, std::max<uint64_t>(ne_ + nins, 1));
         ensure(chunks_, chunks_cap_, chunk_bound);
         ensure(ccount_, ccount_cap_, chunk_bound + 1);
         ensure(cpos_, cpos_cap_, chunk_bound + 1);
@@ -1358,6 +1359,11 @@ public:
                                tidx_, chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, &ctl_->seqcur,
                                reads_old_lists() ? 0 : 1, &ctl_->pad[1]);
             SG_LAUNCHED();
+            if (host_edges_) {  // HBM holds only what this batch stages, not a copy of the edge array
+                SG_HIP(hipMemcpyAsync(h_ctl_, ctl_, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+                sync();
+                ensure(tmp_, tmp_cap_, std::max<uint64_t>(h_ctl_->seqcur, 1));
+            }
             begin_timed(timing(), SG_KCLASS_MERGE);
             hipLaunchKernelGGL(k_chunk_filter, dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_,
                                (const MChunk *)chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, ccount_, tmp_, col_);
