@@ -425,8 +425,8 @@ __global__ void __launch_bounds__(kBlock) k_mpairs(MultiState ms, BatchView bv, 
 // Edge-centric sweep for the batch: every live edge (u, v) whose target has a
 // reset slot relaxes v's reset slots from u (the sweep hits). Same wave tiles
 // as k_sweep_edges: the tile's hits are compacted in place in the wave's LDS
-// (owner into own[k], target into the dead base words) and relaxed 64 / QS at
-// a time, one lane per (hit, slot), as the pair pass does for inserts.
+// (owner into own[k], target into the dead base words) and relaxed one hit
+// per lane, over the reset slots of the target only.
 __global__ void __launch_bounds__(kSweepBlock) k_msweep(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
                                                         MultiState ms, PassIO io, uint32_t *stamp, uint32_t stamp_val) {
     __shared__ SweepWave s_wave[kSweepWaves];
@@ -436,7 +436,6 @@ __global__ void __launch_bounds__(kSweepBlock) k_msweep(GraphView g, const unsig
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
     SweepWave &sh = s_wave[w];
     OutStage st{sh.out, 0, kSweepOutStage};
-    const QLane q = qlane(ms);
     uint32_t n_edges = 0;
     const uint32_t cs = ms.cshift;
     load_filter(s_filter, ms.coarse, coarse_words(g.nv, cs));
@@ -474,22 +473,42 @@ __global__ void __launch_bounds__(kSweepBlock) k_msweep(GraphView g, const unsig
             }
         }
         __builtin_amdgcn_wave_barrier();
-        for (uint32_t i0 = 0; i0 < nh; i0 += q.per) {
-            const uint32_t i = i0 + q.sub;
-            const bool ok = i < nh;
-            uint32_t u = 0, v = 0;
-            if (ok) {
+        // one lane per hit: relax exactly the reset slots of its target (usually
+        // a few of the group's QS), then flag the improved ones and queue v once
+        for (uint32_t i0 = 0; i0 < nh; i0 += kWave) {
+            const uint32_t i = i0 + lane;
+            bool app = false;
+            uint32_t v = 0;
+            if (i < nh) {
                 v = hv[i];
-                u = v0 + sh.own[i];
+                const uint32_t u = v0 + sh.own[i];
+                const uint32_t wt = sg_weight(ms.wmode, ms.wseed, u, v);
+                bool any = false;
+                for (uint32_t grp = 0; grp < ms.G; ++grp) {
+                    uint64_t m = ms.inv[(uint64_t)v * ms.G + grp];
+                    uint64_t better = 0;
+                    while (m) {
+                        const uint32_t slot = (uint32_t)__builtin_ctzll(m);
+                        m &= m - 1;
+                        const uint32_t qi = grp * ms.QS + slot;
+                        const uint8_t kind = ms.kind[qi];
+                        if (kind == kQNone) continue;
+                        const uint32_t du = hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + qi]));
+                        if (du == kInf) continue;
+                        const uint32_t cand = du + (kind == kQUnit ? 1u : wt);
+                        unsigned long long *p = &ms.best[(uint64_t)v * ms.KP + qi];
+                        const unsigned long long nw = pack(cand, u);
+                        if (nw >= ld_peek(p)) continue;
+                        if (cand < hi32(atomicMin(p, nw))) better |= 1ull << slot;
+                    }
+                    if (better) {
+                        atomicOr(&ms.act[(uint64_t)v * ms.G + grp], better);
+                        any = true;
+                    }
+                }
+                app = any && atomicExch(&stamp[v], stamp_val) != stamp_val;
             }
-            for (uint32_t grp = 0; grp < ms.G; ++grp) {
-                const uint32_t qi = grp * ms.QS + q.slot;
-                const uint8_t kind = ms.kind[qi];
-                const bool want = ok && kind != kQNone && ((ms.inv[(uint64_t)v * ms.G + grp] >> q.slot) & 1u);
-                const uint32_t du = want ? hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + qi])) : kInf;
-                const bool hit = want && du != kInf && mrelax(ms, u, v, grp, q.slot, du, kind);
-                mark_items(ms, io, st, q, ms.act, grp, v, hit, stamp, stamp_val);
-            }
+            stage_push(app, v, st, io);
         }
         __builtin_amdgcn_wave_barrier();
     }
