@@ -291,16 +291,6 @@ __global__ void __launch_bounds__(kBlock) k_newdeg(GraphView g, const TInfo *tin
 }
 
 // deg[u] = new degree for touched vertices placed in their own range.
-__global__ void k_set_deg(const TInfo *tinfo, const uint32_t *tcount, const uint32_t *newdeg, uint32_t *deg,
-                          const uint8_t *maximal, uint64_t nleaves, uint32_t L, int skip_covered) {
-    const uint32_t n = *tcount;
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t u = tinfo[t].u;
-        if (skip_covered && covered(maximal, nleaves, L, u)) continue;
-        deg[u] = newdeg[u];
-    }
-}
-
 // Upper density threshold at height h of a tree of height H
 // (up_0 = 1.0 at the leaves, up_h = 0.75 at the root, csr_graph.cuh:345-346, 847-851).
 __device__ __host__ __forceinline__ double up_threshold(uint32_t h, uint32_t H) {
@@ -1396,10 +1386,7 @@ public:
             SG_LAUNCHED();
             end_timed(timing());
             hipLaunchKernelGGL(k_lane_place, dim3(grid_for(c.tcount, 256, 4096)), dim3(256), 0, stream_, gview(), bv_, col_,
-                               tinfo_, &ctl_->tcount, maximal_, nleaves_, L_, overflow ? 1 : 0);
-            SG_LAUNCHED();
-            hipLaunchKernelGGL(k_set_deg, dim3(1024), dim3(256), 0, stream_, tinfo_, &ctl_->tcount, newdeg_, deg_, maximal_,
-                               nleaves_, L_, overflow ? 1 : 0);
+                               tinfo_, &ctl_->tcount, newdeg_, deg_, maximal_, nleaves_, L_, overflow ? 1 : 0);
             SG_LAUNCHED();
             if (cache_slots_) {
                 hipLaunchKernelGGL(k_cache_drop, dim3(1024), dim3(256), 0, stream_, tinfo_, &ctl_->tcount, coff_);
@@ -1415,7 +1402,7 @@ public:
     // The batch's per-vertex ranges and marks are kept until the snapshot's
     // recompute has used them; then they are cleared for the next batch.
     void finish_batch() {
-        if (timing()) {
+        if (timing() && ev_used_) {  // timed launches the last round batch did not collect
             sync();
             harvest_events();
         }

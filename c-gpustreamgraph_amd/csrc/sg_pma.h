@@ -364,17 +364,22 @@ __global__ void __launch_bounds__(kBlock) k_chunk_place(GraphView g, BatchView b
     }
 }
 
-// One lane per insert-only vertex with a short insert run: append in place.
+// One lane per touched vertex: an insert-only vertex with a short insert run
+// appends it in place; every touched vertex takes its new degree (vertices
+// in a re-spread window were written by the window pass).
 __global__ void k_lane_place(GraphView g, BatchView bv, uint32_t *col, const TInfo *tinfo, const uint32_t *tcount,
-                             const uint8_t *maximal, uint64_t nleaves, uint32_t L, int skip_covered) {
+                             const uint32_t *newdeg, uint32_t *deg, const uint8_t *maximal, uint64_t nleaves, uint32_t L,
+                             int skip_covered) {
     const uint32_t n = *tcount;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (uint64_t)gridDim.x * blockDim.x) {
         const TInfo ti = tinfo[t];
-        if (chunked(ti) || ti.ni == 0) continue;
         if (skip_covered && covered(maximal, nleaves, L, ti.u)) continue;
-        uint32_t *dst = col + g.off[ti.u] + ti.d0;
-        const uint32_t *src = bv.ins_val + ti.ib;
-        for (uint32_t k = 0; k < ti.ni; ++k) dst[k] = src[k];
+        if (!chunked(ti) && ti.ni) {
+            uint32_t *dst = col + g.off[ti.u] + ti.d0;
+            const uint32_t *src = bv.ins_val + ti.ib;
+            for (uint32_t k = 0; k < ti.ni; ++k) dst[k] = src[k];
+        }
+        deg[ti.u] = newdeg[ti.u];
     }
 }
 
