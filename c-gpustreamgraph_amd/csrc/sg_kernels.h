@@ -177,6 +177,11 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
     return x;
 }
 
+// 64-bit minimum on integers (the generic min template goes through double).
+__host__ __device__ __forceinline__ unsigned long long min_u64(unsigned long long a, unsigned long long b) {
+    return a < b ? a : b;
+}
+
 // Coarse filter over a vertex bitmap: one bit per 2^shift vertices with
 // shift = max(6, ceil(log2 nv) - 16), so the filter has at most 65536 bits
 // (kFilterWords words) and a block can keep a copy in LDS.

@@ -440,7 +440,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_msweep(GraphView g, const unsig
     const uint32_t cs = ms.cshift;
     load_filter(s_filter, ms.coarse, coarse_words(g.nv, cs));
     TilePipe tp;
-    tp.tend = min<unsigned long long>(ntiles, tiles.cap - 1);
+    tp.tend = min_u64(ntiles, tiles.cap - 1);
     tp.nw = (unsigned long long)gridDim.x * kSweepWaves;
     tp.t = (unsigned long long)blockIdx.x * kSweepWaves + w;
     tp.start(g, dpre, tiles);

@@ -502,7 +502,7 @@ __global__ void k_sweep_tiles(uint32_t nv, const unsigned long long *dpre, Sweep
             if ((unsigned long long)m + dpre[m] <= d) lo = m; else hi = m - 1;
         }
         tiles.v[t] = lo;
-        tiles.e[t] = min(d - lo, dpre[lo + 1]);
+        tiles.e[t] = min_u64(d - lo, dpre[lo + 1]);
     }
 }
 
@@ -555,7 +555,7 @@ __device__ __forceinline__ TileHead tile_head(unsigned long long raw) {
     const uint32_t v1 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)raw, 1);
     h.e0 = read_lane_u64(raw, 2);
     const unsigned long long e1 = read_lane_u64(raw, 3);
-    h.ne = (uint32_t)min<unsigned long long>(e1 - h.e0, kSweepTile);
+    h.ne = (uint32_t)min_u64(e1 - h.e0, kSweepTile);
     h.span = min<uint32_t>(v1 - h.v0 + 1, kSweepTile + 1);
     return h;
 }
@@ -715,7 +715,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_edges(GraphView g, const 
     const uint32_t cs = efn.cshift;
     load_filter(s_filter, efn.coarse, coarse_words(g.nv, cs));
     TilePipe tp;
-    tp.tend = min<unsigned long long>(min<unsigned long long>(t_end, ntiles), tiles.cap - 1);
+    tp.tend = min_u64(min_u64(t_end, ntiles), tiles.cap - 1);
     tp.nw = (unsigned long long)gridDim.x * kSweepWaves;
     tp.t = t_begin + (unsigned long long)blockIdx.x * kSweepWaves + w;
     tp.start(g, dpre, tiles);
