@@ -83,7 +83,8 @@ def run_batched(args, row, col, ins, dele, wseed, algos, plain_value):
     """The same queries answered together on one shared graph (one PMA update,
     one boundary sweep and one launch per round for all of them)."""
     nsnap = args.warmup + args.steps
-    e = sgraph.Engine("multi", weight_mode="seeded", weight_seed=wseed, rounds_per_sync=args.rounds_per_sync)
+    e = sgraph.Engine("multi", weight_mode="seeded", weight_seed=wseed, rounds_per_sync=args.rounds_per_sync,
+                      flags=sgraph.FLAG_GRAPH_ROUNDS if args.graph_rounds else 0)
     t = time.perf_counter()
     e.load_csr(row, col)
     e.init_queries([0] * len(algos), algos)
@@ -323,6 +324,7 @@ def main():
     ap.add_argument("--heavy-degree", type=int, default=0)
     ap.add_argument("--round-blocks", type=int, default=0)
     ap.add_argument("--rounds-per-sync", type=int, default=0, help="rounds enqueued per host check (0: engine default)")
+    ap.add_argument("--graph-rounds", action="store_true", help="submit each batch of rounds as one HIP graph")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (shared-graph) measurement")
     ap.add_argument("--batched-only", action="store_true", help="measure only the batched path (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -386,7 +388,7 @@ def main():
     engines = {}
     dev_batches = {}
     for a in algos:
-        flags = 0 if args.no_timing else sgraph.FLAG_KERNEL_TIMING
+        flags = (0 if args.no_timing else sgraph.FLAG_KERNEL_TIMING) | (sgraph.FLAG_GRAPH_ROUNDS if args.graph_rounds else 0)
         e = sgraph.Engine(a, weight_mode="seeded", weight_seed=wseed, flags=flags, wave_budget=args.wave_budget,
                           heavy_degree=args.heavy_degree, round_blocks=args.round_blocks,
                           rounds_per_sync=args.rounds_per_sync)

@@ -27,10 +27,13 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "sgraph.h"
@@ -707,6 +710,7 @@ public:
         sync_ev_ = nullptr;
         if (cstream_) (void)hipStreamDestroy(cstream_);
         cstream_ = nullptr;
+        release_graphs();
         for (auto &ev : events_) { (void)hipEventDestroy(ev.first); (void)hipEventDestroy(ev.second); }
         events_.clear();
         if (pinned_) (void)hipHostFree(pinned_);
@@ -822,6 +826,7 @@ public:
     // --- kernel timing --------------------------------------------------------
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
     std::vector<uint32_t> ev_cls_;  // kernel class of each recorded pair
+    std::vector<uint32_t> ev_n_;    // launches each pair covers
     size_t ev_used_ = 0;
     // per kernel class (sg_kernel_class): launches, summed event time, algorithmic bytes
     uint64_t kt_launches_[SG_KCLASS_COUNT] = {}, kt_bytes_[SG_KCLASS_COUNT] = {};
@@ -1120,8 +1125,8 @@ public:
         return io;
     }
 
-    // HIP events around one launch on the engine's stream.
-    void begin_timed(bool timed, uint32_t cls = SG_KCLASS_RELAX) {
+    // HIP events around one launch (or a graph of nlaunch) on the engine's stream.
+    void begin_timed(bool timed, uint32_t cls = SG_KCLASS_RELAX, uint32_t nlaunch = 1) {
         if (!timed) return;
         if (ev_used_ == events_.size()) {
             hipEvent_t a, b;
@@ -1129,8 +1134,10 @@ public:
             SG_HIP(hipEventCreate(&b));
             events_.push_back({a, b});
             ev_cls_.push_back(0);
+            ev_n_.push_back(1);
         }
         ev_cls_[ev_used_] = cls;
+        ev_n_[ev_used_] = nlaunch;
         SG_HIP(hipEventRecord(events_[ev_used_].first, stream_));
     }
     void end_timed(bool timed) {
@@ -1138,13 +1145,95 @@ public:
     }
     bool timing() const { return (cfg_.flags & SG_FLAG_KERNEL_TIMING) != 0; }
 
+    // ---- round batches as HIP graphs (SG_FLAG_GRAPH_ROUNDS) ----------------------
+    // A batch of k round launches becomes a chain of k kernel nodes. The graph
+    // of a (kernel, k) pair is built and instantiated once; later batches only
+    // rewrite the nodes' arguments (round-dependent queues, counters, stamps)
+    // and launch it: one submission per batch instead of k.
+    struct GraphLaunch {
+        const void *func;
+        dim3 grid, block;
+        std::shared_ptr<void> args;  // tuple of the argument values
+        std::vector<void *> ptrs;    // kernelParams: pointers into args
+    };
+    struct RoundGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t exec = nullptr;
+        std::vector<hipGraphNode_t> nodes;
+        std::vector<std::shared_ptr<void>> keep;  // argument values of the last launch, kept past it
+    };
+    std::vector<GraphLaunch> *batch_ = nullptr;  // set while drive_rounds collects a batch
+    std::map<std::pair<const void *, size_t>, RoundGraph> graphs_;
+
+    template <class... Args, size_t... I>
+    static std::vector<void *> tuple_ptrs(std::tuple<Args...> &t, std::index_sequence<I...>) {
+        return {static_cast<void *>(&std::get<I>(t))...};
+    }
+    // Launch now, or add to the batch being collected.
+    template <class... Args>
+    void round_launch(void (*kernel)(Args...), dim3 grid, dim3 block, Args... args) {
+        if (!batch_) {
+            hipLaunchKernelGGL(kernel, grid, block, 0, stream_, args...);
+            SG_LAUNCHED();
+            return;
+        }
+        auto t = std::make_shared<std::tuple<Args...>>(args...);
+        GraphLaunch l;
+        l.func = reinterpret_cast<const void *>(kernel);
+        l.grid = grid;
+        l.block = block;
+        l.ptrs = tuple_ptrs(*t, std::index_sequence_for<Args...>{});
+        l.args = t;
+        batch_->push_back(std::move(l));
+    }
+    void run_graph_batch(std::vector<GraphLaunch> &b) {
+        if (b.empty()) return;
+        auto key = std::make_pair(b[0].func, b.size());
+        for (auto &l : b)
+            if (l.func != key.first) fail(SG_EDEVICE, "internal: mixed kernels in one round graph");
+        RoundGraph &rg = graphs_[key];
+        auto params = [&](GraphLaunch &l) {
+            hipKernelNodeParams kp{};
+            kp.func = const_cast<void *>(l.func);
+            kp.gridDim = l.grid;
+            kp.blockDim = l.block;
+            kp.sharedMemBytes = 0;
+            kp.kernelParams = l.ptrs.data();
+            kp.extra = nullptr;
+            return kp;
+        };
+        if (!rg.exec) {
+            SG_HIP(hipGraphCreate(&rg.g, 0));
+            rg.nodes.resize(b.size());
+            for (size_t i = 0; i < b.size(); ++i) {
+                const hipKernelNodeParams kp = params(b[i]);
+                SG_HIP(hipGraphAddKernelNode(&rg.nodes[i], rg.g, i ? &rg.nodes[i - 1] : nullptr, i ? 1 : 0, &kp));
+            }
+            SG_HIP(hipGraphInstantiate(&rg.exec, rg.g, nullptr, nullptr, 0));
+        } else {
+            for (size_t i = 0; i < b.size(); ++i) {
+                const hipKernelNodeParams kp = params(b[i]);
+                SG_HIP(hipGraphExecKernelNodeSetParams(rg.exec, rg.nodes[i], &kp));
+            }
+        }
+        SG_HIP(hipGraphLaunch(rg.exec, stream_));
+        rg.keep.clear();  // the previous launch of this graph ended at the wait after its batch
+        for (auto &l : b) rg.keep.push_back(l.args);
+    }
+    void release_graphs() {
+        for (auto &kv : graphs_) {
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+            if (kv.second.g) (void)hipGraphDestroy(kv.second.g);
+        }
+        graphs_.clear();
+    }
+
     template <class VFn, class EFn>
     void launch_pass(const PassIO &io, VFn vfn, EFn efn, bool timed) {
         const uint32_t g1 = io.in_list ? round_blocks() : grid_for(io.range_n, kWave * kWavesPerBlock, round_blocks());
-        begin_timed(timed);
-        hipLaunchKernelGGL((k_round<VFn, EFn>), dim3(g1), dim3(kBlock), 0, stream_, gview(), io, vfn, efn);
-        SG_LAUNCHED();
-        end_timed(timed);
+        begin_timed(timed && !batch_);
+        round_launch(k_round<VFn, EFn>, dim3(g1), dim3(kBlock), gview(), io, vfn, efn);
+        end_timed(timed && !batch_);
     }
 
     // Walk, with efn, the descriptors that a seed pass before round R spilled
@@ -1170,10 +1259,10 @@ public:
             SG_HIP(hipEventElapsedTime(&ms, events_[i].first, events_[i].second));
             const uint32_t c = ev_cls_[i];
             kt_ms_[c] += ms;
-            kt_launches_[c] += 1;
+            kt_launches_[c] += ev_n_[i];
             if (c == SG_KCLASS_RELAX) {
                 st_.relax_ms += ms;
-                st_.relax_launches += 1;
+                st_.relax_launches += ev_n_[i];
             }
         }
         ev_used_ = 0;
@@ -1218,8 +1307,14 @@ public:
         uint32_t rounds = 0;
         bool first = true;
         last_appended_ = 0;
+        const bool graphs = (cfg_.flags & SG_FLAG_GRAPH_ROUNDS) != 0;
+        std::vector<GraphLaunch> batch;
         for (;;) {
             const uint32_t R0 = R_;
+            if (graphs) {
+                batch.clear();
+                batch_ = &batch;
+            }
             for (uint32_t i = 0; i < k; ++i) {
                 PassIO io = round_io(R_);
                 io.log = rlog_;
@@ -1227,6 +1322,12 @@ public:
                 if (timed) io.stats = stats_ + 8;
                 launch(io);
                 ++R_;
+            }
+            if (graphs) {
+                batch_ = nullptr;
+                begin_timed(do_time, SG_KCLASS_RELAX, k);
+                run_graph_batch(batch);
+                end_timed(do_time);
             }
             read_block();
             if (do_time) harvest_events();
@@ -2006,12 +2107,11 @@ public:
 
     template <int kMode>
     void mround(const PassIO &io, bool timed) {
-        begin_timed(timed);
-        hipLaunchKernelGGL((k_mround<kMode>), dim3(round_blocks()), dim3(kBlock), 0, stream_, gview(), mstate(), io,
-                           mspill_[io.round % 3], (const MDesc *)mspill_[(io.round + 2) % 3],
-                           (uint32_t)std::min<size_t>(mspill_cap_, 0xFFFFFFFFu), stamp_);
-        SG_LAUNCHED();
-        end_timed(timed);
+        begin_timed(timed && !batch_);
+        round_launch(k_mround<kMode>, dim3(round_blocks()), dim3(kBlock), gview(), mstate(), io, mspill_[io.round % 3],
+                     (const MDesc *)mspill_[(io.round + 2) % 3], (uint32_t)std::min<size_t>(mspill_cap_, 0xFFFFFFFFu),
+                     stamp_);
+        end_timed(timed && !batch_);
     }
 
     template <int kMode>

@@ -23,6 +23,7 @@ Q_VALUE, Q_BUFFER, Q_PARENT, Q_DEGREE, Q_OFFSETS, Q_SLOTS, Q_SEGMENTS = range(7)
 FLAG_KERNEL_TIMING = 0x1
 FLAG_HOST_EDGES = 0x2
 FLAG_DEFER_CONVERGE = 0x4
+FLAG_GRAPH_ROUNDS = 0x8
 KCLASS = ("relax", "sweep", "merge")
 INF = 0xFFFFFFFF
 DEFAULT_SEED = 27491095  # kRandSeed of the reference (groute/graphs/util.h:68)

@@ -92,6 +92,10 @@ typedef struct sg_config {
  * sg_pr_topk, converges once for all deferred snapshots. The fixed point is
  * the same as converging after each one. Other algorithms ignore the flag. */
 #define SG_FLAG_DEFER_CONVERGE 0x4u
+/* Submit each batch of convergence rounds as one HIP graph (a chain of
+ * kernel nodes built once per batch length and re-parameterised per batch)
+ * instead of one launch per round. Results are identical. */
+#define SG_FLAG_GRAPH_ROUNDS 0x8u
 
 typedef struct sg_stats {
     uint32_t nv;                  /* vertices                                          */

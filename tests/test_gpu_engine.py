@@ -133,6 +133,16 @@ def test_rmat_stream(algo):
     run_stream(algo, row, col, list(zip(ins, dele)), wmode="seeded" if algo == "sssp" else "hash128", seed=99)
 
 
+@pytest.mark.parametrize("algo", ALGOS)
+def test_rmat_stream_graph_rounds(algo):
+    # the same stream with every batch of rounds submitted as a HIP graph: the
+    # graphs are replayed many times, with eager kernels and host waits between
+    nv, row, col = sgraph.gen_rmat(12, 8, seed=11)
+    ins, dele = sgraph.gen_updates(row, col, 12, 4, 600, 600, seed=12)
+    run_stream(algo, row, col, list(zip(ins, dele)), wmode="seeded" if algo == "sssp" else "hash128", seed=99,
+               flags=sgraph.FLAG_GRAPH_ROUNDS)
+
+
 @pytest.mark.parametrize("algo", ["sssp", "bfs", "cc"])
 def test_rmat_stream_larger(algo):
     nv, row, col = sgraph.gen_rmat(15, 16, seed=21)

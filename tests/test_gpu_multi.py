@@ -29,11 +29,11 @@ def check_all(eng, row, col, sources, kinds, wmode, seed, tag):
         assert len(bad) == 0, f"{tag} q{qi} ({k} from {s}): {len(bad)} mismatches, first {bad[:5]} got {got[bad[:5]]} exp {exp[bad[:5]]}"
 
 
-def run_multi(scale, ef, nsnap, nupd, sources, kinds, wmode="seeded", seed=9, gseed=41):
+def run_multi(scale, ef, nsnap, nupd, sources, kinds, wmode="seeded", seed=9, gseed=41, flags=0):
     nv, row, col = sgraph.gen_rmat(scale, ef, seed=gseed)
     ins, dele = sgraph.gen_updates(row, col, scale, nsnap, nupd, nupd, seed=gseed + 1)
-    say(f"[gpu] multi nv={nv} ne={len(col)} nq={len(sources)} snaps={nsnap} w={wmode}")
-    e = sgraph.Engine("multi", weight_mode=wmode, weight_seed=seed)
+    say(f"[gpu] multi nv={nv} ne={len(col)} nq={len(sources)} snaps={nsnap} w={wmode} flags={flags}")
+    e = sgraph.Engine("multi", weight_mode=wmode, weight_seed=seed, flags=flags)
     try:
         e.load_csr(row, col)
         e.init_queries(sources, kinds)
@@ -56,6 +56,13 @@ def test_query_counts(nq):
     sources[0] = 0
     kinds = ["sssp" if i % 3 else "bfs" for i in range(nq)]
     run_multi(10, 8, 3, 200, sources, kinds)
+
+
+def test_graph_rounds_batch():
+    rng = np.random.default_rng(9)
+    sources = rng.integers(0, 1 << 11, 40).astype(np.uint32)
+    kinds = ["sssp" if i % 2 else "bfs" for i in range(40)]
+    run_multi(11, 8, 4, 400, sources, kinds, flags=sgraph.FLAG_GRAPH_ROUNDS)
 
 
 def test_mixed_batch_hash128_with_duplicate_sources():
