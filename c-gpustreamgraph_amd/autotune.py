@@ -5,9 +5,9 @@ stream count are command-line constants, app_skeleton.h:42-100). Given a
 graph and a few update batches of the workload, every candidate of the search
 space runs the same snapshots on a fresh engine; the configuration with the
 lowest total snapshot time wins. Candidates only change how work is
-scheduled (wave budget, heavy-list threshold, rounds per host check, SEGMENT
-count, cache level), never what is computed, so every candidate gives the
-same results.
+scheduled (wave budget, heavy-list threshold, rounds per host check, round
+batches as HIP graphs, SEGMENT count, cache level), never what is computed, so
+every candidate gives the same results.
 """
 import itertools
 import time
@@ -20,6 +20,7 @@ RESIDENT_SPACE = {
     "heavy_degree": [2048, 8192],
     "rounds_per_sync": [0, 2, 4, 8],
     "round_blocks": [1024, 2048],
+    "graph_rounds": [False, True],  # batches of rounds as HIP graphs (SG_FLAG_GRAPH_ROUNDS)
 }
 # knobs of the out-of-memory path
 HOST_SPACE = {
@@ -28,8 +29,16 @@ HOST_SPACE = {
 }
 
 
+def engine_config(cfg):
+    """Engine keyword arguments of a search-space point (graph_rounds becomes a flag)."""
+    cfg = dict(cfg)
+    if cfg.pop("graph_rounds", False):
+        cfg["flags"] = cfg.get("flags", 0) | sgraph.FLAG_GRAPH_ROUNDS
+    return cfg
+
+
 def _run(algo, row, col, ins, dele, source, cfg):
-    e = sgraph.Engine(algo, **cfg)
+    e = sgraph.Engine(algo, **engine_config(cfg))
     try:
         e.load_csr(row, col)
         if algo == "multi":
@@ -49,7 +58,8 @@ def _run(algo, row, col, ins, dele, source, cfg):
 def autotune(algo, row, col, ins, dele, source=0, space=None, repeats=1, **base):
     """Time every point of `space` (dict of knob -> values; by default the
     resident knobs, plus the out-of-memory knobs when base asks for host
-    edges) on the given snapshots. Returns (best config dict, [(config, seconds)])."""
+    edges) on the given snapshots. Returns (engine keyword arguments of the
+    fastest point, [(point, seconds)])."""
     if space is None:
         space = dict(RESIDENT_SPACE)
         if base.get("flags", 0) & sgraph.FLAG_HOST_EDGES or base.get("edge_budget_gb", 0) > 0:
@@ -62,7 +72,7 @@ def autotune(algo, row, col, ins, dele, source=0, space=None, repeats=1, **base)
         t = min(_run(algo, row, col, ins, dele, source, cfg) for _ in range(max(1, repeats)))
         table.append((cfg, t))
     best = min(table, key=lambda x: x[1])[0]
-    return best, table
+    return engine_config(best), table
 
 
 def main():

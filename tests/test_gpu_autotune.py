@@ -14,10 +14,11 @@ pytestmark = pytest.mark.gpu
 def test_autotune_picks_a_candidate_with_correct_results():
     nv, row, col = sgraph.gen_rmat(11, 8, seed=21)
     ins, dele = sgraph.gen_updates(row, col, 11, 2, 300, 300, seed=22)
-    space = {"wave_budget": [512, 2048], "rounds_per_sync": [2, 8]}
+    space = {"wave_budget": [512, 2048], "rounds_per_sync": [2, 8], "graph_rounds": [False, True]}
     best, table = autotune.autotune("sssp", row, col, ins, dele, space=space, weight_mode="seeded", weight_seed=2)
-    assert len(table) == 4
+    assert len(table) == 8
     assert best["wave_budget"] in (512, 2048) and best["rounds_per_sync"] in (2, 8)
+    assert "graph_rounds" not in best and best.get("flags", 0) in (0, sgraph.FLAG_GRAPH_ROUNDS)
     e = sgraph.Engine("sssp", **best)
     e.load_csr(row, col)
     e.init(0)
