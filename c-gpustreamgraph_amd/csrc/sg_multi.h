@@ -123,6 +123,70 @@ __device__ __forceinline__ void reset_children(const MultiState &ms, const PassI
     stage_push(app, v, st, io);
 }
 
+// Relax walk for a sparse slot mask (at most kSparseSlots queries active at
+// u): one lane per edge, each lane trying the few active slots in turn,
+// instead of QS lanes per edge of which most would idle.
+constexpr int kSparseSlots = 4;
+constexpr int kSparseUnroll = 4;  // edges per lane in flight
+__device__ void mwalk_sparse(const GraphView &g, const MultiState &ms, const PassIO &io, OutStage &st, uint32_t *stamp,
+                             uint32_t stamp_val, uint32_t u, uint32_t grp, uint64_t mask, uint64_t start, uint32_t len,
+                             unsigned long long &edges) {
+    uint32_t sl[kSparseSlots], dus[kSparseSlots];
+    bool unit[kSparseSlots];
+    int na = 0;
+    uint64_t m = mask;
+#pragma unroll
+    for (int a = 0; a < kSparseSlots; ++a) {  // the active slots and u's distances (wave-uniform)
+        sl[a] = 0;
+        dus[a] = kInf;
+        unit[a] = false;
+        if (m) {
+            const uint32_t s = (uint32_t)__builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t qi = grp * ms.QS + s;
+            const uint8_t kind = ms.kind[qi];
+            const uint32_t du = kind == kQNone ? kInf : hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + qi]));
+            sl[a] = s;
+            dus[a] = du;
+            unit[a] = kind == kQUnit;
+            if (du != kInf) na = a + 1;
+        }
+    }
+    if (na == 0) return;
+    edges += len;
+    const uint32_t lane = lane_id();
+    for (uint32_t e0 = 0; e0 < len; e0 += kWave * kSparseUnroll) {
+        uint32_t vv[kSparseUnroll];
+#pragma unroll
+        for (int j = 0; j < kSparseUnroll; ++j) {
+            const uint32_t e = e0 + j * kWave + lane;
+            vv[j] = e < len ? edge_at(g, start + e) : kGap;
+        }
+#pragma unroll
+        for (int j = 0; j < kSparseUnroll; ++j) {
+            const uint32_t v = vv[j];
+            bool app = false;
+            if (v != kGap) {
+                const uint32_t w = sg_weight(ms.wmode, ms.wseed, u, v);
+                uint64_t better = 0;
+#pragma unroll
+                for (int a = 0; a < kSparseSlots; ++a) {
+                    if (a >= na || dus[a] == kInf) continue;
+                    const uint32_t cand = dus[a] + (unit[a] ? 1u : w);
+                    unsigned long long *p = &ms.best[(uint64_t)v * ms.KP + grp * ms.QS + sl[a]];
+                    const unsigned long long nw = pack(cand, u);
+                    if (nw < ld_peek(p) && cand < hi32(atomicMin(p, nw))) better |= 1ull << sl[a];
+                }
+                if (better) {
+                    atomicOr(&ms.act[(uint64_t)v * ms.G + grp], better);
+                    app = atomicExch(&stamp[v], stamp_val) != stamp_val;
+                }
+            }
+            stage_push(app, v, st, io);
+        }
+    }
+}
+
 // Walk len edges of u from slot position start for the slots in mask.
 // kMode 0: relax (activations go to act); kMode 1: reset tree children whose
 // parent is u (resets go to inv / ninv and the vertex bitmaps).
@@ -130,6 +194,10 @@ template <int kMode>
 __device__ void mwalk(const GraphView &g, const MultiState &ms, const PassIO &io, OutStage &st, uint32_t *stamp,
                       uint32_t stamp_val, uint32_t u, uint32_t grp, uint64_t mask, uint64_t start, uint32_t len,
                       unsigned long long &edges) {
+    if (kMode == 0 && ms.QS > kSparseSlots && __popcll(mask) <= kSparseSlots) {
+        mwalk_sparse(g, ms, io, st, stamp, stamp_val, u, grp, mask, start, len, edges);
+        return;
+    }
     const QLane q = qlane(ms);
     const bool on = (mask >> q.slot) & 1u;
     const uint32_t qi = grp * ms.QS + q.slot;
