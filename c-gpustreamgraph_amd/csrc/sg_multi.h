@@ -123,36 +123,42 @@ __device__ __forceinline__ void reset_children(const MultiState &ms, const PassI
     stage_push(app, v, st, io);
 }
 
-// Relax walk for a sparse slot mask (at most kSparseSlots queries active at
-// u): one lane per edge, each lane trying the few active slots in turn,
-// instead of QS lanes per edge of which most would idle.
+// Walk for a sparse slot mask (at most kSparseSlots queries active at u):
+// one lane per edge, each lane trying the few active slots in turn, instead
+// of QS lanes per edge of which most would idle. kMode as mwalk.
 constexpr int kSparseSlots = 4;
 constexpr int kSparseUnroll = 4;  // edges per lane in flight
+template <int kMode>
 __device__ void mwalk_sparse(const GraphView &g, const MultiState &ms, const PassIO &io, OutStage &st, uint32_t *stamp,
                              uint32_t stamp_val, uint32_t u, uint32_t grp, uint64_t mask, uint64_t start, uint32_t len,
                              unsigned long long &edges) {
     uint32_t sl[kSparseSlots], dus[kSparseSlots];
-    bool unit[kSparseSlots];
-    int na = 0;
+    bool unit[kSparseSlots], live[kSparseSlots];
+    bool any = false;
     uint64_t m = mask;
 #pragma unroll
     for (int a = 0; a < kSparseSlots; ++a) {  // the active slots and u's distances (wave-uniform)
         sl[a] = 0;
         dus[a] = kInf;
         unit[a] = false;
+        live[a] = false;
         if (m) {
             const uint32_t s = (uint32_t)__builtin_ctzll(m);
             m &= m - 1;
             const uint32_t qi = grp * ms.QS + s;
             const uint8_t kind = ms.kind[qi];
-            const uint32_t du = kind == kQNone ? kInf : hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + qi]));
             sl[a] = s;
-            dus[a] = du;
             unit[a] = kind == kQUnit;
-            if (du != kInf) na = a + 1;
+            if (kMode == 0) {
+                dus[a] = kind == kQNone ? kInf : hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + qi]));
+                live[a] = dus[a] != kInf;
+            } else {
+                live[a] = kind != kQNone;
+            }
+            any = any || live[a];
         }
     }
-    if (na == 0) return;
+    if (!any) return;
     edges += len;
     const uint32_t lane = lane_id();
     for (uint32_t e0 = 0; e0 < len; e0 += kWave * kSparseUnroll) {
@@ -167,19 +173,40 @@ __device__ void mwalk_sparse(const GraphView &g, const MultiState &ms, const Pas
             const uint32_t v = vv[j];
             bool app = false;
             if (v != kGap) {
-                const uint32_t w = sg_weight(ms.wmode, ms.wseed, u, v);
-                uint64_t better = 0;
+                if (kMode == 0) {
+                    const uint32_t w = sg_weight(ms.wmode, ms.wseed, u, v);
+                    uint64_t better = 0;
 #pragma unroll
-                for (int a = 0; a < kSparseSlots; ++a) {
-                    if (a >= na || dus[a] == kInf) continue;
-                    const uint32_t cand = dus[a] + (unit[a] ? 1u : w);
-                    unsigned long long *p = &ms.best[(uint64_t)v * ms.KP + grp * ms.QS + sl[a]];
-                    const unsigned long long nw = pack(cand, u);
-                    if (nw < ld_peek(p) && cand < hi32(atomicMin(p, nw))) better |= 1ull << sl[a];
-                }
-                if (better) {
-                    atomicOr(&ms.act[(uint64_t)v * ms.G + grp], better);
-                    app = atomicExch(&stamp[v], stamp_val) != stamp_val;
+                    for (int a = 0; a < kSparseSlots; ++a) {
+                        if (!live[a]) continue;
+                        const uint32_t cand = dus[a] + (unit[a] ? 1u : w);
+                        unsigned long long *p = &ms.best[(uint64_t)v * ms.KP + grp * ms.QS + sl[a]];
+                        const unsigned long long nw = pack(cand, u);
+                        if (nw < ld_peek(p) && cand < hi32(atomicMin(p, nw))) better |= 1ull << sl[a];
+                    }
+                    if (better) {
+                        atomicOr(&ms.act[(uint64_t)v * ms.G + grp], better);
+                        app = atomicExch(&stamp[v], stamp_val) != stamp_val;
+                    }
+                } else {
+                    // slots of v whose tree parent is u: claim them once, reset them
+                    uint64_t child = 0;
+#pragma unroll
+                    for (int a = 0; a < kSparseSlots; ++a) {
+                        if (!live[a]) continue;
+                        const unsigned long long b = ld_relaxed(&ms.best[(uint64_t)v * ms.KP + grp * ms.QS + sl[a]]);
+                        if (lo32(b) == u && hi32(b) != kInf) child |= 1ull << sl[a];
+                    }
+                    uint64_t fresh = 0;
+                    if (child) fresh = child & ~atomicOr(&ms.inv[(uint64_t)v * ms.G + grp], child);
+                    if (fresh) {
+                        atomicOr(&ms.ninv[(uint64_t)v * ms.G + grp], fresh);
+                        atomicOr(&ms.invbit[v >> 5], 1u << (v & 31));
+                        coarse_mark(ms.coarse, ms.cshift, v);
+                        app = atomicExch(&stamp[v], stamp_val) != stamp_val;
+                        for (uint64_t f = fresh; f; f &= f - 1)
+                            ms.best[(uint64_t)v * ms.KP + grp * ms.QS + (uint32_t)__builtin_ctzll(f)] = ~0ull;
+                    }
                 }
             }
             stage_push(app, v, st, io);
@@ -194,8 +221,8 @@ template <int kMode>
 __device__ void mwalk(const GraphView &g, const MultiState &ms, const PassIO &io, OutStage &st, uint32_t *stamp,
                       uint32_t stamp_val, uint32_t u, uint32_t grp, uint64_t mask, uint64_t start, uint32_t len,
                       unsigned long long &edges) {
-    if (kMode == 0 && ms.QS > kSparseSlots && __popcll(mask) <= kSparseSlots) {
-        mwalk_sparse(g, ms, io, st, stamp, stamp_val, u, grp, mask, start, len, edges);
+    if (ms.QS > kSparseSlots && __popcll(mask) <= kSparseSlots) {
+        mwalk_sparse<kMode>(g, ms, io, st, stamp, stamp_val, u, grp, mask, start, len, edges);
         return;
     }
     const QLane q = qlane(ms);
