@@ -319,7 +319,9 @@ def main():
     ap.add_argument("--algos", default=None, help="default sssp,bfs (incremental) or pr,cc (oom)")
     ap.add_argument("--seed", type=int, default=sgraph.DEFAULT_SEED)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-timing", action="store_true", help="do not time the relax kernel with HIP events")
+    ap.add_argument("--no-timing", action="store_true", help="no HIP-event kernel timing (no roofline)")
+    ap.add_argument("--all-kernel-timing", action="store_true",
+                    help="time every kernel class, not only the sweep (breakdown; the events slow the snapshot)")
     ap.add_argument("--wave-budget", type=int, default=0)
     ap.add_argument("--heavy-degree", type=int, default=0)
     ap.add_argument("--round-blocks", type=int, default=0)
@@ -388,7 +390,8 @@ def main():
     engines = {}
     dev_batches = {}
     for a in algos:
-        flags = (0 if args.no_timing else sgraph.FLAG_KERNEL_TIMING) | (sgraph.FLAG_GRAPH_ROUNDS if args.graph_rounds else 0)
+        timing = 0 if args.no_timing else sgraph.FLAG_KERNEL_TIMING | (0 if args.all_kernel_timing else sgraph.FLAG_TIME_SWEEP)
+        flags = timing | (sgraph.FLAG_GRAPH_ROUNDS if args.graph_rounds else 0)
         e = sgraph.Engine(a, weight_mode="seeded", weight_seed=wseed, flags=flags, wave_budget=args.wave_budget,
                           heavy_degree=args.heavy_degree, round_blocks=args.round_blocks,
                           rounds_per_sync=args.rounds_per_sync)
@@ -443,8 +446,9 @@ def main():
                      "achieved_gbs": round(b / (ms * 1e-3) / 1e9, 2) if ms and b else None}
                for cls, (n, ms, b) in kt.items()}
     # the roofline is quoted for the class with the most kernel time that has a byte model
-    dom = max((c for c in kt if kt[c][2] > 0), key=lambda c: kt[c][1])
-    dn, dms, db = kt[dom]
+    timed = [c for c in kt if kt[c][2] > 0 and kt[c][1] > 0]
+    dom = max(timed, key=lambda c: kt[c][1]) if timed else None  # None under --no-timing
+    dn, dms, db = kt[dom] if dom else (0, 0.0, 0)
     achieved = db / (dms * 1e-3) / 1e9 if dms > 0 else None
     traffic, traffic_src = (args.traffic, "--traffic") if args.traffic else (
         measured_traffic("sg::k_sweep_edges") if dom == "sweep" and args.scale == 22 and args.edge_factor == 16

@@ -1143,7 +1143,10 @@ public:
     void end_timed(bool timed) {
         if (timed) SG_HIP(hipEventRecord(events_[ev_used_++].second, stream_));
     }
-    bool timing() const { return (cfg_.flags & SG_FLAG_KERNEL_TIMING) != 0; }
+    // Kernel class cls is timed: SG_FLAG_KERNEL_TIMING, narrowed to the sweep by SG_FLAG_TIME_SWEEP.
+    bool timing(uint32_t cls) const {
+        return (cfg_.flags & SG_FLAG_KERNEL_TIMING) && (!(cfg_.flags & SG_FLAG_TIME_SWEEP) || cls == SG_KCLASS_SWEEP);
+    }
 
     // ---- round batches as HIP graphs (SG_FLAG_GRAPH_ROUNDS) ----------------------
     // A batch of k round launches becomes a chain of k kernel nodes. The graph
@@ -1282,7 +1285,7 @@ public:
     // returns {vfn, efn}. Timed runs count into stats [8..11]. Returns the rounds run.
     template <class MakeFns>
     uint32_t run_rounds(MakeFns make_fns, bool timed, int phase = -1) {
-        const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
+        const bool do_time = timed && timing(SG_KCLASS_RELAX);
         return drive_rounds([&](const PassIO &io) {
             auto fns = make_fns(io.round + 1);
             launch_pass(io, fns.first, fns.second, do_time);
@@ -1302,7 +1305,7 @@ public:
         if (adaptive) k0 = last_rounds_[phase] + 1;
         uint32_t k = std::max<uint32_t>(1, std::min<uint32_t>(k0, kRoundLog));
         const uint32_t maxr = cfg_.max_rounds ? cfg_.max_rounds : 1000000u;
-        const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
+        const bool do_time = timed && timing(SG_KCLASS_RELAX);
         if (timed) fill<unsigned long long>(stats_ + 8, 8, 0ull);
         uint32_t rounds = 0;
         bool first = true;
@@ -1369,7 +1372,7 @@ public:
         if (timed) {
             timed_edges_ += s[8] + s[10];
             timed_verts_ += s[9];
-            if (cfg_.flags & SG_FLAG_KERNEL_TIMING) {
+            if (timing(SG_KCLASS_RELAX)) {
                 st_.relax_bytes += 12ull * (s[8] + s[10]) + 32ull * s[9];
                 kt_bytes_[SG_KCLASS_RELAX] += 12ull * (s[8] + s[10]) + 32ull * s[9];
             }
@@ -1455,11 +1458,11 @@ public:
                 sync();
                 ensure(tmp_, tmp_cap_, std::max<uint64_t>(h_ctl_->seqcur, 1));
             }
-            begin_timed(timing(), SG_KCLASS_MERGE);
+            begin_timed(timing(SG_KCLASS_MERGE), SG_KCLASS_MERGE);
             hipLaunchKernelGGL(k_chunk_filter, dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_,
                                (const MChunk *)chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, ccount_, tmp_, col_);
             SG_LAUNCHED();
-            end_timed(timing());
+            end_timed(timing(SG_KCLASS_MERGE));
         }
         b = cub_cap_;
         SG_HIP(hipcub::DeviceScan::ExclusiveSum(cub_tmp_, b, ccount_, cpos_, (int)(chunk_bound + 1), stream_));
@@ -1480,12 +1483,12 @@ public:
         const uint64_t ne_new = ne_ + (uint64_t)(long long)c.ne_delta;
         if (nins_ + ndel_) {
             if (overflow) rebalance(ne_new);  // moves the touched vertices of its windows
-            begin_timed(timing(), SG_KCLASS_MERGE);
+            begin_timed(timing(SG_KCLASS_MERGE), SG_KCLASS_MERGE);
             hipLaunchKernelGGL(k_chunk_place, dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_, col_, tinfo_, chunks_,
                                &ctl_->nchunks, (uint32_t)chunk_bound_, cpos_, tmp_, newdeg_, maximal_, nleaves_, L_,
                                overflow ? 1 : 0);
             SG_LAUNCHED();
-            end_timed(timing());
+            end_timed(timing(SG_KCLASS_MERGE));
             hipLaunchKernelGGL(k_lane_place, dim3(grid_for(c.tcount, 256, 4096)), dim3(256), 0, stream_, gview(), bv_, col_,
                                tinfo_, &ctl_->tcount, newdeg_, deg_, maximal_, nleaves_, L_, overflow ? 1 : 0);
             SG_LAUNCHED();
@@ -1503,7 +1506,7 @@ public:
     // The batch's per-vertex ranges and marks are kept until the snapshot's
     // recompute has used them; then they are cleared for the next batch.
     void finish_batch() {
-        if (timing() && ev_used_) {  // timed launches the last round batch did not collect
+        if (ev_used_) {  // timed launches the last round batch did not collect
             sync();
             harvest_events();
         }
@@ -1663,13 +1666,13 @@ public:
         const uint64_t ntiles = ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile;
         if (!host_edges_) {
             const uint32_t grid = sweep_grid(k_sweep_edges<Op, EFn>, ntiles);
-            begin_timed(timing(), SG_KCLASS_SWEEP);
+            begin_timed(timing(SG_KCLASS_SWEEP), SG_KCLASS_SWEEP);
             hipLaunchKernelGGL((k_sweep_edges<Op, EFn>), dim3(grid), dim3(kSweepBlock), 0, stream_, gview(), dpre_, tl,
                                (const uint32_t *)col_, (uint64_t)0, 0u, nv_, (uint64_t)0, ntiles, op, buffer, ef, io);
             SG_LAUNCHED();
-            end_timed(timing());
+            end_timed(timing(SG_KCLASS_SWEEP));
             // algorithmic bytes: every live target id, and off + dpre of every source
-            if (timing()) kt_bytes_[SG_KCLASS_SWEEP] += 4ull * ne_ + 16ull * nv_;
+            if (timing(SG_KCLASS_SWEEP)) kt_bytes_[SG_KCLASS_SWEEP] += 4ull * ne_ + 16ull * nv_;
             return;
         }
         streamed_sweep([&](const uint32_t *colp, uint64_t slo, uint32_t va, uint32_t vb, uint64_t tlo, uint64_t thi) {
@@ -2116,7 +2119,7 @@ public:
 
     template <int kMode>
     uint32_t mrounds(bool timed, int phase = -1) {
-        const bool do_time = timed && (cfg_.flags & SG_FLAG_KERNEL_TIMING);
+        const bool do_time = timed && timing(SG_KCLASS_RELAX);
         return drive_rounds([&](const PassIO &io) { mround<kMode>(io, do_time); }, timed, phase);
     }
 

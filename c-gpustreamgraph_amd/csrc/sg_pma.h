@@ -288,18 +288,22 @@ __global__ void __launch_bounds__(kBlock) k_chunk_filter(GraphView g, BatchView 
         const uint32_t *old = g.col + m.base;
         const uint32_t *ins = bv.ins_val + ti.ib;
         const uint32_t ns = stage_deletes(bv, ti, s_del[w], s_mask[w]);
-        // the whole chunk is loaded before the first probe: one memory latency per chunk
+        // the whole chunk is loaded before the first probe: one memory latency
+        // per chunk; only the nj wave rows the chunk covers are loaded and probed
+        const uint32_t nj = (m.hi - m.lo + kWave - 1) / kWave;
         uint32_t x[kMergePerLane];
 #pragma unroll
         for (int j = 0; j < kMergePerLane; ++j) {
             const uint32_t i = m.lo + j * kWave + lane;
-            x[j] = i < m.hi ? (i < ti.d0 ? old[i] : ins[i - ti.d0]) : 0;
+            x[j] = 0;
+            if (j < (int)nj && i < m.hi) x[j] = i < ti.d0 ? old[i] : ins[i - ti.d0];
         }
         uint32_t cnt = 0;
         const bool inpl = m.ibase == kInPlace;
         uint32_t *dst = inpl ? col + m.base : out + m.ibase;
 #pragma unroll
         for (int j = 0; j < kMergePerLane; ++j) {  // survivors stay in sequence order
+            if (j >= (int)nj) break;
             const uint32_t i = m.lo + j * kWave + lane;
             const bool keep = i < m.hi && survives(x[j], bv, ti, s_del[w], s_mask[w], ns);
             const uint64_t b = __ballot(keep);
@@ -346,16 +350,20 @@ __global__ void __launch_bounds__(kBlock) k_chunk_place(GraphView g, BatchView b
         const uint64_t p0 = (hd ? 0 : ti.d0) + (cpos[c] - cpos[ti.cbase]);
         const uint64_t len = cpos[c + 1] - cpos[c];
         const uint32_t *srcp = hd ? merged + m.ibase : bv.ins_val + ti.ib + m.lo;
-        uint32_t x[kMergePerLane];  // len <= kMergeChunk: one load per lane and slot, all in flight
+        // len <= kMergeChunk: one load per lane and covered wave row, all in flight
+        const uint32_t nj = (uint32_t)((len + kWave - 1) / kWave);
+        const uint32_t *sp = srcp + lane;
+        uint32_t *dp = lst + p0 + lane;
+        uint32_t x[kMergePerLane];
 #pragma unroll
         for (int j = 0; j < kMergePerLane; ++j) {
-            const uint64_t k = (uint64_t)j * kWave + lane;
-            x[j] = k < len ? srcp[k] : 0;
+            x[j] = 0;
+            if (j < (int)nj && (uint32_t)j * kWave + lane < len) x[j] = sp[j * kWave];
         }
 #pragma unroll
         for (int j = 0; j < kMergePerLane; ++j) {
-            const uint64_t k = (uint64_t)j * kWave + lane;
-            if (k < len) lst[p0 + k] = x[j];
+            if (j >= (int)nj) break;
+            if ((uint32_t)j * kWave + lane < len) dp[j * kWave] = x[j];
         }
         if (hd) {
             const uint32_t nd = newdeg[ti.u];

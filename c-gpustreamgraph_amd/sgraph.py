@@ -24,6 +24,7 @@ FLAG_KERNEL_TIMING = 0x1
 FLAG_HOST_EDGES = 0x2
 FLAG_DEFER_CONVERGE = 0x4
 FLAG_GRAPH_ROUNDS = 0x8
+FLAG_TIME_SWEEP = 0x10
 KCLASS = ("relax", "sweep", "merge")
 INF = 0xFFFFFFFF
 DEFAULT_SEED = 27491095  # kRandSeed of the reference (groute/graphs/util.h:68)

@@ -81,7 +81,8 @@ typedef struct sg_config {
     uint32_t reserved[1];
 } sg_config;
 
-/* Time the dominant relax kernel with HIP events recorded on its stream. */
+/* Time the kernel classes (sg_kernel_class) with HIP events recorded on the
+ * engine's stream around their launches. */
 #define SG_FLAG_KERNEL_TIMING 0x1u
 /* Keep the edge array in pinned host memory whatever its size (the
  * out-of-memory path of -cache / -SEGMENT, framework.cuh:310-384). */
@@ -96,6 +97,11 @@ typedef struct sg_config {
  * kernel nodes built once per batch length and re-parameterised per batch)
  * instead of one launch per round. Results are identical. */
 #define SG_FLAG_GRAPH_ROUNDS 0x8u
+/* With SG_FLAG_KERNEL_TIMING: time only SG_KCLASS_SWEEP (one event pair per
+ * snapshot); the other classes report no launches. Events between round
+ * launches and their harvest after each round batch cost the timed snapshot
+ * several percent, which this keeps out of the measurement. */
+#define SG_FLAG_TIME_SWEEP 0x10u
 
 typedef struct sg_stats {
     uint32_t nv;                  /* vertices                                          */

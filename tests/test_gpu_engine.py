@@ -143,6 +143,31 @@ def test_rmat_stream_graph_rounds(algo):
                flags=sgraph.FLAG_GRAPH_ROUNDS)
 
 
+@pytest.mark.parametrize("timing", ["all", "sweep"])
+def test_kernel_timing_classes(timing):
+    # event timing changes nothing computed; SG_FLAG_TIME_SWEEP times the sweep alone
+    nv, row, col = sgraph.gen_rmat(11, 8, seed=21)
+    ins, dele = sgraph.gen_updates(row, col, 11, 3, 400, 400, seed=22)
+    flags = sgraph.FLAG_KERNEL_TIMING | (sgraph.FLAG_TIME_SWEEP if timing == "sweep" else 0)
+    e = sgraph.Engine("sssp", weight_mode="seeded", weight_seed=5, flags=flags)
+    try:
+        e.load_csr(row, col)
+        e.init(0)
+        r, c = row, col
+        for t in range(3):
+            e.apply(ins[t], dele[t])
+            r, c = O.apply_updates(r, c, ins[t], dele[t])
+            check_values("sssp", e, r, c, 0, "seeded", 5, f"timing={timing} G{t + 1}")
+        kt = e.kernel_times()
+        assert 1 <= kt["sweep"][0] <= 3 and kt["sweep"][1] > 0 and kt["sweep"][2] > 0  # one per snapshot that invalidates
+        if timing == "sweep":
+            assert kt["relax"][0] == 0 and kt["merge"][0] == 0
+        else:
+            assert kt["relax"][0] > 0 and kt["merge"][0] > 0
+    finally:
+        e.close()
+
+
 @pytest.mark.parametrize("algo", ["sssp", "bfs", "cc"])
 def test_rmat_stream_larger(algo):
     nv, row, col = sgraph.gen_rmat(15, 16, seed=21)
