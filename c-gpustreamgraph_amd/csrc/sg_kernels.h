@@ -313,17 +313,29 @@ __device__ __forceinline__ void walk_staged(const GraphView &g, const PassIO &io
 // longer lists, and lists past the budget, are cut into kChunk-edge
 // descriptors for the heavy tier, so no wave carries more than
 // budget + heavy_deg edges per batch.
+// Power of two in [1, kWave] at least ceil(n / nwaves).
+__device__ __forceinline__ uint32_t light_width(uint32_t n, uint64_t nwaves) {
+    const uint64_t q = (n + nwaves - 1) / nwaves;
+    uint32_t per = 1;
+    while (per < (uint32_t)kWave && per < q) per <<= 1;
+    return per;
+}
+
 template <class VFn, class EFn>
 __device__ __forceinline__ void light_body(const GraphView &g, const PassIO &io, const VFn &vfn, const EFn &efn,
                                            uint32_t n, uint64_t gwave, uint64_t nwaves, LightShared &sh, OutStage &st,
                                            WaveStats &ws) {
     const uint32_t w = threadIdx.x / kWave, lane = lane_id();
-    for (uint64_t base = gwave * kWave; base < n; base += nwaves * kWave) {
+    // frontier vertices per wave batch: 64 when the frontier fills every wave,
+    // fewer (a power of two, down to 1) when it is small, so that its lists
+    // spread over all waves instead of a few waves walking 64 lists each
+    const uint32_t per = light_width(n, nwaves);
+    for (uint64_t base = gwave * per; base < n; base += nwaves * per) {
         const uint64_t idx = base + lane;
         uint32_t u = 0, d = 0, push = 0;
         uint64_t start = 0;
         bool self = false, sched = false;
-        if (idx < n) {
+        if (lane < per && idx < n) {
             u = io.in_list ? io.in_list[idx] : io.range_lo + (uint32_t)idx;
             const uint32_t du = g.deg[u];
             sched = vfn(u, du, push, self);
