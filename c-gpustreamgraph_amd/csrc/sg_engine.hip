@@ -1430,15 +1430,18 @@ public:
                                d_del, ndel, ins_k_, ins_v_, del_k_, ctl_);
             SG_LAUNCHED();
         }
+        // sort only the bits ids can hold: kb = bit length of nv, so a masked
+        // kGap / ~0 (dropped pair) still compares above every valid key
+        const int kb = nv_ ? 32 - __builtin_clz(nv_) : 1;
         size_t b1 = 0, b2 = 0, b3 = 0;
-        SG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, ins_k_, ins_k2_, ins_v_, ins_v2_, (int)nins, 0, 32, stream_));
-        SG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, b2, del_k_, del_k2_, (int)ndel, 0, 64, stream_));
+        SG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, b1, ins_k_, ins_k2_, ins_v_, ins_v2_, (int)nins, 0, kb, stream_));
+        SG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, b2, del_k_, del_k2_, (int)ndel, 0, 32 + kb, stream_));
         SG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, ccount_, cpos_, (int)(chunk_bound + 1), stream_));
         ensure(cub_tmp_, cub_cap_, std::max({b1, b2, b3}) + 256);
         size_t b = cub_cap_;
-        if (nins) SG_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp_, b, ins_k_, ins_k2_, ins_v_, ins_v2_, (int)nins, 0, 32, stream_));
+        if (nins) SG_HIP(hipcub::DeviceRadixSort::SortPairs(cub_tmp_, b, ins_k_, ins_k2_, ins_v_, ins_v2_, (int)nins, 0, kb, stream_));
         b = cub_cap_;
-        if (ndel) SG_HIP(hipcub::DeviceRadixSort::SortKeys(cub_tmp_, b, del_k_, del_k2_, (int)ndel, 0, 64, stream_));
+        if (ndel) SG_HIP(hipcub::DeviceRadixSort::SortKeys(cub_tmp_, b, del_k_, del_k2_, (int)ndel, 0, 32 + kb, stream_));
         bv_.ins_key = ins_k2_;
         bv_.ins_val = ins_v2_;
         bv_.nins = nins;
