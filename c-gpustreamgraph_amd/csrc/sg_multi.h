@@ -298,11 +298,12 @@ __device__ void mlight(const GraphView &g, const MultiState &ms, const PassIO &i
     const uint32_t lane = lane_id();
     const uint32_t stamp_val = io.round + 1;
     const uint8_t kind = ms.kind[q.slot];
-    for (uint64_t b = gw * kWave; b < n; b += nw * kWave) {
+    const uint32_t per = light_width(n, nw);  // small frontiers spread over all waves
+    for (uint64_t b = gw * per; b < n; b += nw * per) {
         const uint64_t idx = b + lane;
         uint32_t u = 0, m = 0, len = 0, deg = 0;
         uint64_t start = 0;
-        if (idx < n) {
+        if (lane < per && idx < n) {
             u = io.in_list[idx];
             m = (uint32_t)atomicExch(&masks[u], 0ull);
             if (m) {
