@@ -2143,7 +2143,12 @@ public:
 
     void init_queries(const uint32_t *sources, const uint32_t *kinds, uint32_t nq) override {
         if (!loaded_) fail(SG_EINVAL, "sg_init before sg_load_csr");
-        if (nq == 0 || !sources || !kinds) fail(SG_EINVAL, "need at least one query");
+        if (nq == 0) {  // drop the queries: later snapshots only update the graph
+            nq_ = 0;
+            inited_ = true;
+            return;
+        }
+        if (!sources || !kinds) fail(SG_EINVAL, "null sources or kinds");
         if (nq > 4096) fail(SG_EINVAL, "at most 4096 queries per engine");
         for (uint32_t i = 0; i < nq; ++i) {
             if (sources[i] >= nv_) fail(SG_EINVAL, "query " + std::to_string(i) + ": source id >= nv");
@@ -2203,6 +2208,14 @@ public:
         auto t0 = std::chrono::steady_clock::now();
         const bool overflow = prepare(d_ins, nins, d_del, ndel);
         commit(overflow);
+        if (nq_ == 0) {  // no queries: the PMA update is the whole snapshot
+            st_.invalidated = 0;
+            st_.rounds = 0;
+            finish_batch();
+            st_.snapshots += 1;
+            st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            return;
+        }
         ensure_mspill();
         // 1. reset the slots whose tree edge was deleted, then their tree children (fixed point)
         frontier_reset();

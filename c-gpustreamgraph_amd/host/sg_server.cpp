@@ -214,6 +214,10 @@ struct sg_server {
                 }
                 if (g != SG_OK) fail_request(r, g, engine_error(multi));
             }
+            // the answers are copied out: drop the queries so that later
+            // snapshots do not maintain them (dropping makes no device call
+            // and cannot fail on a loaded engine)
+            if (s == SG_OK) (void)sg_init_queries(multi, nullptr, nullptr, 0);
             for (size_t i = 0; i < Q.size(); ++i) done.push_back({Q[i], qr[i]});
             st.queries += Q.size();
             st.edges_answered += ne * Q.size();
@@ -343,6 +347,10 @@ sg_status sg_server_create(const sg_server_config *cfg, uint32_t nv, uint64_t ne
     s->slots = cfg->query_slots ? std::min<uint32_t>(cfg->query_slots, 4096) : 64;
     std::string err;
     sg_status st = make_engine(cfg, SG_ALGO_MULTI, nv, ne, row, col, &s->multi, err);
+    if (st == SG_OK) {  // no queries yet: updates before the first one only change the graph
+        st = sg_init_queries(s->multi, nullptr, nullptr, 0);
+        if (st != SG_OK) err = sg_last_error(s->multi);
+    }
     if (st == SG_OK && (cfg->standing & SG_SERVE_CC)) st = make_engine(cfg, SG_ALGO_CC, nv, ne, row, col, &s->cc, err);
     if (st == SG_OK && (cfg->standing & SG_SERVE_PR)) {
         sg_server_config pc = *cfg;

@@ -186,6 +186,9 @@ sg_status sg_init(sg_engine *eng, const uint32_t *sources, uint32_t nsrc);
  * distances and parents equal those of a single-source SSSP/BFS engine.
  * sg_init on such an engine starts nsrc queries of one kind (BFS when
  * cfg.weight_mode is SG_WEIGHT_ONE, else SSSP). At most 4096 queries.
+ * nq = 0 (sources and kinds may be NULL) drops the queries: later snapshots
+ * only update the graph, which is how a request loop keeps answered one-shot
+ * queries from being maintained.
  * Batches the per-source runs of Engine::InitGraph + Engine::Start
  * (framework.cuh:389-439, 549-584); no reference counterpart as one call. */
 sg_status sg_init_queries(sg_engine *eng, const uint32_t *sources, const uint32_t *kinds, uint32_t nq);

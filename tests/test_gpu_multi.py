@@ -107,6 +107,33 @@ def test_same_result_alone_and_in_batch():
             s.close()
 
 
+def test_dropped_queries_leave_a_graph_only_engine():
+    # nq = 0 drops the queries; later snapshots only update the graph, and new
+    # queries on it match the oracle on the latest snapshot
+    nv, row, col = sgraph.gen_rmat(11, 8, seed=61)
+    ins, dele = sgraph.gen_updates(row, col, 11, 4, 400, 400, seed=62)
+    m = sgraph.Engine("multi", weight_mode="seeded", weight_seed=3)
+    try:
+        m.load_csr(row, col)
+        m.init_queries([], [])  # graph only from the start
+        r, c = row, col
+        m.apply(ins[0], dele[0])
+        r, c = O.apply_updates(r, c, ins[0], dele[0])
+        m.init_queries([0, 9], ["bfs", "sssp"])
+        check_all(m, r, c, [0, 9], ["bfs", "sssp"], "seeded", 3, "G1")
+        m.init_queries([], [])
+        with pytest.raises(sgraph.SgError):
+            m.query(0)
+        for t in (1, 2, 3):
+            m.apply(ins[t], dele[t])
+            r, c = O.apply_updates(r, c, ins[t], dele[t])
+            assert m.stats()["rounds"] == 0
+        m.init_queries([5, 5, 100], ["sssp", "bfs", "bfs"])
+        check_all(m, r, c, [5, 5, 100], ["sssp", "bfs", "bfs"], "seeded", 3, "G4")
+    finally:
+        m.close()
+
+
 def test_errors():
     nv, row, col = sgraph.gen_rmat(8, 4, seed=1)
     s = sgraph.Engine("sssp")

@@ -85,6 +85,23 @@ def test_mixed_requests_match_their_snapshots():
             assert ranks.min() >= kth * (1 - 1e-4), tag
 
 
+def test_updates_before_the_first_query():
+    nv, row, col = sgraph.gen_rmat(10, 8, seed=31)
+    ins, dele = sgraph.gen_updates(row, col, 10, 2, 200, 200, seed=32)
+    srv = sgraph.Server(row, col, query_slots=8)
+    try:
+        for t in range(2):  # no query has been seen: the batched engine holds only the graph
+            _, info = srv.wait(srv.submit(sgraph.REQ_UPDATE, ins=ins[t], dele=dele[t]))
+            assert info["snapshot"] == t + 1
+        r, c = snapshots(row, col, ins, dele)[2]
+        out, info = srv.wait(srv.submit(sgraph.REQ_BFS, 3), sgraph.REQ_BFS, 3)
+        assert info["snapshot"] == 2 and np.array_equal(out, O.bfs(r, c, 3))
+        out, info = srv.wait(srv.submit(sgraph.REQ_CC, 7), sgraph.REQ_CC, 7)
+        assert out[0] == O.cc(r, c)[7]
+    finally:
+        srv.close()
+
+
 def test_queries_are_batched_and_target_answers():
     nv, row, col = sgraph.gen_rmat(10, 8, seed=5)
     srv = sgraph.Server(row, col, query_slots=64, standing=0)
