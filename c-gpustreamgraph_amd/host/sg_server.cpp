@@ -45,6 +45,10 @@ double pct(std::vector<double> v, double p) {
 
 }  // namespace
 
+// Last error of a failed sg_server_* call, per calling thread (several
+// client threads submit and wait at once).
+thread_local std::string t_error;
+
 // An update batch the worker has applied, on its way to the PageRank thread.
 struct PrUpdate {
     uint64_t snapshot = 0;
@@ -65,7 +69,6 @@ struct sg_server {
     uint64_t next_id = 1;
     bool stop = false;
     std::thread worker;
-    std::string error;
 
     // PageRank thread: applies the worker's snapshots in order with deferred
     // convergence and answers top-k requests on the snapshot it has reached
@@ -358,6 +361,7 @@ sg_status sg_server_create(const sg_server_config *cfg, uint32_t nv, uint64_t ne
         st = make_engine(&pc, SG_ALGO_PR, nv, ne, row, col, &s->pr, err);
     }
     if (st != SG_OK) {
+        t_error = err;
         sg_destroy(s->multi);
         sg_destroy(s->cc);
         sg_destroy(s->pr);
@@ -396,33 +400,33 @@ sg_status sg_server_submit(sg_server *s, uint32_t kind, uint32_t arg, uint32_t t
     r->target = target;
     switch (kind) {
         case SG_REQ_UPDATE:
-            if ((nins && !ins_uv) || (ndel && !del_uv)) { s->error = "null update array"; return SG_EINVAL; }
+            if ((nins && !ins_uv) || (ndel && !del_uv)) { t_error = "null update array"; return SG_EINVAL; }
             for (uint64_t i = 0; i < 2 * nins; ++i)
-                if (ins_uv[i] >= s->nv) { s->error = "insert id >= nv"; return SG_EINVAL; }
+                if (ins_uv[i] >= s->nv) { t_error = "insert id >= nv"; return SG_EINVAL; }
             for (uint64_t i = 0; i < 2 * ndel; ++i)
-                if (del_uv[i] >= s->nv) { s->error = "delete id >= nv"; return SG_EINVAL; }
+                if (del_uv[i] >= s->nv) { t_error = "delete id >= nv"; return SG_EINVAL; }
             r->ins.assign(ins_uv, ins_uv + 2 * nins);
             r->del.assign(del_uv, del_uv + 2 * ndel);
             break;
         case SG_REQ_BFS:
         case SG_REQ_SSSP:
-            if (arg >= s->nv || (target != 0xFFFFFFFFu && target >= s->nv)) { s->error = "vertex id >= nv"; return SG_EINVAL; }
+            if (arg >= s->nv || (target != 0xFFFFFFFFu && target >= s->nv)) { t_error = "vertex id >= nv"; return SG_EINVAL; }
             break;
         case SG_REQ_CC:
-            if (!s->cc) { s->error = "CC labels are not maintained (SG_SERVE_CC)"; return SG_EINVAL; }
-            if (arg >= s->nv) { s->error = "vertex id >= nv"; return SG_EINVAL; }
+            if (!s->cc) { t_error = "CC labels are not maintained (SG_SERVE_CC)"; return SG_EINVAL; }
+            if (arg >= s->nv) { t_error = "vertex id >= nv"; return SG_EINVAL; }
             break;
         case SG_REQ_PR_TOPK:
-            if (!s->pr) { s->error = "PageRank is not maintained (SG_SERVE_PR)"; return SG_EINVAL; }
-            if (arg == 0) { s->error = "k must be at least 1"; return SG_EINVAL; }
+            if (!s->pr) { t_error = "PageRank is not maintained (SG_SERVE_PR)"; return SG_EINVAL; }
+            if (arg == 0) { t_error = "k must be at least 1"; return SG_EINVAL; }
             break;
         default:
-            s->error = "unknown request kind";
+            t_error = "unknown request kind";
             return SG_EINVAL;
     }
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->stop) { s->error = "server is stopping"; return SG_EINVAL; }
+    if (s->stop) { t_error = "server is stopping"; return SG_EINVAL; }
     const uint64_t id = s->next_id++;
     s->reqs[id] = std::move(r);
     *ticket = id;
@@ -442,14 +446,14 @@ sg_status sg_server_wait(sg_server *s, uint64_t ticket, void *out, size_t bytes,
     if (!s) return SG_EINVAL;
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end()) { s->error = "unknown ticket"; return SG_EINVAL; }
+    if (it == s->reqs.end()) { t_error = "unknown ticket"; return SG_EINVAL; }
     Request *r = it->second.get();
     s->cv_done.wait(lk, [&] { return r->done; });
     sg_status st = r->status;
-    if (st != SG_OK) s->error = r->error;
+    if (st != SG_OK) t_error = r->error;
     else if (!r->result.empty()) {
         if (!out || bytes < r->result.size()) {
-            s->error = "output smaller than the result (" + std::to_string(r->result.size()) + " bytes)";
+            t_error = "output smaller than the result (" + std::to_string(r->result.size()) + " bytes)";
             st = SG_EINVAL;
         } else {
             std::memcpy(out, r->result.data(), r->result.size());
@@ -474,7 +478,7 @@ sg_status sg_server_get_stats(sg_server *s, sg_server_stats *out) {
     return SG_OK;
 }
 
-const char *sg_server_last_error(const sg_server *s) { return s ? s->error.c_str() : "null server"; }
+const char *sg_server_last_error(const sg_server *) { return t_error.c_str(); }
 
 void sg_server_destroy(sg_server *s) {
     if (!s) return;

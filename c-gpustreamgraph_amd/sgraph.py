@@ -404,7 +404,7 @@ class Server:
         h = _P()
         st = lib().sg_server_create(C.byref(cfg), self.nv, len(col), _ptr(row), _ptr(col), C.byref(h))
         if st != SG_OK:
-            raise SgError(st, lib().sg_last_error(None).decode())
+            raise SgError(st, lib().sg_server_last_error(None).decode())
         self.h = h
 
     def _err(self, st):

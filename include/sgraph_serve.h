@@ -102,7 +102,8 @@ sg_status sg_server_wait(sg_server *s, uint64_t ticket, void *out, size_t bytes,
 
 sg_status sg_server_get_stats(sg_server *s, sg_server_stats *out);
 
-/* Text of the last error (the worker's, or of the last failed call). */
+/* Text of the calling thread's last failed sg_server_* call (for a failed
+ * request, the worker's message; for a failed sg_server_create, s may be NULL). */
 const char *sg_server_last_error(const sg_server *s);
 
 /* Finish the queued requests, stop the worker, release everything. */

@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 import sgraph
@@ -74,3 +75,19 @@ def test_create_without_device_fails_cleanly():
     st = sgraph.lib().sg_create(C.byref(cfg), C.byref(h))
     assert st == sgraph.SG_EDEVICE
     assert b"device" in sgraph.lib().sg_last_error(None).lower() or sgraph.lib().sg_last_error(None)
+
+
+def test_server_create_failure_reports_its_reason():
+    # device 99 does not exist (here no device does): the server's engines
+    # cannot be made, and the reason is the calling thread's last server error
+    nv, row, col = sgraph.gen_rmat(6, 4, seed=1)
+    cfg = sgraph.ServerConfig()
+    sgraph.lib().sg_server_config_default(C.byref(cfg))
+    cfg.engine.device = 99
+    h = C.c_void_p()
+    row = np.ascontiguousarray(row, dtype=np.uint64)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    st = sgraph.lib().sg_server_create(C.byref(cfg), nv, len(col), sgraph._ptr(row), sgraph._ptr(col), C.byref(h))
+    assert st == sgraph.SG_EDEVICE and not h.value
+    msg = sgraph.lib().sg_server_last_error(None)
+    assert msg and b"device" in msg.lower()
