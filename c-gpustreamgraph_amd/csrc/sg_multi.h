@@ -214,6 +214,103 @@ __device__ void mwalk_sparse(const GraphView &g, const MultiState &ms, const Pas
     }
 }
 
+// Position of the r-th (0-based) set bit of m; r < popcount(m).
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t m, uint32_t r) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t width = 32; width >= 1; width >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(m & ((1ull << width) - 1));
+        if (r >= c) {
+            r -= c;
+            m >>= width;
+            pos += width;
+        }
+    }
+    return pos;
+}
+
+// Walk for a mid-density slot mask (W/2 < popcount <= W, W = 8, 16 or 32
+// below the group's QS): the active slots are packed into W lanes, so a wave
+// step covers 64 / W edges instead of 64 / QS with most lanes idle. Lane
+// rank r takes the r-th active slot; the slot bits of an edge's hits are
+// OR-reduced over its W lanes before the one atomic per item. kMode as mwalk.
+template <int kMode>
+__device__ __forceinline__ void mwalk_compact(const GraphView &g, const MultiState &ms, const PassIO &io, OutStage &st, uint32_t *stamp,
+                              uint32_t stamp_val, uint32_t u, uint32_t grp, uint64_t mask, uint32_t wlog, uint64_t start,
+                              uint32_t len, unsigned long long &edges) {
+    const uint32_t lane = lane_id();
+    const uint32_t W = 1u << wlog;
+    const uint32_t r = lane & (W - 1);  // rank among the active slots
+    const uint32_t sub = lane >> wlog;   // edge of the wave step
+    const uint32_t per = kWave >> wlog;  // edges per wave step
+    const bool on = r < (uint32_t)__popcll(mask);
+    const uint32_t slot = on ? nth_set_bit(mask, r) : 0u;
+    const uint32_t qi = grp * ms.QS + slot;
+    const uint8_t kind = on ? ms.kind[qi] : (uint8_t)kQNone;
+    uint32_t du = kInf;
+    if (kMode == 0 && on) du = hi32(ld_relaxed(&ms.best[(uint64_t)u * ms.KP + qi]));
+    const bool live = on && kind != kQNone && (kMode == 1 || du != kInf);
+    if (!__any(live)) return;
+    edges += len;
+    for (uint32_t e0 = 0; e0 < len; e0 += per * kMUnroll) {
+        uint32_t vv[kMUnroll];
+#pragma unroll
+        for (int j = 0; j < kMUnroll; ++j) {
+            const uint32_t e = e0 + j * per + sub;
+            vv[j] = e < len ? edge_at(g, start + e) : kGap;
+        }
+        uint64_t pk[kMUnroll];
+        uint32_t cand[kMUnroll];
+#pragma unroll
+        for (int j = 0; j < kMUnroll; ++j) {
+            const bool ok = live && vv[j] != kGap;
+            if (kMode == 0) {
+                cand[j] = ok ? du + qweight(ms, kind, u, vv[j]) : kInf;
+                pk[j] = ok ? ld_peek(&ms.best[(uint64_t)vv[j] * ms.KP + qi]) : 0ull;
+            } else {
+                cand[j] = 0;
+                pk[j] = ok ? ld_relaxed(&ms.best[(uint64_t)vv[j] * ms.KP + qi]) : 0ull;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kMUnroll; ++j) {
+            const uint32_t v = vv[j];
+            bool hit = false;
+            if (kMode == 0) {
+                const unsigned long long nw = pack(cand[j], u);
+                if (cand[j] != kInf && nw < pk[j]) {
+                    const unsigned long long old = atomicMin(&ms.best[(uint64_t)v * ms.KP + qi], nw);
+                    hit = cand[j] < hi32(old);
+                }
+            } else {
+                hit = live && v != kGap && lo32(pk[j]) == u && hi32(pk[j]) != kInf;
+            }
+            // slot bits of this edge's hits, gathered in every lane of its W-lane group
+            unsigned long long bits = hit ? (1ull << slot) : 0ull;
+            for (uint32_t o = 1; o < W; o <<= 1) bits |= __shfl_xor(bits, (int)o, kWave);
+            bool app = false;
+            if (kMode == 0) {
+                if (r == 0 && bits) {
+                    atomicOr(&ms.act[(uint64_t)v * ms.G + grp], bits);
+                    app = atomicExch(&stamp[v], stamp_val) != stamp_val;
+                }
+            } else {
+                unsigned long long fresh = 0;
+                if (r == 0 && bits) fresh = bits & ~atomicOr(&ms.inv[(uint64_t)v * ms.G + grp], bits);
+                fresh = __shfl(fresh, (int)(sub << wlog), kWave);
+                if (r == 0 && fresh) {
+                    atomicOr(&ms.ninv[(uint64_t)v * ms.G + grp], fresh);
+                    atomicOr(&ms.invbit[v >> 5], 1u << (v & 31));
+                    coarse_mark(ms.coarse, ms.cshift, v);
+                    app = atomicExch(&stamp[v], stamp_val) != stamp_val;
+                }
+                if (on && ((fresh >> slot) & 1u)) ms.best[(uint64_t)v * ms.KP + qi] = ~0ull;
+            }
+            stage_push(app, v, st, io);
+        }
+    }
+}
+
 // Walk len edges of u from slot position start for the slots in mask.
 // kMode 0: relax (activations go to act); kMode 1: reset tree children whose
 // parent is u (resets go to inv / ninv and the vertex bitmaps).
@@ -221,8 +318,14 @@ template <int kMode>
 __device__ void mwalk(const GraphView &g, const MultiState &ms, const PassIO &io, OutStage &st, uint32_t *stamp,
                       uint32_t stamp_val, uint32_t u, uint32_t grp, uint64_t mask, uint64_t start, uint32_t len,
                       unsigned long long &edges) {
-    if (ms.QS > kSparseSlots && __popcll(mask) <= kSparseSlots) {
+    const uint32_t np = (uint32_t)__popcll(mask);
+    if (ms.QS > kSparseSlots && np <= kSparseSlots) {
         mwalk_sparse<kMode>(g, ms, io, st, stamp, stamp_val, u, grp, mask, start, len, edges);
+        return;
+    }
+    if (ms.QS >= 16 && 2 * np <= ms.QS) {  // 8, 16 or 32 lanes per edge instead of QS
+        const uint32_t wlog = np <= 8 ? 3u : np <= 16 ? 4u : 5u;
+        mwalk_compact<kMode>(g, ms, io, st, stamp, stamp_val, u, grp, mask, wlog, start, len, edges);
         return;
     }
     const QLane q = qlane(ms);
@@ -421,34 +524,45 @@ __global__ void __launch_bounds__(kBlock) k_mround(GraphView g, MultiState ms, P
     unsigned long long edges = 0, verts = 0;
     const bool light = ms.G == 1 && ms.QS <= kMLightQS;
     if (light) mlight<kMode>(g, ms, io, st, s_light[w], spill, spill_cap, stamp, n, gw, nw, edges, verts);
+    // (frontier vertex, group) items, then the previous round's spilled
+    // descriptors, in one loop with one walk call site
     const uint64_t items = light ? 0 : (uint64_t)n * ms.G;
-    for (uint64_t it = gw; it < items; it += nw) {
-        const uint32_t u = io.in_list[it / ms.G], grp = (uint32_t)(it % ms.G);
-        unsigned long long m = 0;
-        if (lane == 0) m = atomicExch(&masks[(uint64_t)u * ms.G + grp], 0ull);
-        m = __shfl(m, 0, kWave);
-        if (m == 0) continue;
-        ++verts;
-        const uint64_t start = list_start(g, u);
-        if (lane == 0) count_hot(g, u);
-        const uint32_t deg = g.deg[u];
-        const uint32_t first = min(deg, kMEdgeChunk);
-        if (deg > first) {
-            const uint32_t nch = (deg - first + kMEdgeChunk - 1) / kMEdgeChunk;
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(io.heavy_cnt, nch);
-            base = __shfl(base, 0, kWave);
-            for (uint32_t c = lane; c < nch; c += kWave) {
-                const uint32_t lo = first + c * kMEdgeChunk;
-                if (base + c < spill_cap) spill[base + c] = MDesc{start + lo, m, u, min(kMEdgeChunk, deg - lo), grp, 0};
-                else atomicOr(io.stats + 3, 2ull);
+    for (uint64_t it = gw; it < items + nh; it += nw) {
+        uint32_t u, grp, len;
+        unsigned long long m;
+        uint64_t start;
+        if (it < items) {
+            u = io.in_list[it / ms.G];
+            grp = (uint32_t)(it % ms.G);
+            m = 0;
+            if (lane == 0) m = atomicExch(&masks[(uint64_t)u * ms.G + grp], 0ull);
+            m = __shfl(m, 0, kWave);
+            if (m == 0) continue;
+            ++verts;
+            start = list_start(g, u);
+            if (lane == 0) count_hot(g, u);
+            const uint32_t deg = g.deg[u];
+            len = min(deg, kMEdgeChunk);
+            if (deg > len) {
+                const uint32_t nch = (deg - len + kMEdgeChunk - 1) / kMEdgeChunk;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(io.heavy_cnt, nch);
+                base = __shfl(base, 0, kWave);
+                for (uint32_t c = lane; c < nch; c += kWave) {
+                    const uint32_t lo = len + c * kMEdgeChunk;
+                    if (base + c < spill_cap) spill[base + c] = MDesc{start + lo, m, u, min(kMEdgeChunk, deg - lo), grp, 0};
+                    else atomicOr(io.stats + 3, 2ull);
+                }
             }
+        } else {
+            const MDesc d = sin[it - items];
+            u = d.u;
+            grp = d.grp;
+            m = d.mask;
+            start = d.start;
+            len = d.len;
         }
-        mwalk<kMode>(g, ms, io, st, stamp, io.round + 1, u, grp, m, start, first, edges);
-    }
-    for (uint64_t h = gw; h < nh; h += nw) {
-        const MDesc d = sin[h];
-        mwalk<kMode>(g, ms, io, st, stamp, io.round + 1, d.u, d.grp, d.mask, d.start, d.len, edges);
+        mwalk<kMode>(g, ms, io, st, stamp, io.round + 1, u, grp, m, start, len, edges);
     }
     stage_flush(st, io);
     // one atomic per counter and block
