@@ -48,7 +48,7 @@ def run_multi(scale, ef, nsnap, nupd, sources, kinds, wmode="seeded", seed=9, gs
         e.close()
 
 
-@pytest.mark.parametrize("nq", [1, 2, 3, 64, 65])
+@pytest.mark.parametrize("nq", [1, 2, 3, 12, 20, 64, 65])  # QS 1, 2, 4, 16, 32, 64, 64 x 2 groups
 def test_query_counts(nq):
     rng = np.random.default_rng(nq)
     nv = 1 << 10
