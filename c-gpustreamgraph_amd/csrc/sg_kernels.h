@@ -308,11 +308,6 @@ __device__ __forceinline__ void walk_staged(const GraphView &g, const PassIO &io
     }
 }
 
-// Light tier body: waves [gwave, +nwaves) take 64 work items at a time, run
-// the vertex functor, and walk the scheduled lists that fit the wave budget;
-// longer lists, and lists past the budget, are cut into kChunk-edge
-// descriptors for the heavy tier, so no wave carries more than
-// budget + heavy_deg edges per batch.
 // Power of two in [1, kWave] at least ceil(n / nwaves).
 __device__ __forceinline__ uint32_t light_width(uint32_t n, uint64_t nwaves) {
     const uint64_t q = (n + nwaves - 1) / nwaves;
@@ -321,6 +316,11 @@ __device__ __forceinline__ uint32_t light_width(uint32_t n, uint64_t nwaves) {
     return per;
 }
 
+// Light tier body: waves [gwave, +nwaves) take up to 64 work items at a time
+// (light_width), run the vertex functor, and walk the scheduled lists that
+// fit the wave budget; longer lists, and lists past the budget, are cut into
+// kChunk-edge descriptors for the heavy tier, so no wave carries more than
+// budget + heavy_deg edges per batch.
 template <class VFn, class EFn>
 __device__ __forceinline__ void light_body(const GraphView &g, const PassIO &io, const VFn &vfn, const EFn &efn,
                                            uint32_t n, uint64_t gwave, uint64_t nwaves, LightShared &sh, OutStage &st,
