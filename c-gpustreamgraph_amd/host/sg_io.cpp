@@ -187,11 +187,107 @@ struct KeyMap {
     }
 };
 
+// Snapshot file: magic, then u32 nv, u32 nsnap, u64 ne, u64 total inserts,
+// u64 total deletes, then row[nv + 1] (u64), col[ne] (u32), sizes[2 nsnap]
+// (u64), inserts and deletes as (u, v) u32 pairs; all little-endian.
+constexpr char kSnapMagic[8] = {'S', 'G', 'S', 'N', 'A', 'P', '0', '1'};
+
+struct FileCloser {
+    FILE *f;
+    ~FileCloser() {
+        if (f) std::fclose(f);
+    }
+};
+
+bool write_all(FILE *f, const void *p, size_t bytes) { return bytes == 0 || std::fwrite(p, 1, bytes, f) == bytes; }
+bool read_all(FILE *f, void *p, size_t bytes) { return bytes == 0 || std::fread(p, 1, bytes, f) == bytes; }
+
 }  // namespace
 
 extern "C" {
 
 void sg_free(void *p) { std::free(p); }
+
+sg_status sg_write_snapshot_file(const char *path, uint32_t nv, uint64_t ne, const uint64_t *row, const uint32_t *col,
+                                 uint32_t nsnap, const uint64_t *sizes, const uint32_t *ins_uv,
+                                 const uint32_t *del_uv) {
+    if (!path || !row || (ne && !col) || (nsnap && !sizes)) return SG_EINVAL;
+    if (row[0] != 0 || row[nv] != ne) return SG_EINVAL;
+    uint64_t ti = 0, td = 0;
+    for (uint32_t i = 0; i < nsnap; ++i) {
+        ti += sizes[2 * i];
+        td += sizes[2 * i + 1];
+    }
+    if ((ti && !ins_uv) || (td && !del_uv)) return SG_EINVAL;
+    FileCloser fc{std::fopen(path, "wb")};
+    if (!fc.f) return SG_EIO;
+    const uint32_t h32[2] = {nv, nsnap};
+    const uint64_t h64[3] = {ne, ti, td};
+    const bool ok = write_all(fc.f, kSnapMagic, sizeof kSnapMagic) && write_all(fc.f, h32, sizeof h32) &&
+                    write_all(fc.f, h64, sizeof h64) && write_all(fc.f, row, ((size_t)nv + 1) * 8) &&
+                    write_all(fc.f, col, (size_t)ne * 4) && write_all(fc.f, sizes, (size_t)nsnap * 16) &&
+                    write_all(fc.f, ins_uv, (size_t)ti * 8) && write_all(fc.f, del_uv, (size_t)td * 8);
+    if (!ok) return SG_EIO;
+    const int rc = std::fclose(fc.f);
+    fc.f = nullptr;
+    return rc == 0 ? SG_OK : SG_EIO;
+}
+
+sg_status sg_read_snapshot_file(const char *path, uint32_t *nv_out, uint64_t *ne_out, uint64_t **row_out,
+                                uint32_t **col_out, uint32_t *nsnap_out, uint64_t **sizes_out, uint32_t **ins_out,
+                                uint32_t **del_out) {
+    if (!path || !nv_out || !ne_out || !row_out || !col_out || !nsnap_out || !sizes_out || !ins_out || !del_out)
+        return SG_EINVAL;
+    FileCloser fc{std::fopen(path, "rb")};
+    if (!fc.f) return SG_EIO;
+    char magic[8];
+    uint32_t h32[2];
+    uint64_t h64[3];
+    if (!read_all(fc.f, magic, 8) || std::memcmp(magic, kSnapMagic, 8) != 0 || !read_all(fc.f, h32, sizeof h32) ||
+        !read_all(fc.f, h64, sizeof h64))
+        return SG_EIO;
+    const uint32_t nv = h32[0], nsnap = h32[1];
+    const uint64_t ne = h64[0], ti = h64[1], td = h64[2];
+    // the header must agree with the file's length before anything is allocated
+    const uint64_t want = 8 + sizeof h32 + sizeof h64 + ((uint64_t)nv + 1) * 8 + ne * 4 + (uint64_t)nsnap * 16 +
+                          ti * 8 + td * 8;
+    if (std::fseek(fc.f, 0, SEEK_END) != 0) return SG_EIO;
+    const long long len = std::ftell(fc.f);
+    if (len < 0 || (uint64_t)len != want || std::fseek(fc.f, 8 + sizeof h32 + sizeof h64, SEEK_SET) != 0) return SG_EIO;
+    uint64_t *row = (uint64_t *)std::malloc(((size_t)nv + 1) * 8);
+    uint32_t *col = (uint32_t *)std::malloc(std::max<size_t>((size_t)ne * 4, 4));
+    uint64_t *sizes = (uint64_t *)std::malloc(std::max<size_t>((size_t)nsnap * 16, 16));
+    uint32_t *ins = (uint32_t *)std::malloc(std::max<size_t>((size_t)ti * 8, 8));
+    uint32_t *del = (uint32_t *)std::malloc(std::max<size_t>((size_t)td * 8, 8));
+    auto drop = [&](sg_status st) {
+        std::free(row); std::free(col); std::free(sizes); std::free(ins); std::free(del);
+        return st;
+    };
+    if (!row || !col || !sizes || !ins || !del) return drop(SG_ENOMEM);
+    if (!read_all(fc.f, row, ((size_t)nv + 1) * 8) || !read_all(fc.f, col, (size_t)ne * 4) ||
+        !read_all(fc.f, sizes, (size_t)nsnap * 16) || !read_all(fc.f, ins, (size_t)ti * 8) ||
+        !read_all(fc.f, del, (size_t)td * 8))
+        return drop(SG_EIO);
+    // structure: offsets from 0 to ne, non-decreasing; stream sizes that sum to the totals
+    if (row[0] != 0 || row[nv] != ne) return drop(SG_EIO);
+    for (uint32_t v = 0; v < nv; ++v)
+        if (row[v + 1] < row[v]) return drop(SG_EIO);
+    uint64_t si = 0, sd = 0;
+    for (uint32_t i = 0; i < nsnap; ++i) {
+        si += sizes[2 * i];
+        sd += sizes[2 * i + 1];
+    }
+    if (si != ti || sd != td) return drop(SG_EIO);
+    *nv_out = nv;
+    *ne_out = ne;
+    *row_out = row;
+    *col_out = col;
+    *nsnap_out = nsnap;
+    *sizes_out = sizes;
+    *ins_out = ins;
+    *del_out = del;
+    return SG_OK;
+}
 
 uint32_t sg_edge_weight(uint32_t mode, uint64_t seed, uint32_t u, uint32_t v) {
     return sg_weight(mode, seed, u, v);

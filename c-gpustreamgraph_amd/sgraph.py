@@ -132,6 +132,10 @@ SIGNATURES = [
     ("sg_read_stream_sizes", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(_P)]),
     ("sg_read_updates", C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(_P), C.POINTER(C.c_uint64), C.POINTER(_P)]),
     ("sg_write_updates", C.c_int, [C.c_char_p, C.c_char_p, C.c_uint32, _P, _P, _P]),
+    ("sg_write_snapshot_file", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint64, _P, _P, C.c_uint32, _P, _P, _P]),
+    ("sg_read_snapshot_file", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(_P),
+                                        C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P), C.POINTER(_P),
+                                        C.POINTER(_P)]),
     ("sg_gen_rmat", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_int,
                               C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(_P), C.POINTER(_P)]),
     ("sg_gen_updates", C.c_int, [C.c_uint32, _P, _P, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint32,
@@ -206,6 +210,31 @@ def write_updates(update_path, size_path, sizes, ins, dele):
     ins = np.ascontiguousarray(ins, dtype=np.uint32).reshape(-1)
     dele = np.ascontiguousarray(dele, dtype=np.uint32).reshape(-1)
     _check(lib().sg_write_updates(update_path.encode(), size_path.encode(), len(sizes) // 2, _ptr(sizes), _ptr(ins), _ptr(dele)))
+
+
+def write_snapshot_file(path, row, col, sizes, ins, dele):
+    """One binary file with G_0 (CSR), the stream sizes [[n_add, n_del], ...] and
+    the insert / delete sequences as (u, v) pairs."""
+    row = np.ascontiguousarray(row, dtype=np.uint64)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    sizes = np.ascontiguousarray(sizes, dtype=np.uint64).reshape(-1)
+    ins = np.ascontiguousarray(ins, dtype=np.uint32).reshape(-1)
+    dele = np.ascontiguousarray(dele, dtype=np.uint32).reshape(-1)
+    _check(lib().sg_write_snapshot_file(path.encode(), len(row) - 1, len(col), _ptr(row), _ptr(col), len(sizes) // 2,
+                                        _ptr(sizes), _ptr(ins), _ptr(dele)))
+
+
+def read_snapshot_file(path):
+    """(row, col, sizes [nsnap, 2], ins [n, 2], dele [n, 2]) of a snapshot file."""
+    nv, ne, ns = C.c_uint32(), C.c_uint64(), C.c_uint32()
+    pr, pc, psz, pi, pd = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    _check(lib().sg_read_snapshot_file(path.encode(), C.byref(nv), C.byref(ne), C.byref(pr), C.byref(pc), C.byref(ns),
+                                       C.byref(psz), C.byref(pi), C.byref(pd)))
+    row = _take(pr, nv.value + 1, np.uint64)
+    col = _take(pc, ne.value, np.uint32)
+    sizes = _take(psz, 2 * ns.value, np.uint64).reshape(-1, 2)
+    ti, td = int(sizes[:, 0].sum()), int(sizes[:, 1].sum())
+    return row, col, sizes, _take(pi, 2 * ti, np.uint32).reshape(-1, 2), _take(pd, 2 * td, np.uint32).reshape(-1, 2)
 
 
 def gen_rmat(scale, edge_factor, seed=DEFAULT_SEED, a=0.57, b=0.19, c=0.19, permute=False):

@@ -75,6 +75,22 @@ sg_status sg_gen_updates(uint32_t nv, const uint64_t *row, const uint32_t *col,
                          double hub_frac, double hub_top, uint64_t seed,
                          uint32_t **ins_uv, uint32_t **del_uv);
 
+/* Binary snapshot file: G_0 as CSR, the stream sizes and the insert / delete
+ * sequences in one file, read without parsing. No counterpart in the
+ * reference, which re-parses its text files on every run (the graph twice,
+ * src/utils/parser.cpp:324-401). Layout, little-endian: "SGSNAP01", u32 nv,
+ * u32 nsnap, u64 ne, u64 total inserts, u64 total deletes, u64 row[nv + 1],
+ * u32 col[ne], u64 sizes[2 * nsnap] (as sg_read_stream_sizes), u32 pairs of
+ * the inserts, then of the deletes. Ids are checked by sg_load_csr and the
+ * apply calls, as for the text formats; the reader checks the header against
+ * the file length, the offsets and the size totals (SG_EIO otherwise). */
+sg_status sg_write_snapshot_file(const char *path, uint32_t nv, uint64_t ne, const uint64_t *row,
+                                 const uint32_t *col, uint32_t nsnap, const uint64_t *sizes,
+                                 const uint32_t *ins_uv, const uint32_t *del_uv);
+sg_status sg_read_snapshot_file(const char *path, uint32_t *nv, uint64_t *ne, uint64_t **row,
+                                uint32_t **col, uint32_t *nsnap, uint64_t **sizes,
+                                uint32_t **ins_uv, uint32_t **del_uv);
+
 /* SSSP edge weight of the engine for (u, v) (sg_weight_mode). */
 uint32_t sg_edge_weight(uint32_t mode, uint64_t seed, uint32_t u, uint32_t v);
 

@@ -110,3 +110,45 @@ def test_weights_match_oracle():
             assert w == O.weight(m, 1234567, u, v)
             assert 1 <= w <= 128
         assert sgraph.edge_weight("seeded", 99, u, v) <= 63
+
+
+def test_snapshot_file_round_trip_and_checks(tmp_path):
+    # the binary snapshot file holds exactly what the text files hold: the
+    # README fixture's graph and stream read back equal, and a truncated or
+    # altered file is refused (SG_EIO), never half-read
+    nv, row, col = sgraph.gen_rmat(9, 8, seed=5)
+    ins, dele = sgraph.gen_updates(row, col, 9, 3, 70, 40, seed=6)
+    sizes = np.array([[70, 40]] * 3, dtype=np.uint64)
+    p = str(tmp_path / "g.sgsnap")
+    sgraph.write_snapshot_file(p, row, col, sizes, np.concatenate(ins), np.concatenate(dele))
+    r2, c2, s2, i2, d2 = sgraph.read_snapshot_file(p)
+    assert np.array_equal(r2, row) and np.array_equal(c2, col) and np.array_equal(s2, sizes)
+    assert np.array_equal(i2, np.concatenate(ins).reshape(-1, 2)) and np.array_equal(d2, np.concatenate(dele).reshape(-1, 2))
+    # snapshot t of the file applies like the generator's snapshot t
+    r, c = O.apply_updates(r2, c2, i2[70:140], d2[40:80])
+    r0, c0 = O.apply_updates(row, col, ins[1], dele[1])
+    assert np.array_equal(r, r0) and np.array_equal(c, c0)
+    raw = open(p, "rb").read()
+    for bad in (raw[:-4], raw + b"\0", b"SGSNAP02" + raw[8:]):
+        q = str(tmp_path / "bad.sgsnap")
+        open(q, "wb").write(bad)
+        with pytest.raises(sgraph.SgError) as e:
+            sgraph.read_snapshot_file(q)
+        assert e.value.status == sgraph.SG_EIO
+    # offsets that do not end at ne
+    hdr = 8 + 8 + 24
+    b = bytearray(raw)
+    b[hdr + 8 * nv: hdr + 8 * nv + 8] = (int(row[-1]) - 1).to_bytes(8, "little")
+    q = str(tmp_path / "bad2.sgsnap")
+    open(q, "wb").write(bytes(b))
+    with pytest.raises(sgraph.SgError):
+        sgraph.read_snapshot_file(q)
+    # the README fixture's three text files as one snapshot file
+    _, gr, gc = sgraph.read_edge_list(os.path.join(GOLDEN, "readme_graph.txt"))
+    gi, gd = sgraph.read_updates(os.path.join(GOLDEN, "readme_update.txt"))
+    gs = sgraph.read_stream_sizes(os.path.join(GOLDEN, "readme_stream_size.txt"))
+    q = str(tmp_path / "readme.sgsnap")
+    sgraph.write_snapshot_file(q, gr, gc, gs, gi, gd)
+    r3, c3, s3, i3, d3 = sgraph.read_snapshot_file(q)
+    assert np.array_equal(r3, gr) and np.array_equal(c3, gc) and np.array_equal(s3.reshape(-1), np.asarray(gs).reshape(-1))
+    assert np.array_equal(i3, gi) and np.array_equal(d3, gd)
