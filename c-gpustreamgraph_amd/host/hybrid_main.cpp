@@ -60,6 +60,8 @@ void define_flags() {
     def_s("format", "market_big", "graph format (market_big)");
     def_s("updatefile", "", "update file: 'a u v' / 'd u v' lines");
     def_s("update_size", "", "stream-size file: 'n_add n_del' per snapshot");
+    def_s("snapshot_file", "", "binary snapshot file (graph, stream sizes, updates) read instead of the three text files");
+    def_s("write_snapshot_file", "", "write the inputs read as a binary snapshot file, then run");
     def_i("cache", 4, "HBM edge cache level 0..4 (graph_datum.cuh:151-163); used when the edge array is in host memory");
     def_i("SEGMENT", 1, "vertex segments streamed from host memory (framework.cuh:310-384)");
     def_i("n_stream", 3, "number of streams");
@@ -210,7 +212,10 @@ int main(int argc, char **argv) {
     if (!parse_flags(argc, argv)) return 1;
     const uint32_t algo = SG_CLI_ALGO;
     static const char *names[] = {"page rank", "sssp", "bfs", "cc"};
-    if (S("graphfile").empty()) { std::fprintf(stderr, "ERROR: -graphfile is required\n"); return 1; }
+    if (S("graphfile").empty() && S("snapshot_file").empty()) {
+        std::fprintf(stderr, "ERROR: -graphfile (or -snapshot_file) is required\n");
+        return 1;
+    }
     if (S("format") != "market_big" && S("format") != "market") {
         std::fprintf(stderr, "ERROR: -format %s is not supported (market_big)\n", S("format").c_str());
         return 1;
@@ -219,9 +224,53 @@ int main(int argc, char **argv) {
     uint32_t nv = 0;
     uint64_t ne = 0, *row = nullptr;
     uint32_t *col = nullptr;
+    // stream: sizes[2 t] / sizes[2 t + 1] inserts / deletes of snapshot t, taken in order from ins / del
+    uint32_t nsnap = 0;
+    uint64_t *sizes = nullptr, nins = 0, ndel = 0;
+    uint32_t *ins = nullptr, *del = nullptr;
+    bool have_updates = false;
     double t0 = now_ms();
-    sg_status st = sg_read_edge_list(S("graphfile").c_str(), &nv, &ne, &row, &col);
-    if (st != SG_OK || nv == 0) { std::fprintf(stderr, "ERROR: cannot read graph '%s'\n", S("graphfile").c_str()); return 1; }
+    sg_status st;
+    if (!S("snapshot_file").empty()) {
+        st = sg_read_snapshot_file(S("snapshot_file").c_str(), &nv, &ne, &row, &col, &nsnap, &sizes, &ins, &del);
+        if (st != SG_OK || nv == 0) {
+            std::fprintf(stderr, "ERROR: cannot read snapshot file '%s'\n", S("snapshot_file").c_str());
+            return 1;
+        }
+        for (uint32_t t = 0; t < nsnap; ++t) {
+            nins += sizes[2 * t];
+            ndel += sizes[2 * t + 1];
+        }
+        have_updates = nsnap > 0;
+    } else {
+        st = sg_read_edge_list(S("graphfile").c_str(), &nv, &ne, &row, &col);
+        if (st != SG_OK || nv == 0) { std::fprintf(stderr, "ERROR: cannot read graph '%s'\n", S("graphfile").c_str()); return 1; }
+        if (!S("updatefile").empty()) {
+            if (S("update_size").empty() || sg_read_stream_sizes(S("update_size").c_str(), &nsnap, &sizes) != SG_OK) {
+                std::fprintf(stderr, "ERROR: cannot read -update_size file\n");
+                return 1;
+            }
+            if (sg_read_updates(S("updatefile").c_str(), &nins, &ins, &ndel, &del) != SG_OK) {
+                std::fprintf(stderr, "ERROR: cannot read -updatefile\n");
+                return 1;
+            }
+            have_updates = true;
+        }
+    }
+    if (!S("write_snapshot_file").empty()) {
+        // the stream the run would apply: the snapshots the updates cover
+        uint32_t ns = 0;
+        uint64_t ta = 0, td = 0;
+        while (ns < nsnap && ta + sizes[2 * ns] <= nins && td + sizes[2 * ns + 1] <= ndel) {
+            ta += sizes[2 * ns];
+            td += sizes[2 * ns + 1];
+            ++ns;
+        }
+        if (sg_write_snapshot_file(S("write_snapshot_file").c_str(), nv, ne, row, col, ns, sizes, ins, del) != SG_OK) {
+            std::fprintf(stderr, "ERROR: cannot write snapshot file '%s'\n", S("write_snapshot_file").c_str());
+            return 1;
+        }
+    }
     if (B("verbose")) std::printf("graph: %u vertices, %llu edges (read %.1f ms)\n", nv, (unsigned long long)ne, now_ms() - t0);
 
     sg_config cfg;
@@ -262,20 +311,7 @@ int main(int argc, char **argv) {
     std::vector<uint32_t> u32;
     std::vector<float> f32;
 
-    if (!S("updatefile").empty()) {
-        uint32_t nsnap = 0;
-        uint64_t *sizes = nullptr, nins = 0, ndel = 0;
-        uint32_t *ins = nullptr, *del = nullptr;
-        if (S("update_size").empty() || sg_read_stream_sizes(S("update_size").c_str(), &nsnap, &sizes) != SG_OK) {
-            std::fprintf(stderr, "ERROR: cannot read -update_size file\n");
-            sg_destroy(eng);
-            return 1;
-        }
-        if (sg_read_updates(S("updatefile").c_str(), &nins, &ins, &ndel, &del) != SG_OK) {
-            std::fprintf(stderr, "ERROR: cannot read -updatefile\n");
-            sg_destroy(eng);
-            return 1;
-        }
+    if (have_updates) {
         if (I("snapshots") >= 0) nsnap = (uint32_t)std::min<long long>(nsnap, I("snapshots"));
         uint64_t ia = 0, id = 0;
         double total = 0;
@@ -304,10 +340,10 @@ int main(int argc, char **argv) {
             }
         }
         std::printf("%u snapshots in %.3f ms\n", nsnap, total);
-        sg_free(sizes);
-        sg_free(ins);
-        sg_free(del);
     }
+    sg_free(sizes);
+    sg_free(ins);
+    sg_free(del);
     if ((st = gather(u32, f32)) != SG_OK) return fail_engine(eng, "sg_gather", st);
     if (algo == SG_ALGO_PR) {
         for (uint32_t i = 0; i < std::min<uint32_t>(20, nv); ++i) std::printf("v %u data %f\n", i, f32[i]);  // hybrid_pr.cu:190-193

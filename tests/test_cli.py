@@ -95,3 +95,34 @@ def test_rmat_files_seeded_weights(tmp_path):
     for t in range(3):
         rr, cc = O.apply_updates(rr, cc, ins[t], dele[t])
         assert read_result(str(tmp_path / f"snapshot_{t}.txt")) == O.sssp(rr, cc, 3, "seeded", 7).tolist()
+
+
+def test_write_snapshot_file_converts_text_inputs(tmp_path):
+    # the conversion happens before any device call, so it runs without a GPU
+    import sgraph
+    snap = str(tmp_path / "readme.sgsnap")
+    run("hybrid_bfs", *README_ARGS, "-write_snapshot_file", snap)
+    row, col, sizes, ins, dele = sgraph.read_snapshot_file(snap)
+    _, gr, gc = sgraph.read_edge_list(G)
+    gi, gd = sgraph.read_updates(U)
+    gs = sgraph.read_stream_sizes(S)
+    assert np.array_equal(row, gr) and np.array_equal(col, gc) and np.array_equal(sizes, gs)
+    assert np.array_equal(ins, gi[: int(gs[:, 0].sum())]) and np.array_equal(dele, gd[: int(gs[:, 1].sum())])
+    r = run("hybrid_bfs", "-snapshot_file", str(tmp_path / "missing.sgsnap"))
+    assert r.returncode == 1 and "snapshot file" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,key", [("hybrid_sssp", "sssp_hash128"), ("hybrid_cc", "cc")])
+def test_snapshot_file_run_equals_text_run(tmp_path, name, key):
+    exp = json.load(open(os.path.join(GOLDEN, "readme_expected.json")))["snapshots"]
+    snap = str(tmp_path / "readme.sgsnap")
+    out = str(tmp_path / "text.txt")
+    r = run(name, *README_ARGS, "-source_node", "0", "-output", out, "-write_snapshot_file", snap)
+    assert r.returncode == 0, r.stderr
+    out2 = str(tmp_path / "bin.txt")
+    r = run(name, "-snapshot_file", snap, "-source_node", "0", "-output", out2, "-dump_dir", str(tmp_path))
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    assert read_result(out2) == read_result(out) == exp[2][key]
+    assert read_result(str(tmp_path / "snapshot_0.txt")) == exp[1][key]
