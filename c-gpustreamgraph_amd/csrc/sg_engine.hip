@@ -1786,6 +1786,113 @@ public:
         }
         sync();
     }
+
+    // ---- checkpoint (sg_save_state / sg_restore_state) -------------------------
+    // Device arrays of the vertex state (allocated when the engine has none
+    // yet) and the traversal source.
+    virtual void state_arrays(std::vector<std::pair<void *, size_t>> &, uint32_t &) {
+        fail(SG_EINVAL, "checkpoints are for single-program engines (PR, SSSP, BFS, CC)");
+    }
+    // The arrays hold a restored state: take the source, make the engine ready.
+    virtual void restored(uint32_t) {}
+
+    struct StateHeader {
+        char magic[8];
+        uint32_t algo, nv;
+        uint64_t ne;
+        uint32_t source, narr, weight_mode, pad;
+        uint64_t weight_seed;
+        double pr_alpha, pr_epsilon;
+    };
+    static constexpr char kStateMagic[8] = {'S', 'G', 'S', 'T', 'A', 'T', 'E', '1'};
+
+    // File: StateHeader, narr x (u64 bytes, bytes of data), u64 row[nv + 1],
+    // u32 col[ne] (the snapshot's adjacency lists in PMA order).
+    void save_state(const char *path) {
+        if (!inited_) fail(SG_EINVAL, "no vertex state before sg_init");
+        converge();  // nothing deferred is left out of the saved state
+        std::vector<std::pair<void *, size_t>> arrs;
+        uint32_t src = 0;
+        state_arrays(arrs, src);
+        std::vector<uint64_t> off((size_t)nv_ + 1);
+        std::vector<uint32_t> deg(nv_), slots(cap_);
+        gather_graph(SG_Q_OFFSETS, off.data(), off.size() * 8);
+        gather_graph(SG_Q_DEGREE, deg.data(), deg.size() * 4);
+        gather_graph(SG_Q_SLOTS, slots.data(), slots.size() * 4);
+        std::vector<uint64_t> row((size_t)nv_ + 1, 0);
+        for (uint32_t v = 0; v < nv_; ++v) row[v + 1] = row[v] + deg[v];
+        if (row[nv_] != ne_) fail(SG_EDEVICE, "degree total differs from the edge count");
+        std::vector<uint32_t> col(std::max<uint64_t>(ne_, 1));
+        for (uint32_t v = 0; v < nv_; ++v)
+            if (deg[v]) std::memcpy(&col[row[v]], &slots[off[v]], (size_t)deg[v] * 4);
+        std::vector<std::vector<uint8_t>> host(arrs.size());
+        for (size_t i = 0; i < arrs.size(); ++i) {
+            host[i].resize(arrs[i].second);
+            SG_HIP(hipMemcpyAsync(host[i].data(), arrs[i].first, arrs[i].second, hipMemcpyDeviceToHost, stream_));
+        }
+        sync();
+        StateHeader h{};
+        std::memcpy(h.magic, kStateMagic, 8);
+        h.algo = cfg_.algo;
+        h.nv = nv_;
+        h.ne = ne_;
+        h.source = src;
+        h.narr = (uint32_t)arrs.size();
+        h.weight_mode = cfg_.weight_mode;
+        h.weight_seed = cfg_.weight_seed;
+        h.pr_alpha = cfg_.pr_alpha;
+        h.pr_epsilon = cfg_.pr_epsilon;
+        FILE *f = std::fopen(path, "wb");
+        if (!f) fail(SG_EIO, std::string("cannot open ") + path);
+        bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+        for (auto &x : host) {
+            const uint64_t n = x.size();
+            ok = ok && std::fwrite(&n, 8, 1, f) == 1 && (n == 0 || std::fwrite(x.data(), 1, n, f) == n);
+        }
+        ok = ok && std::fwrite(row.data(), 8, row.size(), f) == row.size() &&
+             (ne_ == 0 || std::fwrite(col.data(), 4, ne_, f) == ne_);
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok) fail(SG_EIO, std::string("cannot write ") + path);
+    }
+
+    void restore_state(const char *path) {
+        if (loaded_) fail(SG_EINVAL, "sg_restore_state needs a fresh engine (no sg_load_csr before it)");
+        FILE *f = std::fopen(path, "rb");
+        if (!f) fail(SG_EIO, std::string("cannot open ") + path);
+        std::unique_ptr<FILE, int (*)(FILE *)> fc(f, std::fclose);
+        StateHeader h{};
+        if (std::fread(&h, sizeof h, 1, f) != 1 || std::memcmp(h.magic, kStateMagic, 8) != 0)
+            fail(SG_EIO, "not a state checkpoint");
+        if (h.algo != cfg_.algo) fail(SG_EINVAL, "checkpoint of another program");
+        if ((h.algo == SG_ALGO_PR && (h.pr_alpha != cfg_.pr_alpha || h.pr_epsilon != cfg_.pr_epsilon)) ||
+            (h.algo == SG_ALGO_SSSP && (h.weight_mode != cfg_.weight_mode || h.weight_seed != cfg_.weight_seed)))
+            fail(SG_EINVAL, "engine configuration differs from the checkpoint's");
+        if (h.nv == 0 || h.narr > 8) fail(SG_EIO, "bad checkpoint header");
+        std::vector<std::vector<uint8_t>> host(h.narr);
+        for (auto &x : host) {
+            uint64_t n = 0;
+            if (std::fread(&n, 8, 1, f) != 1 || n > (uint64_t)h.nv * 16) fail(SG_EIO, "bad checkpoint state array");
+            x.resize(n);
+            if (n && std::fread(x.data(), 1, n, f) != n) fail(SG_EIO, "truncated checkpoint");
+        }
+        std::vector<uint64_t> row((size_t)h.nv + 1);
+        std::vector<uint32_t> col(std::max<uint64_t>(h.ne, 1));
+        if (std::fread(row.data(), 8, row.size(), f) != row.size() ||
+            (h.ne && std::fread(col.data(), 4, h.ne, f) != h.ne) || std::fgetc(f) != EOF)
+            fail(SG_EIO, "truncated or oversized checkpoint");
+        load_csr(h.nv, h.ne, row.data(), col.data());  // validates the offsets and ids
+        std::vector<std::pair<void *, size_t>> arrs;
+        uint32_t unused = 0;
+        state_arrays(arrs, unused);
+        if (arrs.size() != host.size()) fail(SG_EIO, "checkpoint state does not fit this program");
+        for (size_t i = 0; i < arrs.size(); ++i) {
+            if (arrs[i].second != host[i].size()) fail(SG_EIO, "checkpoint state does not fit this program");
+            SG_HIP(hipMemcpyAsync(arrs[i].first, host[i].data(), host[i].size(), hipMemcpyHostToDevice, stream_));
+        }
+        sync();
+        if (h.algo != SG_ALGO_CC && h.algo != SG_ALGO_PR && h.source >= h.nv) fail(SG_EIO, "bad checkpoint source");
+        restored(h.source);
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -1805,6 +1912,21 @@ public:
         VCombine<Op> vf{op_, value_, buffer_};
         ERelax<Op> ef{op_, buffer_, stamp_, stamp_val, cfg_.weight_mode, cfg_.weight_seed};
         return {vf, ef};
+    }
+
+    void state_arrays(std::vector<std::pair<void *, size_t>> &a, uint32_t &source) override {
+        if (!value_) {
+            value_ = dalloc<uint32_t>(nv_);
+            buffer_ = dalloc<unsigned long long>(nv_);
+        }
+        a = {{value_, (size_t)nv_ * 4}, {buffer_, (size_t)nv_ * 8}};
+        source = 0;
+        if constexpr (!std::is_same<Op, api::CC>::value) source = op_.m_source_node;
+    }
+    void restored(uint32_t source) override {
+        if constexpr (!std::is_same<Op, api::CC>::value) op_.m_source_node = source;
+        cache_rebuild();
+        inited_ = true;
     }
 
     void init(const uint32_t *sources, uint32_t nsrc) override {
@@ -1934,6 +2056,24 @@ public:
         std::memcpy(h, h_u64_, sizeof h);
         st_.pr_residual_l1 = h[0];
         st_.pr_residual_max = h[1];
+    }
+
+    void state_arrays(std::vector<std::pair<void *, size_t>> &a, uint32_t &source) override {
+        if (!value_) {
+            value_ = dalloc<float>(nv_);
+            buffer_ = dalloc<float>(nv_);
+            d_sum_ = dalloc<double>(2);
+        }
+        a = {{value_, (size_t)nv_ * 4}, {buffer_, (size_t)nv_ * 4}};
+        source = 0;
+    }
+    void restored(uint32_t) override {
+        op_.m_alpha = cfg_.pr_alpha;
+        op_.m_error = cfg_.pr_epsilon;
+        pending_ = false;
+        residual();
+        cache_rebuild();
+        inited_ = true;
     }
 
     void init(const uint32_t *, uint32_t) override {
@@ -2378,6 +2518,20 @@ sg_status sg_load_csr(sg_engine *eng, uint32_t nv, uint64_t ne, const uint64_t *
 
 sg_status sg_init(sg_engine *eng, const uint32_t *sources, uint32_t nsrc) {
     return guarded(eng, [&](sg::Engine &e) { e.init(sources, nsrc); });
+}
+
+sg_status sg_save_state(sg_engine *eng, const char *path) {
+    return guarded(eng, [&](sg::Engine &e) {
+        if (!path) sg::fail(SG_EINVAL, "null path");
+        e.save_state(path);
+    });
+}
+
+sg_status sg_restore_state(sg_engine *eng, const char *path) {
+    return guarded(eng, [&](sg::Engine &e) {
+        if (!path) sg::fail(SG_EINVAL, "null path");
+        e.restore_state(path);
+    });
 }
 
 sg_status sg_init_queries(sg_engine *eng, const uint32_t *sources, const uint32_t *kinds, uint32_t nq) {

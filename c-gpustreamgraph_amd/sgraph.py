@@ -132,6 +132,8 @@ SIGNATURES = [
     ("sg_read_stream_sizes", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(_P)]),
     ("sg_read_updates", C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(_P), C.POINTER(C.c_uint64), C.POINTER(_P)]),
     ("sg_write_updates", C.c_int, [C.c_char_p, C.c_char_p, C.c_uint32, _P, _P, _P]),
+    ("sg_save_state", C.c_int, [_P, C.c_char_p]),
+    ("sg_restore_state", C.c_int, [_P, C.c_char_p]),
     ("sg_write_snapshot_file", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint64, _P, _P, C.c_uint32, _P, _P, _P]),
     ("sg_read_snapshot_file", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(_P),
                                         C.POINTER(_P), C.POINTER(C.c_uint32), C.POINTER(_P), C.POINTER(_P),
@@ -311,6 +313,15 @@ class Engine:
         else:
             s = np.array([source], dtype=np.uint32)
             _check(lib().sg_init(self.h, _ptr(s), 1), self.h)
+
+    def save_state(self, path):
+        """Checkpoint: graph, vertex state and source to a file."""
+        _check(lib().sg_save_state(self.h, path.encode()), self.h)
+
+    def restore_state(self, path):
+        """Load a checkpoint into this fresh engine (instead of load_csr + init)."""
+        _check(lib().sg_restore_state(self.h, path.encode()), self.h)
+        self.nv = int(self.stats()["nv"])
 
     def init_queries(self, sources, kinds):
         """Batched queries (engine created with algo "multi"): kinds are "sssp" / "bfs"."""

@@ -178,6 +178,17 @@ sg_status sg_load_csr(sg_engine *eng, uint32_t nv, uint64_t ne,
  * Replaces Engine::InitGraph + Engine::Start (framework.cuh:389-439, 549-584). */
 sg_status sg_init(sg_engine *eng, const uint32_t *sources, uint32_t nsrc);
 
+/* Checkpoint of a single-program engine (PR, SSSP, BFS, CC): sg_save_state
+ * writes the current snapshot's adjacency lists (in PMA order), the vertex
+ * state arrays and the traversal source to path (deferred PageRank snapshots
+ * are converged first); sg_restore_state loads such a file into a fresh
+ * engine of the same program and configuration (no sg_load_csr / sg_init
+ * before it) without a static solve. Later snapshots then give the results
+ * the saved engine would have given. No counterpart in the reference, whose
+ * state lives only in the running process (GraphDatum, graph_datum.cuh:33-357). */
+sg_status sg_save_state(sg_engine *eng, const char *path);
+sg_status sg_restore_state(sg_engine *eng, const char *path);
+
 /* SG_ALGO_MULTI engines: start nq queries on the loaded graph, query i from
  * sources[i], of kind kinds[i] (SG_ALGO_SSSP: weights of cfg.weight_mode;
  * SG_ALGO_BFS: unit weights, i.e. levels), and run them to convergence

@@ -168,6 +168,58 @@ def test_kernel_timing_classes(timing):
         e.close()
 
 
+@pytest.mark.parametrize("algo", ALGOS)
+def test_checkpoint_restore_continues_the_stream(tmp_path, algo):
+    # a restored engine holds the saved graph and state and gives the saved
+    # engine's results on the later snapshots
+    nv, row, col = sgraph.gen_rmat(11, 8, seed=71)
+    ins, dele = sgraph.gen_updates(row, col, 11, 4, 300, 300, seed=72)
+    kw = dict(weight_mode="seeded", weight_seed=5)
+    src = 0 if algo in ("bfs", "sssp") else None
+    a = sgraph.Engine(algo, **kw)
+    b = sgraph.Engine(algo, **kw)
+    try:
+        a.load_csr(row, col)
+        a.init(src)
+        r, c = row, col
+        for t in range(2):
+            a.apply(ins[t], dele[t])
+            r, c = O.apply_updates(r, c, ins[t], dele[t])
+        path = str(tmp_path / "state.bin")
+        a.save_state(path)
+        b.restore_state(path)
+        assert b.nv == nv and np.array_equal(b.values(), a.values())
+        check_pma(b, r, c, f"{algo} restored")
+        for t in (2, 3):
+            a.apply(ins[t], dele[t])
+            b.apply(ins[t], dele[t])
+            r, c = O.apply_updates(r, c, ins[t], dele[t])
+            check_values(algo, b, r, c, 0, "seeded", 5, f"{algo} restored G{t + 1}")
+            if algo != "pr":
+                assert np.array_equal(b.values(), a.values())
+                if algo != "cc":
+                    assert np.array_equal(b.gather(sgraph.Q_PARENT), a.gather(sgraph.Q_PARENT))
+    finally:
+        a.close()
+        b.close()
+    # the checkpoint only loads into a fresh engine of the same program (and,
+    # for SSSP, the same weights)
+    wrong = sgraph.Engine("bfs" if algo != "bfs" else "sssp", **kw)
+    with pytest.raises(sgraph.SgError):
+        wrong.restore_state(path)
+    wrong.close()
+    if algo == "sssp":
+        w2 = sgraph.Engine("sssp", weight_mode="seeded", weight_seed=6)
+        with pytest.raises(sgraph.SgError):
+            w2.restore_state(path)
+        w2.close()
+    used = sgraph.Engine(algo, **kw)
+    used.load_csr(row, col)
+    with pytest.raises(sgraph.SgError):
+        used.restore_state(path)
+    used.close()
+
+
 @pytest.mark.parametrize("algo", ["sssp", "bfs", "cc"])
 def test_rmat_stream_larger(algo):
     nv, row, col = sgraph.gen_rmat(15, 16, seed=21)
