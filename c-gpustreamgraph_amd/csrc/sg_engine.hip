@@ -2474,6 +2474,147 @@ void sg_config_default(sg_config *c) {
     c->edge_budget_gb = 0;
 }
 
+namespace sg {
+// ---- seeded R-MAT on the device (sg_gen_rmat_device) ---------------------------
+// Draw i is sg_rmat_draw of the splitmix stream (seed, i), as on the host;
+// self-loops become ~0 and sort past every edge.
+__global__ void k_rmat_keys(uint64_t ndraw, uint32_t scale, uint64_t seed, SgRmatThresh th, const uint32_t *perm,
+                            unsigned long long *keys) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ndraw; i += stride) {
+        SgRng rng(seed, i);
+        uint32_t s, d;
+        sg_rmat_draw(rng, scale, th, s, d);
+        if (perm) {
+            s = perm[s];
+            d = perm[d];
+        }
+        keys[i] = s == d ? ~0ull : (((unsigned long long)s << 32) | d);
+    }
+}
+
+// keep[i]: first copy of a key that is an edge.
+__global__ void k_rmat_keep(uint64_t n, const unsigned long long *keys, uint8_t *keep) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        keep[i] = keys[i] != ~0ull && (i == 0 || keys[i] != keys[i - 1]);
+}
+
+// row[v] = first edge of source v (lower bound of v << 32 in the sorted keys),
+// col[i] = target of edge i.
+__global__ void k_rmat_csr(uint32_t nv, uint64_t ne, const unsigned long long *keys, uint64_t *row, uint32_t *col) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) col[i] = (uint32_t)keys[i];
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= nv; v += stride) {
+        const unsigned long long key = v << 32;
+        uint64_t lo = 0, hi = ne;
+        while (lo < hi) {
+            const uint64_t m = (lo + hi) >> 1;
+            if (keys[m] < key) lo = m + 1;
+            else hi = m;
+        }
+        row[v] = lo;
+    }
+}
+}  // namespace sg
+
+sg_status sg_gen_rmat_device(int32_t device, uint32_t scale, uint32_t edge_factor, uint64_t seed, double a, double b,
+                             double c, int permute, uint32_t *nv_out, uint64_t *ne_out, uint64_t **row_out,
+                             uint32_t **col_out) {
+    if (!nv_out || !ne_out || !row_out || !col_out) return SG_EINVAL;
+    if (scale < 1 || scale > 31 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return SG_EINVAL;
+    const uint32_t nv = 1u << scale;
+    const uint64_t ndraw = (uint64_t)edge_factor * nv;
+    SgRmatThresh th;
+    auto q = [](double x) { return (uint64_t)std::llround(std::min(1.0, std::max(0.0, x)) * 4294967296.0); };
+    th.a = q(a);
+    th.ab = q(a + b);
+    th.abc = q(a + b + c);
+    std::vector<uint32_t> perm;
+    if (permute) {  // the host generator's seeded Fisher-Yates permutation
+        perm.resize(nv);
+        for (uint32_t i = 0; i < nv; ++i) perm[i] = i;
+        SgRng rng(seed ^ 0x5045524d55544531ull, 0);
+        for (uint32_t i = nv - 1; i > 0; --i) std::swap(perm[i], perm[(uint32_t)(rng.next() % ((uint64_t)i + 1))]);
+    }
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) {
+        sg::g_create_error = "no usable device " + std::to_string(device);
+        return SG_EDEVICE;
+    }
+    hipStream_t st = nullptr;
+    void *d_keys = nullptr, *d_sorted = nullptr, *d_keep = nullptr, *d_perm = nullptr, *d_tmp = nullptr, *d_cnt = nullptr;
+    void *d_row = nullptr, *d_col = nullptr;
+    uint64_t *row = nullptr;
+    uint32_t *col = nullptr;
+    sg_status status = SG_OK;
+    try {
+        SG_HIP(hipStreamCreate(&st));
+        SG_HIP(hipMalloc(&d_keys, std::max<uint64_t>(ndraw, 1) * 8));
+        SG_HIP(hipMalloc(&d_sorted, std::max<uint64_t>(ndraw, 1) * 8));
+        SG_HIP(hipMalloc(&d_keep, std::max<uint64_t>(ndraw, 1)));
+        SG_HIP(hipMalloc(&d_cnt, 8));
+        if (permute) {
+            SG_HIP(hipMalloc(&d_perm, (size_t)nv * 4));
+            SG_HIP(hipMemcpyAsync(d_perm, perm.data(), (size_t)nv * 4, hipMemcpyHostToDevice, st));
+        }
+        const uint32_t grid = 4096;
+        hipLaunchKernelGGL(sg::k_rmat_keys, dim3(grid), dim3(256), 0, st, ndraw, scale, seed, th, (const uint32_t *)d_perm,
+                           (unsigned long long *)d_keys);
+        SG_LAUNCHED();
+        size_t b1 = 0, b2 = 0;
+        SG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, b1, (const unsigned long long *)d_keys,
+                                                 (unsigned long long *)d_sorted, ndraw, 0, 64, st));
+        SG_HIP(hipcub::DeviceSelect::Flagged(nullptr, b2, (const unsigned long long *)d_sorted, (const uint8_t *)d_keep,
+                                             (unsigned long long *)d_keys, (unsigned long long *)d_cnt, (int64_t)ndraw, st));
+        size_t tb = std::max(b1, b2) + 256;
+        SG_HIP(hipMalloc(&d_tmp, tb));
+        size_t bb = tb;
+        SG_HIP(hipcub::DeviceRadixSort::SortKeys(d_tmp, bb, (const unsigned long long *)d_keys,
+                                                 (unsigned long long *)d_sorted, ndraw, 0, 64, st));
+        hipLaunchKernelGGL(sg::k_rmat_keep, dim3(grid), dim3(256), 0, st, ndraw, (const unsigned long long *)d_sorted,
+                           (uint8_t *)d_keep);
+        SG_LAUNCHED();
+        bb = tb;
+        SG_HIP(hipcub::DeviceSelect::Flagged(d_tmp, bb, (const unsigned long long *)d_sorted, (const uint8_t *)d_keep,
+                                             (unsigned long long *)d_keys, (unsigned long long *)d_cnt, (int64_t)ndraw, st));
+        unsigned long long ne = 0;
+        SG_HIP(hipMemcpyAsync(&ne, d_cnt, 8, hipMemcpyDeviceToHost, st));
+        SG_HIP(hipStreamSynchronize(st));
+        if (ne > ndraw) throw sg::Error{SG_EDEVICE, "edge count beyond the draws"};
+        SG_HIP(hipMalloc(&d_row, ((size_t)nv + 1) * 8));
+        SG_HIP(hipMalloc(&d_col, std::max<uint64_t>(ne, 1) * 4));
+        hipLaunchKernelGGL(sg::k_rmat_csr, dim3(grid), dim3(256), 0, st, nv, (uint64_t)ne, (const unsigned long long *)d_keys,
+                           (uint64_t *)d_row, (uint32_t *)d_col);
+        SG_LAUNCHED();
+        row = (uint64_t *)std::malloc(((size_t)nv + 1) * 8);
+        col = (uint32_t *)std::malloc(std::max<uint64_t>(ne, 1) * 4);
+        if (!row || !col) throw std::bad_alloc();
+        SG_HIP(hipMemcpyAsync(row, d_row, ((size_t)nv + 1) * 8, hipMemcpyDeviceToHost, st));
+        SG_HIP(hipMemcpyAsync(col, d_col, (size_t)ne * 4, hipMemcpyDeviceToHost, st));
+        SG_HIP(hipStreamSynchronize(st));
+        *nv_out = nv;
+        *ne_out = ne;
+        *row_out = row;
+        *col_out = col;
+        row = nullptr;
+        col = nullptr;
+    } catch (const sg::Error &e) {
+        sg::g_create_error = e.msg;
+        status = e.status;
+    } catch (const std::bad_alloc &) {
+        sg::g_create_error = "host allocation failed";
+        status = SG_ENOMEM;
+    }
+    std::free(row);
+    std::free(col);
+    for (void *p : {d_keys, d_sorted, d_keep, d_perm, d_tmp, d_cnt, d_row, d_col})
+        if (p) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipSetDevice(prev);
+    return status;
+}
+
 sg_status sg_create(const sg_config *cfg, sg_engine **out) {
     if (!cfg || !out) { sg::g_create_error = "null argument"; return SG_EINVAL; }
     *out = nullptr;

@@ -119,31 +119,16 @@ void parallel_sort_keys(std::vector<uint64_t> &keys, uint32_t nv_bits) {
     keys.swap(out);
 }
 
-// One R-MAT draw of `scale` levels; 32 random bits per level.
-struct RmatThresh {
-    uint64_t ab, a, abc;  // thresholds on a 32-bit uniform
+// R-MAT thresholds on a 32-bit uniform (the draw itself is sg_rmat_draw).
+struct RmatThresh : SgRmatThresh {
     RmatThresh(double A, double B, double C) {
         auto q = [](double x) { return (uint64_t)std::llround(std::min(1.0, std::max(0.0, x)) * 4294967296.0); };
         a = q(A); ab = q(A + B); abc = q(A + B + C);
     }
 };
 
-template <class Rng>
-inline void rmat_draw(Rng &rng, uint32_t scale, const RmatThresh &t, uint32_t &src, uint32_t &dst) {
-    uint32_t s = 0, d = 0;
-    uint64_t bits = 0;
-    for (uint32_t lvl = 0; lvl < scale; ++lvl) {
-        if ((lvl & 1) == 0) bits = rng.next();
-        uint64_t r = (lvl & 1) ? (bits >> 32) : (bits & 0xFFFFFFFFull);
-        s <<= 1; d <<= 1;
-        if (r < t.ab) {
-            if (r >= t.a) d |= 1;
-        } else {
-            s |= 1;
-            if (r >= t.abc) d |= 1;
-        }
-    }
-    src = s; dst = d;
+inline void rmat_draw(SgRng &rng, uint32_t scale, const RmatThresh &t, uint32_t &src, uint32_t &dst) {
+    sg_rmat_draw(rng, scale, t, src, dst);
 }
 
 // Open-addressing set/map of 64-bit keys -> 32-bit values (value UINT32_MAX = empty slot marker not used).

@@ -34,6 +34,31 @@ struct SgRng {
     }
 };
 
+// One R-MAT draw of `scale` levels; 32 random bits per level compared with
+// the thresholds a, a + b, a + b + c scaled to 2^32 (rounded on the host).
+struct SgRmatThresh {
+    uint64_t ab, a, abc;
+};
+
+SG_HD void sg_rmat_draw(SgRng &rng, uint32_t scale, const SgRmatThresh &t, uint32_t &src, uint32_t &dst) {
+    uint32_t s = 0, d = 0;
+    uint64_t bits = 0;
+    for (uint32_t lvl = 0; lvl < scale; ++lvl) {
+        if ((lvl & 1) == 0) bits = rng.next();
+        const uint64_t r = (lvl & 1) ? (bits >> 32) : (bits & 0xFFFFFFFFull);
+        s <<= 1;
+        d <<= 1;
+        if (r < t.ab) {
+            if (r >= t.a) d |= 1;
+        } else {
+            s |= 1;
+            if (r >= t.abc) d |= 1;
+        }
+    }
+    src = s;
+    dst = d;
+}
+
 // Weight modes (sgraph.h sg_weight_mode).
 //   0: (u + v) % 128 + 1  — the reference's device weight, push_functor.h:306
 //   1: 1 + splitmix64(seed ^ (u << 32 | v)) % 63  — seeded, in [1, 63]

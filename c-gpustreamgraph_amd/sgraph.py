@@ -132,6 +132,8 @@ SIGNATURES = [
     ("sg_read_stream_sizes", C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(_P)]),
     ("sg_read_updates", C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(_P), C.POINTER(C.c_uint64), C.POINTER(_P)]),
     ("sg_write_updates", C.c_int, [C.c_char_p, C.c_char_p, C.c_uint32, _P, _P, _P]),
+    ("sg_gen_rmat_device", C.c_int, [C.c_int32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(_P), C.POINTER(_P)]),
     ("sg_save_state", C.c_int, [_P, C.c_char_p]),
     ("sg_restore_state", C.c_int, [_P, C.c_char_p]),
     ("sg_write_snapshot_file", C.c_int, [C.c_char_p, C.c_uint32, C.c_uint64, _P, _P, C.c_uint32, _P, _P, _P]),
@@ -237,6 +239,14 @@ def read_snapshot_file(path):
     sizes = _take(psz, 2 * ns.value, np.uint64).reshape(-1, 2)
     ti, td = int(sizes[:, 0].sum()), int(sizes[:, 1].sum())
     return row, col, sizes, _take(pi, 2 * ti, np.uint32).reshape(-1, 2), _take(pd, 2 * td, np.uint32).reshape(-1, 2)
+
+
+def gen_rmat_device(scale, edge_factor, seed=DEFAULT_SEED, a=0.57, b=0.19, c=0.19, permute=False, device=0):
+    """gen_rmat's graph, generated on a GPU: (nv, row, col)."""
+    nv, ne, row, col = C.c_uint32(), C.c_uint64(), C.c_void_p(), C.c_void_p()
+    _check(lib().sg_gen_rmat_device(device, scale, edge_factor, seed, a, b, c, int(permute), C.byref(nv), C.byref(ne),
+                                    C.byref(row), C.byref(col)))
+    return nv.value, _take(row, nv.value + 1, np.uint64), _take(col, ne.value, np.uint32)
 
 
 def gen_rmat(scale, edge_factor, seed=DEFAULT_SEED, a=0.57, b=0.19, c=0.19, permute=False):

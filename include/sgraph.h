@@ -178,6 +178,15 @@ sg_status sg_load_csr(sg_engine *eng, uint32_t nv, uint64_t ne,
  * Replaces Engine::InitGraph + Engine::Start (framework.cuh:389-439, 549-584). */
 sg_status sg_init(sg_engine *eng, const uint32_t *sources, uint32_t nsrc);
 
+/* The seeded R-MAT graph of sg_gen_rmat (sgraph_io.h), the same vertices and
+ * edges, drawn, sorted and de-duplicated on device `device` (the host
+ * generator takes ≈110 s at scale 27 EF32). Arrays as sg_gen_rmat's (malloc,
+ * sg_free). Needs about 17 bytes of device memory per draw (edge_factor <<
+ * scale draws). No counterpart in the reference, which reads text files. */
+sg_status sg_gen_rmat_device(int32_t device, uint32_t scale, uint32_t edge_factor, uint64_t seed, double a,
+                             double b, double c, int permute, uint32_t *nv, uint64_t *ne, uint64_t **row,
+                             uint32_t **col);
+
 /* Checkpoint of a single-program engine (PR, SSSP, BFS, CC): sg_save_state
  * writes the current snapshot's adjacency lists (in PMA order), the vertex
  * state arrays and the traversal source to path (deferred PageRank snapshots

@@ -91,3 +91,9 @@ def test_server_create_failure_reports_its_reason():
     assert st == sgraph.SG_EDEVICE and not h.value
     msg = sgraph.lib().sg_server_last_error(None)
     assert msg and b"device" in msg.lower()
+
+
+def test_device_generator_without_device_fails_cleanly():
+    with pytest.raises(sgraph.SgError) as e:
+        sgraph.gen_rmat_device(6, 4, device=99)
+    assert e.value.status == sgraph.SG_EDEVICE
