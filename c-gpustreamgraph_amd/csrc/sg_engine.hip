@@ -1034,7 +1034,10 @@ public:
         return g;
     }
     uint32_t budget() const { return cfg_.wave_budget ? cfg_.wave_budget : kWaveBudget; }
-    uint32_t round_blocks() const { return cfg_.round_blocks ? std::min<uint32_t>(cfg_.round_blocks, 65535) : kGridList; }
+    // round kernels: 1024 blocks (4096 waves); small frontiers spread over
+    // them, and fewer blocks than kGridList shorten each round's launch and drain
+    static constexpr uint32_t kRoundBlocks = 1024;
+    uint32_t round_blocks() const { return cfg_.round_blocks ? std::min<uint32_t>(cfg_.round_blocks, 65535) : kRoundBlocks; }
 
     // dpre_[v] = sum of deg[0..v) for v in [0, nv] (64-bit) and the merge-path
     // tiles of the edge-centric sweep, in three kernels.

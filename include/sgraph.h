@@ -77,7 +77,7 @@ typedef struct sg_config {
     double edge_budget_gb;    /* device budget of the edge array in GiB
                                  (-device_graph_budget_gb): an edge array larger than
                                  this lives in pinned host memory (0 = no cap)       */
-    uint32_t round_blocks;    /* blocks of a round kernel launch (0 = 2048)             */
+    uint32_t round_blocks;    /* blocks of a round kernel launch (0 = 1024)             */
     uint32_t reserved[1];
 } sg_config;
 
