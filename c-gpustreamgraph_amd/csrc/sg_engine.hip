@@ -833,6 +833,10 @@ public:
     double kt_ms_[SG_KCLASS_COUNT] = {};
 
     static constexpr uint32_t kGridList = 2048;   // light-kernel blocks for device-sized lists
+    // the chunk filter: one wave per chunk of a vertex with deletes, a latency
+    // chain each; 8192 blocks (about one chunk per wave at 100k deletes) beat
+    // 2048 by 20 us per batch at scale 22
+    static constexpr uint32_t kGridFilter = 8192;
     static constexpr uint32_t kGridHeavy = 1024;
 
     void init_device() {
@@ -1465,7 +1469,7 @@ public:
                 ensure(tmp_, tmp_cap_, std::max<uint64_t>(h_ctl_->seqcur, 1));
             }
             begin_timed(timing(SG_KCLASS_MERGE), SG_KCLASS_MERGE);
-            hipLaunchKernelGGL(k_chunk_filter, dim3(kGridList), dim3(kBlock), 0, stream_, gview(), bv_,
+            hipLaunchKernelGGL(k_chunk_filter, dim3(kGridFilter), dim3(kBlock), 0, stream_, gview(), bv_,
                                (const MChunk *)chunks_, &ctl_->nchunks, (uint32_t)chunk_bound, ccount_, tmp_, col_);
             SG_LAUNCHED();
             end_timed(timing(SG_KCLASS_MERGE));
