@@ -41,6 +41,10 @@
 #include "sg_operators.h"
 #include "sg_pma.h"
 #include "sg_multi.h"
+#include "sg_pull.h"
+
+static_assert(sizeof(sg_config) == 104, "sg_config layout");
+static_assert(sizeof(sg_stats) == 224, "sg_stats layout");
 
 namespace sg {
 
@@ -221,20 +225,22 @@ __global__ void k_cache_drop(const TInfo *tinfo, const uint32_t *tcount, uint64_
         coff[tinfo[t].u] = ~0ull;
 }
 
-// Split (u, v) pairs into sort keys; drop ids >= nv.
+// Split (u, v) pairs into sort keys; drop ids >= nv. swap = 1 reads every
+// pair as (v, u): the batch of the reverse (in-edge) store.
 __global__ void k_split(uint32_t nv, const uint32_t *ins_uv, uint32_t nins, const uint32_t *del_uv, uint32_t ndel,
-                        uint32_t *ins_k, uint32_t *ins_v, unsigned long long *del_k, Ctl *ctl) {
+                        uint32_t *ins_k, uint32_t *ins_v, unsigned long long *del_k, Ctl *ctl, int swap) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const int a = swap ? 1 : 0, b = swap ? 0 : 1;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)nins + ndel; i += stride) {
         if (i < nins) {
-            const uint32_t u = ins_uv[2 * i], v = ins_uv[2 * i + 1];
+            const uint32_t u = ins_uv[2 * i + a], v = ins_uv[2 * i + b];
             const bool ok = u < nv && v < nv;
             ins_k[i] = ok ? u : kGap;
             ins_v[i] = v;
             if (!ok) atomicAdd(&ctl->dropped, 1ull);
         } else {
             const uint64_t j = i - nins;
-            const uint32_t u = del_uv[2 * j], v = del_uv[2 * j + 1];
+            const uint32_t u = del_uv[2 * j + a], v = del_uv[2 * j + b];
             const bool ok = u < nv && v < nv;
             del_k[j] = ok ? (((unsigned long long)u << 32) | v) : ~0ull;
             if (!ok) atomicAdd(&ctl->dropped, 1ull);
@@ -617,18 +623,26 @@ public:
     sg_config cfg_;
     std::string last_error_ = "no error";
     hipStream_t stream_ = nullptr;
+    bool own_stream_ = true;
+    bool swap_pairs_ = false;  // batches are read as (v, u): the in-edge store of another engine
     sg_stats st_{};
 
     // --- memory -------------------------------------------------------------
     uint64_t dev_bytes_ = 0;
     std::vector<std::pair<void *, size_t>> allocs_;
 
+    // An engine and the stores it owns share one device budget: each side
+    // counts the other's bytes.
+    const Engine *budget_peer_ = nullptr;
+    uint64_t budget_used() const {
+        return dev_bytes_ + aux_device_bytes() + (budget_peer_ ? budget_peer_->dev_bytes_ : 0);
+    }
     template <class T>
     T *dalloc(size_t count) {
         const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
         const double budget = cfg_.device_budget_gb * 1073741824.0;
-        if (budget > 0 && (double)(dev_bytes_ + bytes) > budget)
-            fail(SG_EBUDGET, "device budget exceeded: need " + std::to_string(dev_bytes_ + bytes) + " bytes");
+        if (budget > 0 && (double)(budget_used() + bytes) > budget)
+            fail(SG_EBUDGET, "device budget exceeded: need " + std::to_string(budget_used() + bytes) + " bytes");
         void *p = nullptr;
         hipError_t e = hipMalloc(&p, bytes);
         if (e != hipSuccess) {
@@ -705,7 +719,8 @@ public:
         for (auto &a : hallocs_) (void)hipHostFree(a.first);
         hallocs_.clear();
         host_bytes_ = 0;
-        for (auto &ev : seg_ev_) if (ev) (void)hipEventDestroy(ev);
+        for (auto &ev : seg_copied_) if (ev) (void)hipEventDestroy(ev);
+        for (auto &ev : seg_used_) if (ev) (void)hipEventDestroy(ev);
         if (sync_ev_) (void)hipEventDestroy(sync_ev_);
         sync_ev_ = nullptr;
         if (cstream_) (void)hipStreamDestroy(cstream_);
@@ -715,7 +730,7 @@ public:
         events_.clear();
         if (pinned_) (void)hipHostFree(pinned_);
         pinned_ = nullptr;
-        if (stream_) (void)hipStreamDestroy(stream_);
+        if (stream_ && own_stream_) (void)hipStreamDestroy(stream_);
         stream_ = nullptr;
     }
 
@@ -738,16 +753,19 @@ public:
     //   u32 [0..2] frontier counters, [3..5] spill counters, [7] always zero,
     //   u32 [8, 8 + 2 * kRoundLog) round log, then u64 stats[16]:
     //   [0] edges (light), [1] vertices, [2] edges (spilled chunks), [3] overflow
-    //   flags, [4] invalidated vertices, [8..11] the same four for timed rounds.
-    static constexpr size_t kCtlBlockBytes = (8 + 2 * kRoundLog) * 4 + 16 * 8;
+    //   flags, [4] pull rounds, [5] edges read from the HBM cache, [6..7] cache
+    //   rebuild counts, [8..15] the same as [0..7] for timed rounds.
+    //   Then u64 ecnt[4]: out-edges of the frontiers of rounds R % 3 (direction choice).
+    static constexpr size_t kCtlBlockBytes = (8 + 2 * kRoundLog) * 4 + 16 * 8 + 4 * 8;
     uint32_t *cnt_ = nullptr;    // [3] rotating round counters
     uint32_t *hcnt_ = nullptr;   // [3] rotating spill counters
     uint32_t *rlog_ = nullptr;
-    uint64_t timed_edges_ = 0, timed_verts_ = 0, timed_cedges_ = 0;  // host totals of the timed-round stats
+    uint64_t timed_edges_ = 0, timed_verts_ = 0, timed_cedges_ = 0, timed_pulls_ = 0;  // host totals of the timed-round stats
     uint64_t last_appended_ = 0;  // vertices the last run of rounds (and its seeds) put on frontiers
     HeavyDesc *heavy_[3] = {nullptr, nullptr, nullptr};  // rotating spill queues
     size_t heavy_cap_ = 0;
     unsigned long long *stats_ = nullptr;  // [0..3] pass stats, [4] inval count
+    unsigned long long *ecnt_ = nullptr;   // [3] rotating frontier out-edge counters
     uint32_t *stamp_ = nullptr;
     uint32_t *inval_ = nullptr;   // bitmap of invalidated vertices
     uint32_t *coarse_ = nullptr;  // coarse filter of inval_: one bit per 2^cshift_ vertices
@@ -804,10 +822,12 @@ public:
     std::vector<unsigned long long> h_seg_slot_, h_seg_path_;
     bool seg_dirty_ = true;
     hipStream_t cstream_ = nullptr;           // copy stream of the streamed segments
-    hipEvent_t seg_ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // copied[2], used[2], start
-    hipEvent_t sync_ev_ = nullptr;                                            // polled by sync()
-    uint32_t *stage_[2] = {nullptr, nullptr};
-    size_t stage_cap_[2] = {0, 0};
+    static constexpr int kMaxStage = 8;       // staging buffers (n_stream segment copies in flight)
+    hipEvent_t seg_copied_[kMaxStage] = {}, seg_used_[kMaxStage] = {};
+    hipEvent_t sync_ev_ = nullptr;            // polled by sync()
+    uint32_t *stage_[kMaxStage] = {};
+    size_t stage_cap_[kMaxStage] = {};
+    int nstage() const { return (int)std::max<uint32_t>(2, std::min<uint32_t>(cfg_.n_stream, kMaxStage)); }
     // HBM edge cache (host edges only)
     uint64_t cache_slots_ = 0;
     uint32_t *ccol_[2] = {nullptr, nullptr};
@@ -822,6 +842,7 @@ public:
     uint32_t *h_blk_ = nullptr;   // host copy of the control block
     unsigned long long *h_u64_ = nullptr;  // [16]
     const unsigned long long *h_stats() const { return (const unsigned long long *)(h_blk_ + 8 + 2 * kRoundLog); }
+    const unsigned long long *h_ecnt() const { return h_stats() + 16; }
 
     // --- kernel timing --------------------------------------------------------
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events_;
@@ -839,7 +860,9 @@ public:
     static constexpr uint32_t kGridFilter = 8192;
     static constexpr uint32_t kGridHeavy = 1024;
 
-    void init_device() {
+    // shared: run on another engine's stream (a store owned by that engine,
+    // whose work must be ordered with the owner's kernels); null = own stream.
+    void init_device(hipStream_t shared = nullptr) {
         int ndev = 0;
         SG_HIP(hipGetDeviceCount(&ndev));
         if (cfg_.device < 0 || cfg_.device >= ndev) fail(SG_EDEVICE, "no HIP device " + std::to_string(cfg_.device));
@@ -848,7 +871,12 @@ public:
         SG_HIP(hipGetDeviceProperties(&prop, cfg_.device));
         if (prop.warpSize != 64) fail(SG_EDEVICE, "wave64 device required");
         num_cus_ = prop.multiProcessorCount;
-        SG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        if (shared) {
+            stream_ = shared;
+            own_stream_ = false;
+        } else {
+            SG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        }
         SG_HIP(hipEventCreateWithFlags(&sync_ev_, hipEventDisableTiming));
         SG_HIP(hipHostMalloc(&pinned_, 65536, hipHostMallocDefault));
         h_ctl_ = (Ctl *)pinned_;
@@ -917,7 +945,10 @@ public:
             // no larger than twice the edge array: the rest could never be filled
             cache_slots_ = std::min<uint64_t>(kCacheSlots[std::min<uint32_t>(cfg_.cache_level, 4)], 2 * cap_);
             SG_HIP(hipStreamCreateWithFlags(&cstream_, hipStreamNonBlocking));
-            for (auto &ev : seg_ev_) SG_HIP(hipEventCreate(&ev));
+            for (int b = 0; b < kMaxStage; ++b) {
+                SG_HIP(hipEventCreateWithFlags(&seg_copied_[b], hipEventDisableTiming));
+                SG_HIP(hipEventCreateWithFlags(&seg_used_[b], hipEventDisableTiming));
+            }
         }
         if (cache_slots_) {
             ccol_[0] = dalloc<uint32_t>(cache_slots_);
@@ -941,6 +972,7 @@ public:
         hcnt_ = cnt_ + 3;
         rlog_ = cnt_ + 8;
         stats_ = (unsigned long long *)(cnt_ + 8 + 2 * kRoundLog);
+        ecnt_ = stats_ + 16;
         stamp_ = dalloc<uint32_t>(nv);
         inval_ = dalloc<uint32_t>(((size_t)nv + 31) / 32);
         cshift_ = coarse_shift_for(nv);
@@ -1008,7 +1040,15 @@ public:
         st_.capacity = cap_;
         st_.host_edges = host_edges_ ? 1 : 0;
         st_.cache_slots = cache_slots_;
+        after_load(row, col);
     }
+    // Stores an engine keeps beside its PMA (PageRank's in-edge store) are
+    // built here from the loaded CSR.
+    virtual void after_load(const uint64_t *, const uint32_t *) {}
+    // Device and pinned host memory of those stores.
+    virtual uint64_t aux_device_bytes() const { return 0; }
+    virtual uint64_t aux_host_bytes() const { return 0; }
+    virtual uint64_t in_capacity() const { return 0; }
 
     void ensure_heavy(uint64_t edges) {
         // every list past a wave's budget becomes >= 1 descriptor: at most one per
@@ -1093,7 +1133,10 @@ public:
     // and spills into hq[R % 3]; it clears cnt[(R + 2) % 3] and hc[(R + 1) % 3],
     // whose readers have finished. Seed passes before round R append to
     // round R's frontier and spill into hq[(R + 2) % 3].
-    void frontier_reset() { fill<uint32_t>(cnt_, 8, 0); }  // word 7 stays zero: an empty list count
+    void frontier_reset() {  // word 7 stays zero: an empty list count
+        fill<uint32_t>(cnt_, 8, 0);
+        fill<unsigned long long>(ecnt_, 4, 0ull);
+    }
 
     PassIO base_io() const {
         PassIO io{};
@@ -1169,6 +1212,7 @@ public:
     struct RoundGraph {
         hipGraph_t g = nullptr;
         hipGraphExec_t exec = nullptr;
+        std::vector<const void *> funcs;  // kernel of each node
         std::vector<hipGraphNode_t> nodes;
         std::vector<std::shared_ptr<void>> keep;  // argument values of the last launch, kept past it
     };
@@ -1199,9 +1243,19 @@ public:
     void run_graph_batch(std::vector<GraphLaunch> &b) {
         if (b.empty()) return;
         auto key = std::make_pair(b[0].func, b.size());
-        for (auto &l : b)
-            if (l.func != key.first) fail(SG_EDEVICE, "internal: mixed kernels in one round graph");
         RoundGraph &rg = graphs_[key];
+        bool same = rg.exec != nullptr;
+        for (size_t i = 0; same && i < b.size(); ++i) same = rg.funcs[i] == b[i].func;
+        if (rg.exec && !same) {  // another kernel sequence of the same length: rebuild
+            SG_HIP(hipStreamSynchronize(stream_));
+            (void)hipGraphExecDestroy(rg.exec);
+            (void)hipGraphDestroy(rg.g);
+            rg = RoundGraph{};
+        }
+        if (!rg.exec) {
+            rg.funcs.clear();
+            for (auto &l : b) rg.funcs.push_back(l.func);
+        }
         auto params = [&](GraphLaunch &l) {
             hipKernelNodeParams kp{};
             kp.func = const_cast<void *>(l.func);
@@ -1385,6 +1439,8 @@ public:
             }
         }
         if (timed) timed_cedges_ += s[13];
+        if (timed) timed_pulls_ += s[12];
+        st_.pull_rounds = s[4] + timed_pulls_;
         st_.edges_relaxed = s[0] + s[2] + timed_edges_;
         st_.vertices_processed = s[1] + timed_verts_;
         st_.edges_from_cache = s[5] + timed_cedges_;
@@ -1434,7 +1490,7 @@ public:
         fill<unsigned long long>(ccount_, chunk_bound + 1, 0ull);
         if (nins + ndel) {
             hipLaunchKernelGGL(k_split, dim3(grid_for((uint64_t)nins + ndel, 256, 8192)), dim3(256), 0, stream_, nv_, d_ins, nins,
-                               d_del, ndel, ins_k_, ins_v_, del_k_, ctl_);
+                               d_del, ndel, ins_k_, ins_v_, del_k_, ctl_, swap_pairs_ ? 1 : 0);
             SG_LAUNCHED();
         }
         // sort only the bits ids can hold: kb = bit length of nv, so a masked
@@ -1630,10 +1686,12 @@ public:
     }
 
     // Edge-centric sweep with the edge array in pinned host memory: segment by
-    // segment, the slot range is copied to one of two HBM staging buffers on
-    // the copy stream while the sweep kernel walks the previous segment
+    // segment, the slot range is copied to one of n_stream HBM staging buffers
+    // on the copy stream while the sweep kernel walks earlier segments
     // (framework.cuh:310-384 SEGMENT, n_stream streams). launch(colp, slot_lo,
     // v_lo, v_hi, t_lo, t_hi) enqueues the sweep of one segment on stream_.
+    // streamed_bytes / stream_ms count these copies and the wall time of the
+    // whole streamed sweep, nothing else.
     template <class Launch>
     void streamed_sweep(Launch launch) {
         ensure_segments();
@@ -1646,24 +1704,27 @@ public:
         sync();
         size_t span = 0;
         for (uint32_t k = 0; k < nseg_; ++k) span = std::max<size_t>(span, h_seg_slot_[k + 1] - h_seg_slot_[k]);
-        for (int b = 0; b < 2; ++b) ensure(stage_[b], stage_cap_[b], std::max<size_t>(span, 1));
+        const int nb = nstage();
+        for (int b = 0; b < nb; ++b) ensure(stage_[b], stage_cap_[b], std::max<size_t>(span, 1));
         auto t0 = std::chrono::steady_clock::now();
         uint64_t bytes = 0;
+        uint32_t used = 0;  // segments launched (buffer of the j-th: j % nb)
         for (uint32_t k = 0; k < nseg_; ++k) {
             const uint32_t a = h_seg_v_[k], b2 = h_seg_v_[k + 1];
             if (a >= b2) continue;
-            const int b = k & 1;
+            const int b = (int)(used % (uint32_t)nb);
             const uint64_t slo = h_seg_slot_[k], shi = h_seg_slot_[k + 1];
-            if (k >= 2) SG_HIP(hipStreamWaitEvent(cstream_, seg_ev_[2 + b], 0));  // segment k - 2 released buffer b
+            if (used >= (uint32_t)nb) SG_HIP(hipStreamWaitEvent(cstream_, seg_used_[b], 0));  // buffer b released
             if (shi > slo)
                 SG_HIP(hipMemcpyAsync(stage_[b], col_ + slo, (shi - slo) * 4, hipMemcpyHostToDevice, cstream_));
-            SG_HIP(hipEventRecord(seg_ev_[b], cstream_));
-            SG_HIP(hipStreamWaitEvent(stream_, seg_ev_[b], 0));
+            SG_HIP(hipEventRecord(seg_copied_[b], cstream_));
+            SG_HIP(hipStreamWaitEvent(stream_, seg_copied_[b], 0));
             const uint64_t tlo = h_seg_path_[k] / kSweepTile;
             const uint64_t thi = (h_seg_path_[k + 1] + kSweepTile - 1) / kSweepTile;
             launch((const uint32_t *)stage_[b], slo, a, b2, tlo, thi);
-            SG_HIP(hipEventRecord(seg_ev_[2 + b], stream_));
+            SG_HIP(hipEventRecord(seg_used_[b], stream_));
             bytes += (shi - slo) * 4;
+            ++used;
         }
         sync();
         st_.streamed_bytes += bytes;
@@ -1697,6 +1758,8 @@ public:
     // four snapshots' activations and open a new hotness window.
     void cache_rebuild() {
         if (!cache_slots_) return;
+        sync();  // the refill's wall time starts with the stream idle
+        const auto t0 = std::chrono::steady_clock::now();
         hipLaunchKernelGGL(k_cache_keys, dim3(grid_for(nv_, 256, 16384)), dim3(256), 0, stream_, nv_, deg_, hotw_[0], hotw_[1],
                            hotw_[2], hotw_[3], ckeys_);
         SG_LAUNCHED();
@@ -1731,7 +1794,8 @@ public:
         sync();
         std::memcpy(h, h_u64_, 16);
         st_.cached_vertices = h[0];
-        st_.streamed_bytes += 4 * h[1];  // lists fetched into the cache over the host link
+        st_.cache_fill_bytes += 4 * h[1];  // lists fetched into the cache over the host link
+        st_.cache_fill_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 
     // ---- interface ------------------------------------------------------------------------
@@ -2037,6 +2101,41 @@ public:
 };
 
 // ---------------------------------------------------------------------------
+// In-edge store: a PMA of the reversed graph owned by another engine, on that
+// engine's stream, updated by the owner's batches read as (v, u). No vertex
+// program (the reference's pull side, pull_functor.h:22-369, keeps no store
+// of its own; its in-edges would come from a CSC the reference never builds).
+// ---------------------------------------------------------------------------
+class GraphStore final : public Engine {
+public:
+    using Engine::Engine;
+    SweepTiles tiles_{};  // merge-path tiles over the in-lists (refreshed after every batch)
+    void init(const uint32_t *, uint32_t) override { fail(SG_EINVAL, "a graph store has no vertex program"); }
+    void apply(const uint32_t *d_ins, uint64_t nins, const uint32_t *d_del, uint64_t ndel) override {
+        const bool overflow = prepare(d_ins, nins, d_del, ndel);
+        commit(overflow);
+        finish_batch();
+        st_.snapshots += 1;
+        refresh_tiles();
+    }
+    void refresh_tiles() { tiles_ = sweep_tiles(); }
+    uint64_t ntiles() const { return ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile; }
+    void gather(uint32_t q, void *out, size_t bytes) override { gather_graph(q, out, bytes); }
+};
+
+// In-lists of a CSR (counting sort by target; sources ascending in each list).
+static void transpose_csr(uint32_t nv, uint64_t ne, const uint64_t *row, const uint32_t *col, std::vector<uint64_t> &trow,
+                          std::vector<uint32_t> &tcol) {
+    trow.assign((size_t)nv + 1, 0);
+    for (uint64_t e = 0; e < ne; ++e) ++trow[(size_t)col[e] + 1];
+    for (uint32_t v = 0; v < nv; ++v) trow[(size_t)v + 1] += trow[v];
+    std::vector<uint64_t> pos(trow.begin(), trow.end() - 1);
+    tcol.resize(std::max<uint64_t>(ne, 1));
+    for (uint32_t u = 0; u < nv; ++u)
+        for (uint64_t e = row[u]; e < row[(size_t)u + 1]; ++e) tcol[pos[col[e]]++] = u;
+}
+
+// ---------------------------------------------------------------------------
 // PageRank
 // ---------------------------------------------------------------------------
 class PageRankEngine final : public Engine {
@@ -2046,10 +2145,157 @@ public:
     api::PageRank op_{};
     float *value_ = nullptr, *buffer_ = nullptr;
     double *d_sum_ = nullptr;
+    // pull rounds (cfg.pull_divisor != UINT32_MAX): the in-edge store, the
+    // tagged pushes of dense rounds and the partial sums of cut in-lists
+    std::unique_ptr<GraphStore> rev_;
+    unsigned long long *xb_ = nullptr;
+    float *accg_ = nullptr;
 
-    std::pair<VCombine<api::PageRank>, ERelax<api::PageRank>> relax_fns(uint32_t stamp_val) {
-        return {VCombine<api::PageRank>{op_, value_, buffer_},
-                ERelax<api::PageRank>{op_, buffer_, stamp_, stamp_val, 0, 0}};
+    uint64_t aux_device_bytes() const override { return rev_ ? rev_->dev_bytes_ : 0; }
+    uint64_t aux_host_bytes() const override { return rev_ ? rev_->host_bytes_ : 0; }
+    uint64_t in_capacity() const override { return rev_ ? rev_->cap_ : 0; }
+
+    void after_load(const uint64_t *row, const uint32_t *col) override {
+        if (cfg_.pull_divisor == 0xFFFFFFFFu) return;
+        std::vector<uint64_t> trow;
+        std::vector<uint32_t> tcol;
+        transpose_csr(nv_, ne_, row, col, trow, tcol);
+        sg_config rc = cfg_;
+        rc.flags &= SG_FLAG_HOST_EDGES;  // no timing, graphs or deferral of its own
+        rc.cache_level = 0;
+        rev_.reset(new GraphStore(rc));
+        rev_->swap_pairs_ = true;
+        rev_->budget_peer_ = this;
+        rev_->init_device(stream_);
+        rev_->load_csr(nv_, ne_, trow.data(), tcol.data());
+        rev_->refresh_tiles();
+        xb_ = dalloc<unsigned long long>(nv_);
+        accg_ = dalloc<float>(nv_);
+        fill<unsigned long long>(xb_, nv_, ~0ull);  // tag 0xFFFFFFFF: no round's
+        fill<uint32_t>((uint32_t *)accg_, nv_, 0u);
+        sync();
+    }
+
+    // Frontier out-edges above which a round is a pull.
+    unsigned long long dense_thr() const {
+        const uint64_t div = cfg_.pull_divisor ? cfg_.pull_divisor : 24;
+        return std::max<unsigned long long>(1ull, ne_ / div);
+    }
+    PassIO pr_io(PassIO io) const {
+        if (!rev_) return io;
+        const uint32_t R = io.round;
+        io.ecnt_in = ecnt_ + (R % 3);
+        io.ecnt_out = ecnt_ + ((R + 1) % 3);
+        io.ecnt_zero = ecnt_ + ((R + 2) % 3);
+        io.fdeg = deg_;
+        io.dense_thr = dense_thr();
+        return io;
+    }
+    PullPR pull_state(uint32_t R) const { return PullPR{xb_, buffer_, accg_, stamp_, R, (float)cfg_.pr_epsilon}; }
+
+    // Pull pass of round R over the whole in-edge store (resident store).
+    // force = 0: the kernels return unless the round is dense (device flag).
+    void pull_resident(const PassIO &io, int force, bool timed) {
+        const uint64_t nt = rev_->ntiles();
+        const uint32_t grid = sweep_grid(k_pr_pull, nt);
+        begin_timed(timed && !batch_, SG_KCLASS_PULL);
+        round_launch(k_pr_pull, dim3(grid), dim3(kSweepBlock), rev_->gview(), (const unsigned long long *)rev_->dpre_,
+                     rev_->tiles_, (const uint32_t *)rev_->col_, (uint64_t)0, 0u, nv_, (uint64_t)0, nt, pull_state(io.round),
+                     io, force);
+        round_launch(k_pr_pull_fix, dim3(grid_for(nt + 1, kBlock, 4096)), dim3(kBlock),
+                     (const unsigned long long *)rev_->dpre_, nv_, rev_->tiles_, pull_state(io.round), io, force);
+        end_timed(timed && !batch_);
+    }
+    // The same with the store's edges in pinned host memory: its SEGMENT slot
+    // ranges stream through HBM staging buffers while earlier ones are pulled.
+    void pull_streamed(const PassIO &io, bool timed) {
+        const uint64_t nt = rev_->ntiles();
+        const PullPR ps = pull_state(io.round);
+        const GraphView rg = rev_->gview();
+        const unsigned long long *rdpre = rev_->dpre_;
+        const SweepTiles rt = rev_->tiles_;
+        begin_timed(timed, SG_KCLASS_PULL);
+        rev_->streamed_sweep([&](const uint32_t *colp, uint64_t slo, uint32_t va, uint32_t vb, uint64_t tlo, uint64_t thi) {
+            const uint32_t grid = sweep_grid(k_pr_pull, thi - tlo);
+            hipLaunchKernelGGL(k_pr_pull, dim3(grid), dim3(kSweepBlock), 0, stream_, rg, rdpre, rt, colp, slo, va, vb, tlo,
+                               std::min<uint64_t>(thi, nt), ps, io, 1);
+            SG_LAUNCHED();
+        });
+        hipLaunchKernelGGL(k_pr_pull_fix, dim3(grid_for(nt + 1, kBlock, 4096)), dim3(kBlock), 0, stream_, rdpre, nv_, rt, ps,
+                           io, 1);
+        SG_LAUNCHED();
+        end_timed(timed);
+    }
+    void count_pull_bytes(bool timed) {
+        if (timed && timing(SG_KCLASS_PULL)) kt_bytes_[SG_KCLASS_PULL] += 8ull * ne_ + 24ull * nv_;
+    }
+
+    // One round: push (k_round) or, when the frontier's out-edges pass the
+    // threshold, take + pull + fix-up (the round kernel only takes the
+    // frontier's pushes then).
+    void pr_round(const PassIO &io0, bool do_time) {
+        const PassIO io = pr_io(io0);
+        const uint32_t R = io.round;
+        launch_pass(io, VPageRank{op_, value_, buffer_, xb_, R}, ERelax<api::PageRank>{op_, buffer_, stamp_, R + 1, 0, 0},
+                    do_time);
+        if (rev_) pull_resident(io, 0, do_time && timing(SG_KCLASS_PULL));
+    }
+
+    // Rounds from the frontier of round R_ to convergence.
+    uint32_t pr_rounds(bool timed, int phase = -1) {
+        if (rev_) {  // out-edges of the seeded frontier: the first round's direction
+            hipLaunchKernelGGL(k_list_degsum, dim3(256), dim3(kBlock), 0, stream_, (const uint32_t *)q_[R_ & 1],
+                               (const uint32_t *)(cnt_ + (R_ % 3)), nv_, (const uint32_t *)deg_, ecnt_ + (R_ % 3));
+            SG_LAUNCHED();
+        }
+        if (rev_ && rev_->host_edges_) return pr_rounds_host(timed);
+        const bool do_time = timed && timing(SG_KCLASS_RELAX);
+        const uint64_t p0 = st_.pull_rounds;
+        const uint32_t r = drive_rounds([&](const PassIO &io) { pr_round(io, do_time); }, timed, phase);
+        // the pull kernels of push rounds returned at once: bytes only for the rounds that pulled
+        if (timed && timing(SG_KCLASS_PULL))
+            kt_bytes_[SG_KCLASS_PULL] += (st_.pull_rounds - p0) * (8ull * ne_ + 24ull * nv_);
+        return r;
+    }
+
+    // Host-driven rounds for an in-edge store in pinned host memory: the
+    // direction is chosen on the host before each round (a streamed pull moves
+    // the whole store over the host link, so a push round must not start one).
+    uint32_t pr_rounds_host(bool timed) {
+        const bool do_time = timed && timing(SG_KCLASS_RELAX);
+        if (timed) fill<unsigned long long>(stats_ + 8, 8, 0ull);
+        const uint32_t maxr = cfg_.max_rounds ? cfg_.max_rounds : 1000000u;
+        read_block();
+        uint32_t rounds = 0;
+        last_appended_ = h_blk_[R_ % 3];
+        for (;;) {
+            const uint32_t R = R_;
+            PassIO io = round_io(R);
+            io.log = rlog_;
+            io.round = R;
+            if (timed) io.stats = stats_ + 8;
+            io = pr_io(io);
+            const bool dense = h_ecnt()[R % 3] > io.dense_thr;
+            launch_pass(io, VPageRank{op_, value_, buffer_, xb_, R}, ERelax<api::PageRank>{op_, buffer_, stamp_, R + 1, 0, 0},
+                        do_time);
+            if (dense) {
+                pull_streamed(io, do_time && timing(SG_KCLASS_PULL));
+                count_pull_bytes(timed);
+            }
+            ++R_;
+            ++rounds;
+            read_block();
+            if (do_time) harvest_events();
+            const uint32_t front = h_blk_[(R + 1) % 3], spill = h_blk_[3 + R % 3];
+            last_appended_ += front;
+            if (front == 0 && spill == 0) break;
+            if (rounds >= maxr) {
+                finish_rounds(timed);
+                fail(SG_EDEVICE, "no convergence within " + std::to_string(maxr) + " rounds");
+            }
+        }
+        finish_rounds(timed);
+        return rounds;
     }
 
     void residual() {
@@ -2100,7 +2346,7 @@ public:
         hipLaunchKernelGGL(k_seed_active<api::PageRank>, dim3(grid_for(nv_, kBlock, 16384)), dim3(kBlock), 0, stream_, nv_, op_,
                            value_, buffer_, stamp_, R_, q_[R_ & 1], cnt_ + (R_ % 3), stats_ + 3);
         SG_LAUNCHED();
-        st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, false);
+        st_.rounds = pr_rounds(false);
         residual();
         cache_rebuild();
         inited_ = true;
@@ -2109,7 +2355,8 @@ public:
 
     // Cancelation + Compensate (framework.cuh:1110-1301) restricted to the sources
     // whose out-lists change: push -alpha*value/deg over the old list, update the
-    // graph, push +alpha*value/deg' over the new list, converge on |residual| > eps.
+    // graph (and the in-edge store), push +alpha*value/deg' over the new list,
+    // converge on |residual| > eps.
     void apply(const uint32_t *d_ins, uint64_t nins, const uint32_t *d_del, uint64_t ndel) override {
         if (!inited_) fail(SG_EINVAL, "sg_apply before sg_init");
         auto t0 = std::chrono::steady_clock::now();
@@ -2125,6 +2372,7 @@ public:
             drain_seed_spill(R, ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0});
         }
         commit(overflow);
+        if (rev_) rev_->apply(d_ins, nins, d_del, ndel);
         {
             PassIO io = seed_io(R, touched_, tc, 0);
             launch_pass(io, VAmend<api::PageRank>{op_, +1, value_, buffer_},
@@ -2139,7 +2387,7 @@ public:
             pending_ = true;
             st_.rounds = 0;
         } else {
-            st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, true, kPhaseRelax);
+            st_.rounds = pr_rounds(true, kPhaseRelax);
         }
         finish_batch();
         st_.snapshots += 1;
@@ -2158,12 +2406,11 @@ public:
         hipLaunchKernelGGL((k_seed_active<api::PageRank, true>), dim3(grid_for(nv_, kBlock, 16384)), dim3(kBlock), 0, stream_, nv_, op_,
                            value_, buffer_, stamp_, R_, q_[R_ & 1], cnt_ + (R_ % 3), stats_ + 3);
         SG_LAUNCHED();
-        st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, true, kPhaseRelax);
+        st_.rounds = pr_rounds(true, kPhaseRelax);
         pending_ = false;
         residual();
         st_.last_phase_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-
     void word_source(uint32_t q, uint32_t qi, const uint32_t *&base, uint64_t &stride, uint32_t &word) override {
         if (q == SG_Q_VALUE) { base = (const uint32_t *)value_; stride = 1; word = 0; return; }
         if (q == SG_Q_BUFFER) { base = (const uint32_t *)buffer_; stride = 1; word = 0; return; }
@@ -2207,6 +2454,12 @@ public:
         if (q == SG_Q_PARENT) {
             if (bytes < (size_t)nv_ * 4) fail(SG_EINVAL, "output too small");
             std::memset(out, 0xFF, (size_t)nv_ * 4);
+            return;
+        }
+        if (q == SG_Q_IN_DEGREE || q == SG_Q_IN_OFFSETS || q == SG_Q_IN_SLOTS) {
+            if (!rev_) fail(SG_EINVAL, "this engine keeps no in-edge store (pull_divisor = UINT32_MAX)");
+            sync();
+            rev_->gather_graph(q - SG_Q_IN_DEGREE + SG_Q_DEGREE, out, bytes);
             return;
         }
         gather_graph(q, out, bytes);
@@ -2471,7 +2724,7 @@ void sg_config_default(sg_config *c) {
     c->pma_density = 0.25;
     c->segments = 1;
     c->n_stream = 3;
-    c->block_size = 512;
+    c->pull_divisor = 0;
     c->max_rounds = 0;
     c->rounds_per_sync = 0;  // adaptive
     c->flags = 0;
@@ -2537,13 +2790,6 @@ sg_status sg_gen_rmat_device(int32_t device, uint32_t scale, uint32_t edge_facto
     th.a = q(a);
     th.ab = q(a + b);
     th.abc = q(a + b + c);
-    std::vector<uint32_t> perm;
-    if (permute) {  // the host generator's seeded Fisher-Yates permutation
-        perm.resize(nv);
-        for (uint32_t i = 0; i < nv; ++i) perm[i] = i;
-        SgRng rng(seed ^ 0x5045524d55544531ull, 0);
-        for (uint32_t i = nv - 1; i > 0; --i) std::swap(perm[i], perm[(uint32_t)(rng.next() % ((uint64_t)i + 1))]);
-    }
     int prev = 0;
     if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) {
         sg::g_create_error = "no usable device " + std::to_string(device);
@@ -2556,6 +2802,13 @@ sg_status sg_gen_rmat_device(int32_t device, uint32_t scale, uint32_t edge_facto
     uint32_t *col = nullptr;
     sg_status status = SG_OK;
     try {
+        std::vector<uint32_t> perm;
+        if (permute) {  // the host generator's seeded Fisher-Yates permutation
+            perm.resize(nv);
+            for (uint32_t i = 0; i < nv; ++i) perm[i] = i;
+            SgRng rng(seed ^ 0x5045524d55544531ull, 0);
+            for (uint32_t i = nv - 1; i > 0; --i) std::swap(perm[i], perm[(uint32_t)(rng.next() % ((uint64_t)i + 1))]);
+        }
         SG_HIP(hipStreamCreate(&st));
         SG_HIP(hipMalloc(&d_keys, std::max<uint64_t>(ndraw, 1) * 8));
         SG_HIP(hipMalloc(&d_sorted, std::max<uint64_t>(ndraw, 1) * 8));
@@ -2612,6 +2865,9 @@ sg_status sg_gen_rmat_device(int32_t device, uint32_t scale, uint32_t edge_facto
     } catch (const std::bad_alloc &) {
         sg::g_create_error = "host allocation failed";
         status = SG_ENOMEM;
+    } catch (...) {
+        sg::g_create_error = "unexpected exception";
+        status = SG_EDEVICE;
     }
     std::free(row);
     std::free(col);
@@ -2767,8 +3023,9 @@ sg_status sg_kernel_times(sg_engine *eng, uint32_t cls, uint64_t *launches, doub
 sg_status sg_get_stats(sg_engine *eng, sg_stats *out) {
     return guarded(eng, [&](sg::Engine &e) {
         if (!out) sg::fail(SG_EINVAL, "null output");
-        e.st_.device_bytes = e.dev_bytes_;
-        e.st_.host_bytes = e.host_bytes_;
+        e.st_.device_bytes = e.dev_bytes_ + e.aux_device_bytes();
+        e.st_.host_bytes = e.host_bytes_ + e.aux_host_bytes();
+        e.st_.in_capacity = e.in_capacity();
         *out = e.st_;
     });
 }
@@ -2780,7 +3037,10 @@ sg_status sg_reset_stats(sg_engine *eng) {
         e.st_.relax_launches = 0;
         e.st_.relax_ms = 0;
         e.st_.relax_bytes = 0;
-        e.timed_edges_ = e.timed_verts_ = e.timed_cedges_ = 0;
+        e.timed_edges_ = e.timed_verts_ = e.timed_cedges_ = e.timed_pulls_ = 0;
+        e.st_.pull_rounds = 0;
+        e.st_.cache_fill_bytes = 0;
+        e.st_.cache_fill_ms = 0;
         for (int c = 0; c < SG_KCLASS_COUNT; ++c) {
             e.kt_launches_[c] = e.kt_bytes_[c] = 0;
             e.kt_ms_[c] = 0;

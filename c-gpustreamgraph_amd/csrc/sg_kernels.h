@@ -94,6 +94,16 @@ struct PassIO {
     unsigned long long *stats;  // [0] edges (light), [1] vertices, [2] edges (heavy), [3] overflow
     uint32_t *log;            // round log (or null): round R records the counts its inputs had
     uint32_t round;           // R, for the log
+    // Direction choice (PageRank with an in-edge store): ecnt_in holds the
+    // out-edges of this round's frontier; above dense_thr the round is dense
+    // and the vertex functor only takes the frontier's pushes (a pull pass
+    // over the in-edges delivers them). Appended vertices add their out-degree
+    // (fdeg) to ecnt_out. All null / 0 when the engine has no such choice.
+    const unsigned long long *ecnt_in;
+    unsigned long long *ecnt_out;
+    unsigned long long *ecnt_zero;  // edge counter this pass clears for a later round (or null)
+    const uint32_t *fdeg;
+    unsigned long long dense_thr;
 };
 constexpr uint32_t kRoundLog = 256;  // entries of the round log ring
 
@@ -228,6 +238,7 @@ struct OutStage {
     uint32_t *buf;              // this wave's LDS slice [cap]
     uint32_t n;                 // wave-uniform fill
     uint32_t cap = kOutStage;   // slice size (>= kWave)
+    unsigned long long fedges = 0;  // per lane: out-degrees of the items it appended (io.fdeg)
 };
 
 __device__ __forceinline__ void stage_flush(OutStage &st, const PassIO &io) {
@@ -246,10 +257,19 @@ __device__ __forceinline__ void stage_flush(OutStage &st, const PassIO &io) {
 __device__ __forceinline__ void stage_push(bool pred, uint32_t item, OutStage &st, const PassIO &io) {
     const uint64_t m = __ballot(pred);
     if (m == 0) return;
+    if (io.fdeg && pred) st.fedges += io.fdeg[item];
     const uint32_t c = (uint32_t)__popcll(m);
     if (st.n + c > st.cap) stage_flush(st, io);
     if (pred) st.buf[st.n + (uint32_t)__popcll(m & lanemask_lt())] = item;
     st.n += c;
+}
+
+// One atomic per wave: the out-edges of what the wave appended (io.ecnt_out).
+__device__ __forceinline__ void stage_count_flush(OutStage &st, const PassIO &io) {
+    if (!io.ecnt_out) return;
+    const unsigned long long t = wave_sum_u64(st.fedges);
+    if (lane_id() == 0 && t) atomicAdd(io.ecnt_out, t);
+    st.fedges = 0;
 }
 
 // Per-wave counters, reduced per block at the end of a kernel.
@@ -330,6 +350,10 @@ __device__ __forceinline__ void light_body(const GraphView &g, const PassIO &io,
     // fewer (a power of two, down to 1) when it is small, so that its lists
     // spread over all waves instead of a few waves walking 64 lists each
     const uint32_t per = light_width(n, nwaves);
+    // dense round: the functor records each frontier vertex's push for the
+    // pull pass and no list is walked here
+    bool dense = false;
+    if constexpr (VFn::kDense) dense = io.dense_thr && io.ecnt_in && *io.ecnt_in > io.dense_thr;
     for (uint64_t base = gwave * per; base < n; base += nwaves * per) {
         const uint64_t idx = base + lane;
         uint32_t u = 0, d = 0, push = 0;
@@ -339,6 +363,12 @@ __device__ __forceinline__ void light_body(const GraphView &g, const PassIO &io,
             u = io.in_list ? io.in_list[idx] : io.range_lo + (uint32_t)idx;
             const uint32_t du = g.deg[u];
             sched = vfn(u, du, push, self);
+            if constexpr (VFn::kDense) {
+                if (dense) {
+                    if (sched && du) vfn.take(u, push);
+                    sched = false;
+                }
+            }
             if (sched && du) {
                 d = du;
                 start = list_start(g, u);
@@ -447,6 +477,7 @@ __global__ void __launch_bounds__(kBlock) k_round(GraphView g, PassIO io, VFn vf
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (io.zero_cnt) *io.zero_cnt = 0;
         if (io.heavy_zero) *io.heavy_zero = 0;
+        if (io.ecnt_zero) *io.ecnt_zero = 0;
         if (io.log) {  // the inputs of round R are what round R - 1 produced
             const uint32_t i = 2 * ((io.round + kRoundLog - 1) % kRoundLog);
             io.log[i] = *io.in_cnt;
@@ -463,6 +494,7 @@ __global__ void __launch_bounds__(kBlock) k_round(GraphView g, PassIO io, VFn vf
     light_body(g, io, vfn, efn, n, gwave, nwaves, sh, st, ws);
     heavy_body(g, io, efn, nh, gwave, nwaves, st, ws);
     stage_flush(st, io);
+    stage_count_flush(st, io);
     block_stats(ws, sh, io.stats);
 }
 
@@ -473,6 +505,7 @@ __global__ void __launch_bounds__(kBlock) k_round(GraphView g, PassIO io, VFn vf
 // Data-driven relax round: commit the buffer, push (RelaxCTADB, sync_push_dd.cuh:196-230).
 template <class Op>
 struct VCombine {
+    static constexpr bool kDense = false;
     Op op;
     typename Op::value_t *value;
     typename Op::buffer_t *buffer;
@@ -488,6 +521,7 @@ struct VCombine {
 // (RelaxCTADBAmend, sync_push_dd.cuh:375-470).
 template <class Op>
 struct VAmend {
+    static constexpr bool kDense = false;
     Op op;
     int type;
     typename Op::value_t *value;
@@ -507,6 +541,7 @@ struct VAmend {
 // sync_push_dd.cuh:1470-1557).
 template <class Op>
 struct VSweep {
+    static constexpr bool kDense = false;
     Op op;
     typename Op::value_t *value;
     typename Op::buffer_t *buffer;
@@ -526,6 +561,7 @@ struct VSweep {
 
 // Invalidation propagation: expand every invalidated vertex.
 struct VAll {
+    static constexpr bool kDense = false;
     __device__ bool operator()(uint32_t u, uint32_t, uint32_t &push, bool &self) const {
         self = false;
         push = u;

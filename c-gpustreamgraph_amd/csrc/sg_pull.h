@@ -1,0 +1,192 @@
+// Pull rounds of PageRank over the in-edge store (a second PMA of the same
+// graph with every edge reversed, kept up to date by the same batches).
+//
+// A round R takes the residual of every frontier vertex u (value[u] += x,
+// push p[u] = alpha * x / outdeg(u)) and delivers p[u] to every out-neighbour.
+// The push form walks u's out-list with one f32 atomic add per edge; on R-MAT
+// graphs the frontier is a few hub vertices that own most of the edges for
+// tens of rounds, so those rounds are atomic-bound. The pull form walks all
+// in-lists instead: r[v] += sum of p[u] over in-neighbours u in the frontier.
+// Every destination is owned by one wave (or, for a list cut by a tile
+// boundary, finished by a fix-up pass), so no global atomic is issued per
+// edge. The engine picks the form per round from the frontier's out-edge count
+// (direction-optimising, as Beamer's BFS), on the device, so rounds stay
+// queued ahead of the host.
+//
+// Reference: the pull functor shape of include/framework/variants/pull_functor.h:22-369
+// and sync_pull_dd.cuh:1-357 (dead code there: its schedule calls do not match
+// cta_scheduler_hybrid.cuh:63), and PageRank's operator (samples/hybrid_pr/hybrid_pr.cu:66-136).
+#ifndef SG_PULL_H
+#define SG_PULL_H
+
+#include "sg_pma.h"
+
+namespace sg {
+
+// Tag of a take: round R in the high word, the pushed f32 in the low word.
+// A pull pass of round R reads only the pushes tagged R, so the words of
+// earlier rounds never need clearing.
+__device__ __forceinline__ unsigned long long pr_tag(uint32_t round, float p) {
+    return ((unsigned long long)round << 32) | __float_as_uint(p);
+}
+
+// Vertex functor of PageRank rounds (CombineValueBuffer, hybrid_pr.cu:66-85).
+// In a dense round take() records the push instead of walking the list.
+struct VPageRank {
+    static constexpr bool kDense = true;
+    api::PageRank op;
+    float *value;
+    float *buffer;
+    unsigned long long *xb;  // [nv] tagged pushes (null: no pull store)
+    uint32_t round;
+    __device__ bool operator()(uint32_t u, uint32_t deg, uint32_t &push, bool &self) const {
+        self = false;
+        auto p = op.CombineValueBuffer(u, &value[u], &buffer[u], deg);
+        push = __float_as_uint(p.value);
+        return p.scheduled;
+    }
+    __device__ void take(uint32_t u, uint32_t push) const { xb[u] = pr_tag(round, __uint_as_float(push)); }
+};
+
+// State a pull pass reads and updates.
+struct PullPR {
+    const unsigned long long *xb;  // tagged pushes of this round's frontier
+    float *res;                    // residual (the operator's buffer)
+    float *accg;                   // partial sums of lists cut by tile boundaries (zero between rounds)
+    uint32_t *stamp;               // round stamps (frontier de-duplication)
+    uint32_t round;                // R
+    float eps;                     // activity threshold (IsActiveNode / IsActiveNode_INC)
+};
+
+__device__ __forceinline__ bool pull_dense(const PassIO &io, int force) {
+    return force || (io.dense_thr && io.ecnt_in && *io.ecnt_in > io.dense_thr);
+}
+
+// Residual r of v is final: v joins the next frontier when |r| > eps
+// (both signs: incremental snapshots leave negative residuals).
+__device__ __forceinline__ bool pull_finish(const PullPR &p, uint32_t v, float total) {
+    float r = p.res[v];
+    if (total != 0.0f) {
+        r += total;
+        p.res[v] = r;
+    }
+    return fabsf(r) > p.eps && atomicExch(&p.stamp[v], p.round + 1) != p.round + 1;
+}
+
+// Walk the in-edge tiles [t_begin, t_end) of the store rg (edges read from
+// colp[slot - slot_lo]: the store itself or one streamed segment); only
+// destinations in [dst_lo, dst_hi) are handled. One wave per tile: the
+// tile's in-neighbour ids are loaded coalesced (sweep_tile), their tagged
+// pushes gathered, summed per destination in LDS, and the destinations
+// strictly inside the tile finished at once; the tile's first and last
+// destination (whose lists may continue in the neighbouring tiles) add their
+// partial sum to accg for the fix-up pass.
+__global__ void __launch_bounds__(kSweepBlock) k_pr_pull(GraphView rg, const unsigned long long *dpre, SweepTiles tiles,
+                                                         const uint32_t *colp, uint64_t slot_lo, uint32_t dst_lo,
+                                                         uint32_t dst_hi, uint64_t t_begin, uint64_t t_end, PullPR p,
+                                                         PassIO io, int force) {
+    if (!pull_dense(io, force)) return;
+    __shared__ SweepWave s_wave[kSweepWaves];
+    __shared__ unsigned long long s_red[kSweepWaves];
+    const unsigned long long E = dpre[rg.nv];
+    const unsigned long long ntiles = (rg.nv + E + kSweepTile - 1) / kSweepTile;
+    const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+    SweepWave &sh = s_wave[w];
+    float *const acc = reinterpret_cast<float *>(sh.base);  // base[] is dead once a tile's edges are loaded
+    OutStage st{sh.out, 0, kSweepOutStage};
+    uint32_t n_edges = 0;
+    unsigned long long n_done = 0;
+    TilePipe tp;
+    tp.tend = min_u64(min_u64(t_end, ntiles), tiles.cap - 1);
+    tp.nw = (unsigned long long)gridDim.x * kSweepWaves;
+    tp.t = t_begin + (unsigned long long)blockIdx.x * kSweepWaves + w;
+    tp.start(rg, dpre, tiles);
+    while (tp.t < tp.tend) {
+        uint32_t vv[kSweepPerLane];
+        const TileHead h = tp.h;
+        sweep_tile(rg, dpre, h, tp.pre, colp, slot_lo, dst_lo, dst_hi, sh, vv, n_edges);
+        tp.advance(rg, dpre, tiles);
+        unsigned long long xw[kSweepPerLane];
+#pragma unroll
+        for (int j = 0; j < kSweepPerLane; ++j) xw[j] = vv[j] != kGap ? p.xb[vv[j]] : 0ull;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i = lane; i < h.span; i += kWave) acc[i] = 0.0f;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int j = 0; j < kSweepPerLane; ++j) {
+            if (vv[j] != kGap && (uint32_t)(xw[j] >> 32) == p.round) {
+                const float c = __uint_as_float((uint32_t)xw[j]);
+                if (c != 0.0f) atomicAdd(&acc[sh.own[j * kWave + lane]], c);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i0 = 0; i0 < h.span; i0 += kWave) {
+            const uint32_t i = i0 + lane;
+            const uint32_t v = h.v0 + i;
+            bool act = false;
+            if (i < h.span && v >= dst_lo && v < dst_hi) {
+                const float total = acc[i];
+                if (i == 0 || i + 1 == h.span) {
+                    if (total != 0.0f) atomicAdd(&p.accg[v], total);
+                } else {
+                    act = pull_finish(p, v, total);
+                    ++n_done;
+                }
+            }
+            stage_push(act, v, st, io);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    stage_flush(st, io);
+    stage_count_flush(st, io);
+    block_count<kSweepWaves>((unsigned long long)n_edges, s_red, io.stats + 0);
+    __syncthreads();
+    block_count<kSweepWaves>(n_done, s_red, io.stats + 1);
+}
+
+// Fix-up of a pull pass: every tile's first destination (the vertices whose
+// lists can cross a tile boundary, each handled at its first tile) adds its
+// partial sums and is finished. Tile ntiles is the sentinel (vertex nv - 1).
+__global__ void __launch_bounds__(kBlock) k_pr_pull_fix(const unsigned long long *dpre, uint32_t nv, SweepTiles tiles,
+                                                        PullPR p, PassIO io, int force) {
+    if (!pull_dense(io, force)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(io.stats + 4, 1ull);  // a pull round
+    const unsigned long long E = dpre[nv];
+    const unsigned long long nt = min_u64(((unsigned long long)nv + E + kSweepTile - 1) / kSweepTile, tiles.cap - 1);
+    unsigned long long fed = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base <= nt; base += stride) {
+        const uint64_t t = base + threadIdx.x;
+        bool act = false;
+        uint32_t v = 0;
+        if (t <= nt) {
+            v = tiles.v[t];
+            if (t == 0 || tiles.v[t - 1] != v) {
+                const float total = p.accg[v];
+                if (total != 0.0f) p.accg[v] = 0.0f;
+                act = pull_finish(p, v, total);
+                if (act && io.fdeg) fed += io.fdeg[v];
+            }
+        }
+        wave_append(act, v, io.out_list, io.out_cnt, io.out_cap, io.stats + 3);
+    }
+    if (io.ecnt_out) {
+        fed = wave_sum_u64(fed);
+        if (lane_id() == 0 && fed) atomicAdd(io.ecnt_out, fed);
+    }
+}
+
+// Out-edges of a frontier list (the first round's direction choice).
+__global__ void __launch_bounds__(kBlock) k_list_degsum(const uint32_t *list, const uint32_t *cnt, uint32_t cap,
+                                                        const uint32_t *deg, unsigned long long *out) {
+    const uint32_t n = min(*cnt, cap);
+    unsigned long long s = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        s += deg[list[i]];
+    s = wave_sum_u64(s);
+    if (lane_id() == 0 && s) atomicAdd(out, s);
+}
+
+}  // namespace sg
+
+#endif

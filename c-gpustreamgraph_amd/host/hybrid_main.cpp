@@ -65,7 +65,7 @@ void define_flags() {
     def_i("cache", 4, "HBM edge cache level 0..4 (graph_datum.cuh:151-163); used when the edge array is in host memory");
     def_i("SEGMENT", 1, "vertex segments streamed from host memory (framework.cuh:310-384)");
     def_i("n_stream", 3, "number of streams");
-    def_i("block_size", 512, "block size for traversal kernels");
+    def_i("block_size", 512, "accepted for compatibility; launch shapes are the engine's own");
     def_s("lb_push", "", "load-balancing policy for push");
     def_s("lb_pull", "", "load-balancing policy for pull");
     def_i("weight_num", 0, "1: synthesise graph-file weights");
@@ -283,7 +283,6 @@ int main(int argc, char **argv) {
     cfg.pr_epsilon = F("error");
     cfg.segments = (uint32_t)std::max<long long>(1, I("SEGMENT"));
     cfg.n_stream = (uint32_t)std::max<long long>(1, I("n_stream"));
-    cfg.block_size = (uint32_t)I("block_size");
     cfg.edge_budget_gb = F("device_graph_budget_gb");  // an edge array above this stays in pinned host memory
     cfg.cache_level = (uint32_t)std::max<long long>(0, std::min<long long>(4, I("cache")));
     cfg.pma_density = F("pma_density");

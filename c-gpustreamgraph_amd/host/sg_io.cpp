@@ -233,12 +233,18 @@ sg_status sg_read_snapshot_file(const char *path, uint32_t *nv_out, uint64_t *ne
         return SG_EIO;
     const uint32_t nv = h32[0], nsnap = h32[1];
     const uint64_t ne = h64[0], ti = h64[1], td = h64[2];
-    // the header must agree with the file's length before anything is allocated
-    const uint64_t want = 8 + sizeof h32 + sizeof h64 + ((uint64_t)nv + 1) * 8 + ne * 4 + (uint64_t)nsnap * 16 +
-                          ti * 8 + td * 8;
+    // the header must agree with the file's length before anything is allocated;
+    // every count is bounded by the length first (by division), so no product
+    // or sum below can wrap
     if (std::fseek(fc.f, 0, SEEK_END) != 0) return SG_EIO;
-    const long long len = std::ftell(fc.f);
-    if (len < 0 || (uint64_t)len != want || std::fseek(fc.f, 8 + sizeof h32 + sizeof h64, SEEK_SET) != 0) return SG_EIO;
+    const long long slen = std::ftell(fc.f);
+    if (slen < 0) return SG_EIO;
+    const uint64_t len = (uint64_t)slen;
+    if (ne > len / 4 || ti > len / 8 || td > len / 8 || (uint64_t)nsnap > len / 16 || (uint64_t)nv + 1 > len / 8)
+        return SG_EIO;
+    const uint64_t want = 8 + sizeof h32 + sizeof h64 + ((uint64_t)nv + 1) * 8 + ne * 4 + (uint64_t)nsnap * 16 +
+                          ti * 8 + td * 8;  // each term <= len: the sum fits in 64 bits
+    if (len != want || std::fseek(fc.f, 8 + sizeof h32 + sizeof h64, SEEK_SET) != 0) return SG_EIO;
     uint64_t *row = (uint64_t *)std::malloc(((size_t)nv + 1) * 8);
     uint32_t *col = (uint32_t *)std::malloc(std::max<size_t>((size_t)ne * 4, 4));
     uint64_t *sizes = (uint64_t *)std::malloc(std::max<size_t>((size_t)nsnap * 16, 16));
@@ -258,7 +264,8 @@ sg_status sg_read_snapshot_file(const char *path, uint32_t *nv_out, uint64_t *ne
     for (uint32_t v = 0; v < nv; ++v)
         if (row[v + 1] < row[v]) return drop(SG_EIO);
     uint64_t si = 0, sd = 0;
-    for (uint32_t i = 0; i < nsnap; ++i) {
+    for (uint32_t i = 0; i < nsnap; ++i) {  // a size beyond the total (or a wrapping sum) is refused
+        if (sizes[2 * i] > ti - si || sizes[2 * i + 1] > td - sd) return drop(SG_EIO);
         si += sizes[2 * i];
         sd += sizes[2 * i + 1];
     }

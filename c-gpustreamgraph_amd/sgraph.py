@@ -19,13 +19,15 @@ SG_OK, SG_EINVAL, SG_ENOMEM, SG_EBUDGET, SG_EDEVICE, SG_EIO = range(6)
 STATUS_NAMES = {0: "SG_OK", 1: "SG_EINVAL", 2: "SG_ENOMEM", 3: "SG_EBUDGET", 4: "SG_EDEVICE", 5: "SG_EIO"}
 ALGO = {"pr": 0, "sssp": 1, "bfs": 2, "cc": 3, "multi": 4}
 WEIGHT = {"hash128": 0, "seeded": 1, "one": 2}
-Q_VALUE, Q_BUFFER, Q_PARENT, Q_DEGREE, Q_OFFSETS, Q_SLOTS, Q_SEGMENTS = range(7)
+Q_VALUE, Q_BUFFER, Q_PARENT, Q_DEGREE, Q_OFFSETS, Q_SLOTS, Q_SEGMENTS, Q_IN_DEGREE, Q_IN_OFFSETS, Q_IN_SLOTS = range(10)
+PULL_NEVER = 0xFFFFFFFF      # pull_divisor: push rounds only, no in-edge store
+PULL_ALWAYS = 0x7FFFFFFF     # pull_divisor: every round with a non-empty frontier pulls
 FLAG_KERNEL_TIMING = 0x1
 FLAG_HOST_EDGES = 0x2
 FLAG_DEFER_CONVERGE = 0x4
 FLAG_GRAPH_ROUNDS = 0x8
 FLAG_TIME_SWEEP = 0x10
-KCLASS = ("relax", "sweep", "merge")
+KCLASS = ("relax", "sweep", "merge", "pull")
 INF = 0xFFFFFFFF
 DEFAULT_SEED = 27491095  # kRandSeed of the reference (groute/graphs/util.h:68)
 
@@ -40,7 +42,7 @@ class Config(C.Structure):
     _fields_ = [
         ("algo", C.c_uint32), ("weight_mode", C.c_uint32), ("weight_seed", C.c_uint64),
         ("pr_alpha", C.c_double), ("pr_epsilon", C.c_double), ("pma_density", C.c_double),
-        ("segments", C.c_uint32), ("n_stream", C.c_uint32), ("block_size", C.c_uint32),
+        ("segments", C.c_uint32), ("n_stream", C.c_uint32), ("pull_divisor", C.c_uint32),
         ("max_rounds", C.c_uint32), ("rounds_per_sync", C.c_uint32), ("flags", C.c_uint32),
         ("device_budget_gb", C.c_double), ("device", C.c_int32),
         ("wave_budget", C.c_uint32), ("heavy_degree", C.c_uint32), ("cache_level", C.c_uint32),
@@ -58,7 +60,8 @@ class Stats(C.Structure):
         ("pr_residual_max", C.c_double), ("device_bytes", C.c_uint64), ("host_edges", C.c_uint64),
         ("cache_slots", C.c_uint64), ("cached_vertices", C.c_uint64), ("edges_from_cache", C.c_uint64),
         ("streamed_bytes", C.c_uint64), ("stream_ms", C.c_double), ("host_bytes", C.c_uint64),
-        ("reserved", C.c_uint64 * 1),
+        ("cache_fill_bytes", C.c_uint64), ("cache_fill_ms", C.c_double), ("pull_rounds", C.c_uint64),
+        ("in_capacity", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -387,10 +390,12 @@ class Engine:
         return p
 
     def gather(self, q):
-        if q == Q_OFFSETS:
+        if q in (Q_OFFSETS, Q_IN_OFFSETS):
             out = np.empty(self.nv + 1, dtype=np.uint64)
         elif q == Q_SLOTS:
             out = np.empty(self.stats()["capacity"], dtype=np.uint32)
+        elif q == Q_IN_SLOTS:
+            out = np.empty(self.stats()["in_capacity"], dtype=np.uint32)
         elif q == Q_SEGMENTS:
             out = np.empty(max(self.cfg.segments, 1) + 1, dtype=np.uint32)
         elif q in (Q_VALUE, Q_BUFFER) and self.algo == "pr":
@@ -412,6 +417,17 @@ class Engine:
         np.cumsum(deg.astype(np.uint64), out=row[1:])
         idx = np.repeat(off[:-1], deg.astype(np.int64)) + (np.arange(int(row[-1]), dtype=np.uint64) - np.repeat(row[:-1], deg.astype(np.int64)))
         return row, slots[idx.astype(np.int64)], off, deg, slots
+
+    def in_adjacency(self):
+        """In-lists of the in-edge store as a CSR (row, col): the sources of each vertex's in-edges."""
+        off = self.gather(Q_IN_OFFSETS)
+        deg = self.gather(Q_IN_DEGREE)
+        slots = self.gather(Q_IN_SLOTS)
+        row = np.zeros(self.nv + 1, dtype=np.uint64)
+        np.cumsum(deg.astype(np.uint64), out=row[1:])
+        d = deg.astype(np.int64)
+        idx = np.repeat(off[:-1].astype(np.int64), d) + (np.arange(int(row[-1]), dtype=np.int64) - np.repeat(row[:-1].astype(np.int64), d))
+        return row, slots[idx], off, deg, slots
 
     def stats(self):
         s = Stats()

@@ -60,8 +60,12 @@ typedef struct sg_config {
     double pr_epsilon;        /* residual threshold 1e-6, samples/hybrid_pr/main.cu:37    */
     double pma_density;       /* initial PMA edge density 0.25, csr_graph.cuh:349-352,379 */
     uint32_t segments;        /* -SEGMENT vertex segments, framework.cuh:310-384          */
-    uint32_t n_stream;        /* -n_stream streams, app_skeleton.h:89                     */
-    uint32_t block_size;      /* threads per relax block (multiple of 64, <= 1024)        */
+    uint32_t n_stream;        /* -n_stream (app_skeleton.h:89): segment copies in flight
+                                 (staging buffers) of a streamed sweep, 2..8               */
+    uint32_t pull_divisor;    /* PageRank: a round whose frontier's out-edges exceed
+                                 E / pull_divisor is a pull round over the in-edge store
+                                 (0 = 24); UINT32_MAX = push rounds only, no in-edge store
+                                 (pull_functor.h:22-369, sync_pull_dd.cuh:1-357)          */
     uint32_t max_rounds;      /* cap on convergence rounds per phase (0 = 1000000)        */
     uint32_t rounds_per_sync; /* rounds enqueued between host convergence checks; 0 =
                                  adaptive: a snapshot phase first enqueues the rounds it
@@ -129,7 +133,12 @@ typedef struct sg_stats {
     uint64_t streamed_bytes;      /* host-to-device bytes of streamed segments (cumul.) */
     double stream_ms;             /* wall time of the streamed sweeps (cumulative)      */
     uint64_t host_bytes;          /* pinned host memory held by the engine             */
-    uint64_t reserved[1];
+    uint64_t cache_fill_bytes;    /* bytes copied over the host link into the HBM edge
+                                     cache (cumulative; not in streamed_bytes)          */
+    double cache_fill_ms;         /* wall time of those cache refills (cumulative)     */
+    uint64_t pull_rounds;         /* PageRank rounds run as pulls over the in-edge
+                                     store (cumulative)                                 */
+    uint64_t in_capacity;         /* slots of the in-edge store (0 = none)             */
 } sg_stats;
 
 /* What sg_gather copies out. */
@@ -143,8 +152,14 @@ typedef enum sg_query {
     SG_Q_DEGREE = 3,  /* per-vertex u32 out-degree                                      */
     SG_Q_OFFSETS = 4, /* u64[nv + 1] PMA slot offsets (last entry = capacity)           */
     SG_Q_SLOTS = 5,   /* u32[capacity] PMA slots (UINT32_MAX = gap)                     */
-    SG_Q_SEGMENTS = 6 /* u32[segments + 1] first vertex of each SEGMENT (last = nv), the
-                         greedy split by PMA span of Engine::LoadGraph, framework.cuh:310-362 */
+    SG_Q_SEGMENTS = 6, /* u32[segments + 1] first vertex of each SEGMENT (last = nv), the
+                          greedy split by PMA span of Engine::LoadGraph, framework.cuh:310-362 */
+    /* The in-edge store of an engine that keeps one (PageRank unless
+     * pull_divisor is UINT32_MAX): the same three arrays for the reversed
+     * graph, whose lists hold the sources of each vertex's in-edges. */
+    SG_Q_IN_DEGREE = 7,  /* u32[nv] in-degrees                                          */
+    SG_Q_IN_OFFSETS = 8, /* u64[nv + 1] in-store slot offsets (last entry = in_capacity)  */
+    SG_Q_IN_SLOTS = 9    /* u32[in_capacity] in-store slots (UINT32_MAX = gap)            */
 } sg_query;
 
 typedef struct sg_engine sg_engine;
@@ -270,7 +285,10 @@ typedef enum sg_kernel_class {
     SG_KCLASS_SWEEP = 1,  /* boundary sweep over all live edges (resident edges);
                              bytes: 4 per live edge + 16 per vertex (off, dpre)    */
     SG_KCLASS_MERGE = 2,  /* PMA batch merge: chunk count, chunk write, place       */
-    SG_KCLASS_COUNT = 3
+    SG_KCLASS_PULL = 3,   /* PageRank pull rounds over the in-edge store (k_pr_pull +
+                             fix-up); bytes: 8 per in-edge (id + pushed value) + 24 per
+                             vertex (in-store off and dpre, residual read and write) */
+    SG_KCLASS_COUNT = 4
 } sg_kernel_class;
 
 /* Launches, summed HIP-event milliseconds and algorithmic bytes (0 where the

@@ -34,15 +34,15 @@ def test_every_declared_symbol_is_exported():
 def test_config_defaults_are_the_references():
     cfg = sgraph.default_config()
     assert cfg.pr_alpha == 0.85 and cfg.pr_epsilon == 1e-6
-    assert cfg.pma_density == 0.25 and cfg.n_stream == 3 and cfg.block_size == 512
+    assert cfg.pma_density == 0.25 and cfg.n_stream == 3 and cfg.pull_divisor == 0
     assert cfg.segments == 1 and cfg.weight_seed == 27491095
     assert cfg.cache_level == 4 and cfg.edge_budget_gb == 0  # app_skeleton.h:100 -cache 4
 
 
 def test_struct_layouts_match_the_header():
-    # the ctypes mirrors must have the C structs' sizes (sg_config 104 bytes, sg_stats 200)
+    # the ctypes mirrors must have the C structs' sizes (sg_config 104 bytes, sg_stats 224)
     assert C.sizeof(sgraph.Config) == 104
-    assert C.sizeof(sgraph.Stats) == 200
+    assert C.sizeof(sgraph.Stats) == 224
 
 
 def test_server_config_defaults():

@@ -152,3 +152,61 @@ def test_snapshot_file_round_trip_and_checks(tmp_path):
     r3, c3, s3, i3, d3 = sgraph.read_snapshot_file(q)
     assert np.array_equal(r3, gr) and np.array_equal(c3, gc) and np.array_equal(s3.reshape(-1), np.asarray(gs).reshape(-1))
     assert np.array_equal(i3, gi) and np.array_equal(d3, gd)
+
+
+def test_snapshot_file_crafted_counts_are_refused(tmp_path):
+    # header counts whose byte sizes wrap 64 bits (so the wrapped total equals
+    # the file's real length) and stream sizes whose sum wraps are refused
+    # before anything is allocated or read: SG_EIO, never a short buffer
+    nv, row, col = sgraph.gen_rmat(8, 8, seed=11)
+    ins, dele = sgraph.gen_updates(row, col, 8, 3, 70, 40, seed=12)
+    sizes = np.array([[70, 40]] * 3, dtype=np.uint64)
+    p = str(tmp_path / "g.sgsnap")
+    sgraph.write_snapshot_file(p, row, col, sizes, np.concatenate(ins), np.concatenate(dele))
+    raw = bytearray(open(p, "rb").read())
+    ne, ti, td = len(col), 210, 120
+    h64 = 8 + 8  # magic, nv, nsnap; then ne, ti, td (u64 each)
+
+    def refused(b, name):
+        q = str(tmp_path / f"{name}.sgsnap")
+        open(q, "wb").write(bytes(b))
+        with pytest.raises(sgraph.SgError) as e:
+            sgraph.read_snapshot_file(q)
+        assert e.value.status == sgraph.SG_EIO, name
+
+    sz = h64 + 24 + 8 * (nv + 1) + 4 * ne  # stream sizes [n_add, n_del] per snapshot
+    b = bytearray(raw)  # ti * 8 wraps to the real ti * 8, and the sizes still sum to ti
+    b[h64 + 8:h64 + 16] = ((1 << 61) + ti).to_bytes(8, "little")
+    b[sz:sz + 8] = ((1 << 61) + 70).to_bytes(8, "little")
+    refused(b, "wrap_ti")
+    b = bytearray(raw)  # td * 8 wraps
+    b[h64 + 16:h64 + 24] = ((1 << 61) + td).to_bytes(8, "little")
+    b[sz + 8:sz + 16] = ((1 << 61) + 40).to_bytes(8, "little")
+    refused(b, "wrap_td")
+    b = bytearray(raw)  # ne * 4 wraps (and row[nv] claims it)
+    b[h64:h64 + 8] = ((1 << 62) + ne).to_bytes(8, "little")
+    b[h64 + 24 + 8 * nv:h64 + 24 + 8 * nv + 8] = ((1 << 62) + ne).to_bytes(8, "little")
+    refused(b, "wrap_ne")
+    # stream sizes whose sum wraps to the insert total
+    b = bytearray(raw)
+    b[sz:sz + 8] = ((1 << 64) - 100).to_bytes(8, "little")
+    b[sz + 16:sz + 24] = (70 + 100 + 70).to_bytes(8, "little")
+    refused(b, "wrap_sizes")
+
+
+def test_splitmix64_known_answers():
+    # The seeded weight is 1 + splitmix64(seed ^ (u << 32 | v)) % 63, where
+    # splitmix64(x) is the first output of Vigna's splitmix64 generator seeded
+    # with x. Published outputs of that generator seeded with 1234567 (Rosetta
+    # Code, "Pseudo-random numbers/Splitmix64"): the k-th output is
+    # splitmix64(1234567 + (k - 1) * 0x9E3779B97F4A7C15 mod 2^64).
+    published = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                 4593380528125082431, 16408922859458223821]
+    golden = 0x9E3779B97F4A7C15
+    for k, z in enumerate(published):
+        x = (1234567 + k * golden) % (1 << 64)
+        want = 1 + z % 63
+        # seed = x, (u, v) = (0, 0); and seed = 0 with x spread over (u, v)
+        assert sgraph.edge_weight("seeded", x, 0, 0) == want
+        assert sgraph.edge_weight("seeded", 0, x >> 32, x & 0xFFFFFFFF) == want
+        assert O.weight("seeded", x, 0, 0) == want
