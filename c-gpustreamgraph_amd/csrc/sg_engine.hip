@@ -2215,6 +2215,8 @@ public:
         const unsigned long long *rdpre = rev_->dpre_;
         const SweepTiles rt = rev_->tiles_;
         begin_timed(timed, SG_KCLASS_PULL);
+        const uint64_t b0 = rev_->st_.streamed_bytes;
+        const double ms0 = rev_->st_.stream_ms;
         rev_->streamed_sweep([&](const uint32_t *colp, uint64_t slo, uint32_t va, uint32_t vb, uint64_t tlo, uint64_t thi) {
             const uint32_t grid = sweep_grid(k_pr_pull, thi - tlo);
             hipLaunchKernelGGL(k_pr_pull, dim3(grid), dim3(kSweepBlock), 0, stream_, rg, rdpre, rt, colp, slo, va, vb, tlo,
@@ -2225,6 +2227,8 @@ public:
                            io, 1);
         SG_LAUNCHED();
         end_timed(timed);
+        st_.streamed_bytes += rev_->st_.streamed_bytes - b0;  // the in-store's segments crossed the host link
+        st_.stream_ms += rev_->st_.stream_ms - ms0;
     }
     void count_pull_bytes(bool timed) {
         if (timed && timing(SG_KCLASS_PULL)) kt_bytes_[SG_KCLASS_PULL] += 8ull * ne_ + 24ull * nv_;
