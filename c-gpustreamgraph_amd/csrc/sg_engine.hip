@@ -29,6 +29,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <type_traits>
@@ -68,6 +69,17 @@ struct Error {
 #define SG_LAUNCHED() SG_HIP(hipGetLastError())
 
 static thread_local std::string g_create_error = "no error";
+
+// Scratch of sg_gather_at_noalloc: the vertex ids of one call, then their
+// result words. Each device holds its own copy until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns on that
+// device (the name is a host-side stand-in), resolved once per device when an
+// engine is created there.
+constexpr uint64_t kNoAllocGatherMax = 65536;  // vertices per call (2 x 256 KiB)
+static __device__ __align__(256) uint32_t scratch[2 * kNoAllocGatherMax];
+static std::mutex g_scratch_mu;                 // guards the address table
+static std::vector<uint32_t *> g_scratch_addr;  // per device ordinal
+static std::mutex g_scratch_call_mu[64];        // one call at a time per device (shared contents)
 
 // ---------------------------------------------------------------------------
 // PMA / batch kernels
@@ -630,6 +642,7 @@ public:
     // --- memory -------------------------------------------------------------
     uint64_t dev_bytes_ = 0;
     std::vector<std::pair<void *, size_t>> allocs_;
+    uint32_t *scratch_ = nullptr;  // this device's address of the file-scope scratch
 
     // An engine and the stores it owns share one device budget: each side
     // counts the other's bytes.
@@ -876,6 +889,16 @@ public:
             own_stream_ = false;
         } else {
             SG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        }
+        {  // this device's copy of the no-allocation scratch (device current here)
+            std::lock_guard<std::mutex> lk(g_scratch_mu);
+            if ((size_t)cfg_.device >= g_scratch_addr.size()) g_scratch_addr.resize((size_t)cfg_.device + 1, nullptr);
+            if (!g_scratch_addr[cfg_.device]) {
+                void *p = nullptr;
+                SG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+                g_scratch_addr[cfg_.device] = (uint32_t *)p;
+            }
+            scratch_ = g_scratch_addr[cfg_.device];
         }
         SG_HIP(hipEventCreateWithFlags(&sync_ev_, hipEventDisableTiming));
         SG_HIP(hipHostMalloc(&pinned_, 65536, hipHostMallocDefault));
@@ -1819,6 +1842,31 @@ public:
     // Bring deferred snapshots to their fixed point (SG_FLAG_DEFER_CONVERGE).
     virtual void converge() {}
 
+    // True while snapshots wait for a deferred convergence.
+    virtual bool pending() const { return false; }
+
+    // gather_at staged through the file-scope scratch instead of buffers the
+    // engine allocates: no device allocation, n <= kNoAllocGatherMax. The
+    // caller holds g_scratch_call_mu[device] for the whole call.
+    void gather_at_noalloc(uint32_t q, uint32_t qi, const uint32_t *verts, uint64_t n, uint32_t *out) {
+        if (!inited_ && q != SG_Q_DEGREE) fail(SG_EINVAL, "no vertex state before sg_init");
+        if (n > kNoAllocGatherMax) fail(SG_EINVAL, "at most " + std::to_string(kNoAllocGatherMax) + " vertices per call");
+        if (pending()) fail(SG_EINVAL, "deferred snapshots: call sg_converge first");
+        if (!scratch_) fail(SG_EDEVICE, "no scratch address on this device");
+        if (n == 0) return;
+        const uint32_t *base = nullptr;
+        uint64_t stride = 1;
+        uint32_t word = 0;
+        word_source(q, qi, base, stride, word);
+        uint32_t *ids = scratch_, *res = scratch_ + kNoAllocGatherMax;
+        SG_HIP(hipMemcpyAsync(ids, verts, n * 4, hipMemcpyHostToDevice, stream_));
+        hipLaunchKernelGGL(k_gather_words, dim3(grid_for(n, 256, 256)), dim3(256), 0, stream_, base, stride, word, nv_,
+                           (const uint32_t *)ids, n, res);
+        SG_LAUNCHED();
+        SG_HIP(hipMemcpyAsync(out, res, n * 4, hipMemcpyDeviceToHost, stream_));
+        sync();
+    }
+
     uint32_t *gat_v_ = nullptr, *gat_o_ = nullptr;
     size_t gat_v_cap_ = 0, gat_o_cap_ = 0;
     void gather_at(uint32_t q, uint32_t qi, const uint32_t *verts, uint64_t n, uint32_t *out) {
@@ -2403,6 +2451,7 @@ public:
     // starts the rounds (the frontiers of the deferred passes were not kept;
     // cancellations leave negative residuals).
     bool pending_ = false;
+    bool pending() const override { return pending_; }
     void converge() override {
         if (!pending_) return;
         auto t0 = std::chrono::steady_clock::now();
@@ -2990,6 +3039,18 @@ sg_status sg_gather_at(sg_engine *eng, uint32_t query, uint32_t qi, const uint32
         if (!e.loaded_) sg::fail(SG_EINVAL, "no graph loaded");
         e.converge();
         e.gather_at(query, qi, vertices, n, out);
+    });
+}
+
+sg_status sg_gather_at_noalloc(sg_engine *eng, uint32_t query, uint32_t qi, const uint32_t *vertices, uint64_t n,
+                               uint32_t *out) {
+    return guarded(eng, [&](sg::Engine &e) {
+        if (n && (!vertices || !out)) sg::fail(SG_EINVAL, "null vertex list or output");
+        if (!e.loaded_) sg::fail(SG_EINVAL, "no graph loaded");
+        const int dev = e.cfg_.device;
+        if (dev < 0 || dev >= 64) sg::fail(SG_EDEVICE, "device ordinal beyond the scratch table");
+        std::lock_guard<std::mutex> lk(sg::g_scratch_call_mu[dev]);
+        e.gather_at_noalloc(query, qi, vertices, n, out);
     });
 }
 

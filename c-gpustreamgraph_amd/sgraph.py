@@ -110,6 +110,7 @@ SIGNATURES = [
     ("sg_gather", C.c_int, [_P, C.c_uint32, _P, C.c_size_t]),
     ("sg_init_queries", C.c_int, [_P, _P, _P, C.c_uint32]),
     ("sg_gather_at", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint64, _P]),
+    ("sg_gather_at_noalloc", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_uint64, _P]),
     ("sg_pr_topk", C.c_int, [_P, C.c_uint32, _P, _P]),
     ("sg_converge", C.c_int, [_P]),
     ("sg_device_memory", C.c_int, [C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -354,6 +355,13 @@ class Engine:
         v = np.ascontiguousarray(vertices, dtype=np.uint32)
         out = np.empty(len(v), dtype=np.uint32)
         _check(lib().sg_gather_at(self.h, Q_VALUE if q is None else q, qi, _ptr(v), len(v), _ptr(out)), self.h)
+        return out
+
+    def gather_at_noalloc(self, vertices, q=None, qi=0):
+        """gather_at through the library's fixed device scratch (no device allocation; <= 65536 ids)."""
+        v = np.ascontiguousarray(vertices, dtype=np.uint32)
+        out = np.empty(len(v), dtype=np.uint32)
+        _check(lib().sg_gather_at_noalloc(self.h, Q_VALUE if q is None else q, qi, _ptr(v), len(v), _ptr(out)), self.h)
         return out
 
     def converge(self):

@@ -266,6 +266,19 @@ sg_status sg_gather(sg_engine *eng, uint32_t query, void *out, size_t bytes);
 sg_status sg_gather_at(sg_engine *eng, uint32_t query, uint32_t qi, const uint32_t *vertices, uint64_t n,
                        uint32_t *out);
 
+/* sg_gather_at without any device allocation: the ids and results are
+ * staged through a fixed scratch array in the library's device image (512 KiB
+ * per device, held until the process ends) instead of buffers the engine
+ * allocates. n <= 65536 (SG_EINVAL beyond); SG_EINVAL while PageRank
+ * snapshots wait for a deferred convergence (sg_converge first). Every call
+ * on one device shares that device's scratch: the library serialises them
+ * with a per-device lock, so concurrent calls wait for each other. No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+sg_status sg_gather_at_noalloc(sg_engine *eng, uint32_t query, uint32_t qi, const uint32_t *vertices, uint64_t n,
+                               uint32_t *out);
+
 /* PageRank engines: ids[0..k) = the k vertices of highest rank (ties: smaller
  * id first), ranks (optional) their values. The reference prints the top
  * ranks on the host after a full copy (hybrid_pr_host.cu:161-229, -top_ranks);
