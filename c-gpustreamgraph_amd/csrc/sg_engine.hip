@@ -783,8 +783,10 @@ public:
     uint32_t *inval_ = nullptr;   // bitmap of invalidated vertices
     uint32_t *coarse_ = nullptr;  // coarse filter of inval_: one bit per 2^cshift_ vertices
     uint32_t cshift_ = 6;
-    static constexpr int kPhaseInval = 0, kPhaseRelax = 1;
-    uint32_t last_rounds_[2] = {0, 0};  // rounds the phase took at the last snapshot
+    // phases whose round counts seed the next snapshot's first batch (a second
+    // program on the same engine has its own pair)
+    static constexpr int kPhaseInval = 0, kPhaseRelax = 1, kPhaseInval2 = 2, kPhaseRelax2 = 3;
+    uint32_t last_rounds_[4] = {0, 0, 0, 0};  // rounds the phase took at the last snapshot
     std::unordered_map<const void *, int> occ_;  // resident blocks per CU of a kernel
     int num_cus_ = 0;
     uint32_t R_ = 1;             // global round counter (stamps)
@@ -1317,17 +1319,25 @@ public:
 
     template <class VFn, class EFn>
     void launch_pass(const PassIO &io, VFn vfn, EFn efn, bool timed) {
+        launch_pass_view(gview(), io, vfn, efn, timed);
+    }
+    // The same over another store's lists (the in-edge store of a pull).
+    template <class VFn, class EFn>
+    void launch_pass_view(const GraphView &g, const PassIO &io, VFn vfn, EFn efn, bool timed) {
         const uint32_t g1 = io.in_list ? round_blocks() : grid_for(io.range_n, kWave * kWavesPerBlock, round_blocks());
         begin_timed(timed && !batch_);
-        round_launch(k_round<VFn, EFn>, dim3(g1), dim3(kBlock), gview(), io, vfn, efn);
+        round_launch(k_round<VFn, EFn>, dim3(g1), dim3(kBlock), g, io, vfn, efn);
         end_timed(timed && !batch_);
     }
 
     // Walk, with efn, the descriptors that a seed pass before round R spilled
     // (hq[(R + 2) % 3]), then empty that queue. For a pass whose chunks must not
-    // outlive the slot layout they index (PR cancel, before the PMA commit).
+    // outlive the slot layout they index (PR cancel, before the PMA commit), or
+    // whose chunks index another store (g: a pull over the in-edge store).
     template <class EFn>
-    void drain_seed_spill(uint32_t R, EFn efn) {
+    void drain_seed_spill(uint32_t R, EFn efn) { drain_seed_spill_view(gview(), R, efn); }
+    template <class EFn>
+    void drain_seed_spill_view(const GraphView &g, uint32_t R, EFn efn) {
         PassIO io = base_io();
         io.in_list = q_[R & 1];
         io.in_cnt = cnt_ + 7;  // zero: no light items
@@ -1335,7 +1345,7 @@ public:
         io.out_cnt = cnt_ + (R % 3);
         io.hin = heavy_[(R + 2) % 3];
         io.hin_cnt = hcnt_ + ((R + 2) % 3);
-        launch_pass(io, VAll{}, efn, false);
+        launch_pass_view(g, io, VAll{}, efn, false);
         fill<uint32_t>(hcnt_ + ((R + 2) % 3), 1, 0u);
     }
 
@@ -2015,140 +2025,6 @@ public:
 };
 
 // ---------------------------------------------------------------------------
-// traversal operators (SSSP, BFS, CC)
-// ---------------------------------------------------------------------------
-template <class Op>
-class TraversalEngine final : public Engine {
-public:
-    using Engine::Engine;
-    Op op_{};
-    uint32_t *value_ = nullptr;
-    unsigned long long *buffer_ = nullptr;
-
-    using MakeRelax = std::pair<VCombine<Op>, ERelax<Op>>;
-
-    std::pair<VCombine<Op>, ERelax<Op>> relax_fns(uint32_t stamp_val) {
-        VCombine<Op> vf{op_, value_, buffer_};
-        ERelax<Op> ef{op_, buffer_, stamp_, stamp_val, cfg_.weight_mode, cfg_.weight_seed};
-        return {vf, ef};
-    }
-
-    void state_arrays(std::vector<std::pair<void *, size_t>> &a, uint32_t &source) override {
-        if (!value_) {
-            value_ = dalloc<uint32_t>(nv_);
-            buffer_ = dalloc<unsigned long long>(nv_);
-        }
-        a = {{value_, (size_t)nv_ * 4}, {buffer_, (size_t)nv_ * 8}};
-        source = 0;
-        if constexpr (!std::is_same<Op, api::CC>::value) source = op_.m_source_node;
-    }
-    void restored(uint32_t source) override {
-        if constexpr (!std::is_same<Op, api::CC>::value) op_.m_source_node = source;
-        cache_rebuild();
-        inited_ = true;
-    }
-
-    void init(const uint32_t *sources, uint32_t nsrc) override {
-        if (!loaded_) fail(SG_EINVAL, "sg_init before sg_load_csr");
-        if constexpr (!std::is_same<Op, api::CC>::value) {
-            if (nsrc != 1 || !sources) fail(SG_EINVAL, "SSSP/BFS need exactly one source");
-            if (sources[0] >= nv_) fail(SG_EINVAL, "source id >= nv");
-            op_.m_source_node = sources[0];
-        }
-        if (!value_) {
-            value_ = dalloc<uint32_t>(nv_);
-            buffer_ = dalloc<unsigned long long>(nv_);
-        }
-        auto t0 = std::chrono::steady_clock::now();
-        hipLaunchKernelGGL(k_init_state<Op>, dim3(grid_for(nv_, 256, 16384)), dim3(256), 0, stream_, nv_, op_, value_, buffer_);
-        SG_LAUNCHED();
-        frontier_reset();
-        hipLaunchKernelGGL(k_seed_active<Op>, dim3(grid_for(nv_, kBlock, 16384)), dim3(kBlock), 0, stream_, nv_, op_, value_,
-                           buffer_, stamp_, R_, q_[R_ & 1], cnt_ + (R_ % 3), stats_ + 3);
-        SG_LAUNCHED();
-        st_.rounds = run_rounds([this](uint32_t s) { return relax_fns(s); }, false);
-        cache_rebuild();
-        inited_ = true;
-        st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
-
-    void apply(const uint32_t *d_ins, uint64_t nins, const uint32_t *d_del, uint64_t ndel) override {
-        if (!inited_) fail(SG_EINVAL, "sg_apply before sg_init");
-        auto t0 = std::chrono::steady_clock::now();
-        const bool overflow = prepare(d_ins, nins, d_del, ndel);
-        commit(overflow);
-        // 1. invalidate the subtrees hanging from deleted tree edges (fixed point);
-        //    the bitmaps are zero here (cleared by the last batch's activation pass)
-        frontier_reset();
-        if (ndel_) {
-            hipLaunchKernelGGL(k_inval_seed<Op>, dim3(grid_for(ndel_, kBlock, 8192)), dim3(kBlock), 0, stream_, op_,
-                               (const unsigned long long *)bv_.del_key, ndel_, value_, buffer_, inval_, coarse_,
-                               cshift_, q_[R_ & 1], cnt_ + (R_ % 3), nv_, stats_);
-            SG_LAUNCHED();
-            run_rounds([this](uint32_t) {
-                return std::make_pair(VAll{}, EInvalidate<Op>{op_, value_, buffer_, inval_, coarse_, cshift_});
-            }, false, kPhaseInval);
-        }
-        // the invalidation rounds' last read of the control block holds the count
-        // every vertex the invalidation seeds and rounds appended was reset once
-        const uint64_t ninval = ndel_ ? last_appended_ : 0;
-        st_.invalidated = ninval;
-        // 2. seed the recompute: edges into invalidated vertices + inserted edges
-        frontier_reset();
-        const uint32_t R = R_;
-        if (ninval) {
-            // edges into invalidated vertices: edge-centric sweep over all live edges
-            const SweepTiles tl = sweep_tiles();
-            PassIO io = seed_io(R, nullptr, nullptr, nv_);
-            ERelaxInval<Op> ef{ERelax<Op>{op_, buffer_, stamp_, R, cfg_.weight_mode, cfg_.weight_seed}, inval_, coarse_, cshift_};
-            sweep_all(tl, op_, (const typename Op::buffer_t *)buffer_, ef, io);
-            // invalidated vertices active on their own (CC: own label)
-            hipLaunchKernelGGL(k_inval_activate<Op>, dim3(grid_for((nv_ + 31) / 32, 256, 4096)), dim3(256), 0, stream_, nv_,
-                               op_, inval_, coarse_, cshift_, value_, buffer_, stamp_, R, q_[R & 1], cnt_ + (R % 3), nv_,
-                               stats_ + 3);
-            SG_LAUNCHED();
-        }
-        if (nins_) {
-            hipLaunchKernelGGL(k_insert_relax<Op>, dim3(grid_for(nins_, kBlock, 8192)), dim3(kBlock), 0, stream_, op_, bv_,
-                               br_, buffer_, stamp_, R, cfg_.weight_mode, cfg_.weight_seed, q_[R & 1], cnt_ + (R % 3), nv_,
-                               stats_);
-            SG_LAUNCHED();
-        }
-        // 3. converge
-        st_.rounds = run_rounds([this](uint32_t sv) { return relax_fns(sv); }, true, kPhaseRelax);
-        finish_batch();
-        st_.snapshots += 1;
-        st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
-
-    void word_source(uint32_t q, uint32_t qi, const uint32_t *&base, uint64_t &stride, uint32_t &word) override {
-        if (q == SG_Q_VALUE) { base = value_; stride = 1; word = 0; return; }
-        if (q == SG_Q_PARENT) { base = (const uint32_t *)buffer_; stride = 2; word = 0; return; }  // low word
-        if (q == SG_Q_BUFFER) { base = (const uint32_t *)buffer_; stride = 2; word = 1; return; }  // high word
-        Engine::word_source(q, qi, base, stride, word);
-    }
-
-    void gather(uint32_t q, void *out, size_t bytes) override {
-        if (q == SG_Q_VALUE || q == SG_Q_BUFFER || q == SG_Q_PARENT) {
-            if (!value_) fail(SG_EINVAL, "no vertex state before sg_init");
-            if (bytes < (size_t)nv_ * 4) fail(SG_EINVAL, "output too small");
-            if (q == SG_Q_VALUE) {
-                SG_HIP(hipMemcpyAsync(out, value_, (size_t)nv_ * 4, hipMemcpyDeviceToHost, stream_));
-            } else {
-                ensure(tmp_, tmp_cap_, nv_);
-                hipLaunchKernelGGL(k_extract<Op>, dim3(grid_for(nv_, 256, 16384)), dim3(256), 0, stream_, nv_, buffer_, tmp_,
-                                   q == SG_Q_BUFFER ? 1 : 0);
-                SG_LAUNCHED();
-                SG_HIP(hipMemcpyAsync(out, tmp_, (size_t)nv_ * 4, hipMemcpyDeviceToHost, stream_));
-            }
-            sync();
-            return;
-        }
-        gather_graph(q, out, bytes);
-    }
-};
-
-// ---------------------------------------------------------------------------
 // In-edge store: a PMA of the reversed graph owned by another engine, on that
 // engine's stream, updated by the owner's batches read as (v, u). No vertex
 // program (the reference's pull side, pull_functor.h:22-369, keeps no store
@@ -2183,10 +2059,213 @@ static void transpose_csr(uint32_t nv, uint64_t ne, const uint64_t *row, const u
         for (uint64_t e = row[u]; e < row[(size_t)u + 1]; ++e) tcol[pos[col[e]]++] = u;
 }
 
+// The in-edge store of an engine, built from the CSR it loads (its own device
+// budget counted with the owner's, its work on the owner's stream).
+static std::unique_ptr<GraphStore> make_in_store(Engine &owner, const uint64_t *row, const uint32_t *col) {
+    std::vector<uint64_t> trow;
+    std::vector<uint32_t> tcol;
+    transpose_csr(owner.nv_, owner.ne_, row, col, trow, tcol);
+    sg_config rc = owner.cfg_;
+    rc.flags &= SG_FLAG_HOST_EDGES;  // no timing, graphs or deferral of its own
+    rc.cache_level = 0;
+    std::unique_ptr<GraphStore> in(new GraphStore(rc));
+    in->swap_pairs_ = true;
+    in->budget_peer_ = &owner;
+    in->init_device(owner.stream_);
+    in->load_csr(owner.nv_, owner.ne_, trow.data(), tcol.data());
+    in->refresh_tiles();
+    return in;
+}
+
+// ---------------------------------------------------------------------------
+// traversal operators (SSSP, BFS, CC)
+// ---------------------------------------------------------------------------
+// One traversal program over an engine's graph: its vertex state, the static
+// solve and the per-snapshot recompute. A TraversalEngine runs one; the
+// PageRank + CC engine runs a CC program beside PageRank on the same stores.
+template <class Op>
+struct TravProgram {
+    Op op{};
+    uint32_t *value = nullptr;
+    unsigned long long *buffer = nullptr;
+    int phase_inval = Engine::kPhaseInval, phase_relax = Engine::kPhaseRelax;
+
+    void alloc(Engine &e) {
+        if (!value) {
+            value = e.dalloc<uint32_t>(e.nv_);
+            buffer = e.dalloc<unsigned long long>(e.nv_);
+        }
+    }
+    std::pair<VCombine<Op>, ERelax<Op>> relax_fns(const Engine &e, uint32_t stamp_val) const {
+        return {VCombine<Op>{op, value, buffer}, ERelax<Op>{op, buffer, e.stamp_, stamp_val, e.cfg_.weight_mode, e.cfg_.weight_seed}};
+    }
+
+    // InitGraph + Start (framework.cuh:389-439, 549-584): returns the rounds.
+    uint32_t solve(Engine &e) {
+        alloc(e);
+        hipLaunchKernelGGL(k_init_state<Op>, dim3(grid_for(e.nv_, 256, 16384)), dim3(256), 0, e.stream_, e.nv_, op, value, buffer);
+        SG_LAUNCHED();
+        e.frontier_reset();
+        hipLaunchKernelGGL(k_seed_active<Op>, dim3(grid_for(e.nv_, kBlock, 16384)), dim3(kBlock), 0, e.stream_, e.nv_, op, value,
+                           buffer, e.stamp_, e.R_, e.q_[e.R_ & 1], e.cnt_ + (e.R_ % 3), e.stats_ + 3);
+        SG_LAUNCHED();
+        return e.run_rounds([&](uint32_t sv) { return relax_fns(e, sv); }, false);
+    }
+
+    // After the batch is committed: (1) reset the subtrees hanging from deleted
+    // tree edges (to a fixed point); (2) seed the recompute with the best value
+    // each reset vertex can take from a live in-edge (in: from its in-list in
+    // the in-edge store; else from a sweep over every live edge) and with the
+    // inserted edges; (3) converge. Returns the convergence rounds.
+    uint32_t recompute(Engine &e, GraphStore *in) {
+        // the bitmaps are zero here (cleared by the last batch's activation pass)
+        e.frontier_reset();
+        if (e.ndel_) {
+            hipLaunchKernelGGL(k_inval_seed<Op>, dim3(grid_for(e.ndel_, kBlock, 8192)), dim3(kBlock), 0, e.stream_, op,
+                               (const unsigned long long *)e.bv_.del_key, e.ndel_, value, buffer, e.inval_, e.coarse_,
+                               e.cshift_, e.q_[e.R_ & 1], e.cnt_ + (e.R_ % 3), e.nv_, e.stats_);
+            SG_LAUNCHED();
+            e.run_rounds([&](uint32_t) {
+                return std::make_pair(VAll{}, EInvalidate<Op>{op, value, buffer, e.inval_, e.coarse_, e.cshift_});
+            }, false, phase_inval);
+        }
+        // the invalidation rounds' last read of the control block holds the count
+        // every vertex the invalidation seeds and rounds appended was reset once
+        const uint64_t ninval = e.ndel_ ? e.last_appended_ : 0;
+        e.st_.invalidated = ninval;
+        e.frontier_reset();
+        const uint32_t R = e.R_;
+        if (ninval) {
+            if (in) {
+                // the reset vertices as a list in the next round's frontier buffer
+                // (free until round R runs), their in-lists walked by a round kernel
+                uint32_t *list = e.q_[(R + 1) & 1], *lcnt = e.cnt_ + 6;
+                hipLaunchKernelGGL(k_bits_to_list, dim3(grid_for((e.nv_ + 31) / 32, kBlock, 4096)), dim3(kBlock), 0, e.stream_,
+                                   e.nv_, (const uint32_t *)e.inval_, list, lcnt, e.nv_, e.stats_ + 3);
+                SG_LAUNCHED();
+                PassIO io = e.seed_io(R, list, lcnt, 0);
+                const EPullSeed<Op> ef{op, buffer, e.stamp_, R, e.cfg_.weight_mode, e.cfg_.weight_seed};
+                const GraphView ig = in->gview();
+                e.launch_pass_view(ig, io, VAll{}, ef, false);
+                e.drain_seed_spill_view(ig, R, ef);  // long in-lists cut into chunks
+            } else {
+                // edges into invalidated vertices: edge-centric sweep over all live edges
+                const SweepTiles tl = e.sweep_tiles();
+                PassIO io = e.seed_io(R, nullptr, nullptr, e.nv_);
+                ERelaxInval<Op> ef{ERelax<Op>{op, buffer, e.stamp_, R, e.cfg_.weight_mode, e.cfg_.weight_seed}, e.inval_,
+                                   e.coarse_, e.cshift_};
+                e.sweep_all(tl, op, (const typename Op::buffer_t *)buffer, ef, io);
+            }
+            // invalidated vertices active on their own (CC: own label); clears the bitmaps
+            hipLaunchKernelGGL(k_inval_activate<Op>, dim3(grid_for((e.nv_ + 31) / 32, 256, 4096)), dim3(256), 0, e.stream_, e.nv_,
+                               op, e.inval_, e.coarse_, e.cshift_, value, buffer, e.stamp_, R, e.q_[R & 1], e.cnt_ + (R % 3),
+                               e.nv_, e.stats_ + 3);
+            SG_LAUNCHED();
+        }
+        if (e.nins_) {
+            hipLaunchKernelGGL(k_insert_relax<Op>, dim3(grid_for(e.nins_, kBlock, 8192)), dim3(kBlock), 0, e.stream_, op, e.bv_,
+                               e.br_, buffer, e.stamp_, R, e.cfg_.weight_mode, e.cfg_.weight_seed, e.q_[R & 1],
+                               e.cnt_ + (R % 3), e.nv_, e.stats_);
+            SG_LAUNCHED();
+        }
+        return e.run_rounds([&](uint32_t sv) { return relax_fns(e, sv); }, true, phase_relax);
+    }
+
+    void word_source(const Engine &, uint32_t q, const uint32_t *&base, uint64_t &stride, uint32_t &word) const {
+        if (q == SG_Q_VALUE) { base = value; stride = 1; word = 0; return; }
+        if (q == SG_Q_PARENT) { base = (const uint32_t *)buffer; stride = 2; word = 0; return; }  // low word
+        if (q == SG_Q_BUFFER) { base = (const uint32_t *)buffer; stride = 2; word = 1; return; }  // high word
+        fail(SG_EINVAL, "query not available pointwise");
+    }
+
+    void gather(Engine &e, uint32_t q, void *out, size_t bytes) const {
+        if (!value) fail(SG_EINVAL, "no vertex state before sg_init");
+        if (bytes < (size_t)e.nv_ * 4) fail(SG_EINVAL, "output too small");
+        if (q == SG_Q_VALUE) {
+            SG_HIP(hipMemcpyAsync(out, value, (size_t)e.nv_ * 4, hipMemcpyDeviceToHost, e.stream_));
+        } else {
+            e.ensure(e.tmp_, e.tmp_cap_, e.nv_);
+            hipLaunchKernelGGL(k_extract<Op>, dim3(grid_for(e.nv_, 256, 16384)), dim3(256), 0, e.stream_, e.nv_, buffer, e.tmp_,
+                               q == SG_Q_BUFFER ? 1 : 0);
+            SG_LAUNCHED();
+            SG_HIP(hipMemcpyAsync(out, e.tmp_, (size_t)e.nv_ * 4, hipMemcpyDeviceToHost, e.stream_));
+        }
+        e.sync();
+    }
+};
+
+template <class Op>
+class TraversalEngine final : public Engine {
+public:
+    using Engine::Engine;
+    TravProgram<Op> prog_;
+    std::unique_ptr<GraphStore> in_;  // SG_FLAG_IN_EDGES: the in-edge store the recompute seeds from
+
+    void after_load(const uint64_t *row, const uint32_t *col) override {
+        if (cfg_.flags & SG_FLAG_IN_EDGES) in_ = make_in_store(*this, row, col);
+    }
+    uint64_t aux_device_bytes() const override { return in_ ? in_->dev_bytes_ : 0; }
+    uint64_t aux_host_bytes() const override { return in_ ? in_->host_bytes_ : 0; }
+    uint64_t in_capacity() const override { return in_ ? in_->cap_ : 0; }
+
+    void state_arrays(std::vector<std::pair<void *, size_t>> &a, uint32_t &source) override {
+        prog_.alloc(*this);
+        a = {{prog_.value, (size_t)nv_ * 4}, {prog_.buffer, (size_t)nv_ * 8}};
+        source = 0;
+        if constexpr (!std::is_same<Op, api::CC>::value) source = prog_.op.m_source_node;
+    }
+    void restored(uint32_t source) override {
+        if constexpr (!std::is_same<Op, api::CC>::value) prog_.op.m_source_node = source;
+        cache_rebuild();
+        inited_ = true;
+    }
+
+    void init(const uint32_t *sources, uint32_t nsrc) override {
+        if (!loaded_) fail(SG_EINVAL, "sg_init before sg_load_csr");
+        if constexpr (!std::is_same<Op, api::CC>::value) {
+            if (nsrc != 1 || !sources) fail(SG_EINVAL, "SSSP/BFS need exactly one source");
+            if (sources[0] >= nv_) fail(SG_EINVAL, "source id >= nv");
+            prog_.op.m_source_node = sources[0];
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        st_.rounds = prog_.solve(*this);
+        cache_rebuild();
+        inited_ = true;
+        st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    void apply(const uint32_t *d_ins, uint64_t nins, const uint32_t *d_del, uint64_t ndel) override {
+        if (!inited_) fail(SG_EINVAL, "sg_apply before sg_init");
+        auto t0 = std::chrono::steady_clock::now();
+        const bool overflow = prepare(d_ins, nins, d_del, ndel);
+        commit(overflow);
+        if (in_) in_->apply(d_ins, nins, d_del, ndel);
+        st_.rounds = prog_.recompute(*this, in_.get());
+        finish_batch();
+        st_.snapshots += 1;
+        st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    void word_source(uint32_t q, uint32_t qi, const uint32_t *&base, uint64_t &stride, uint32_t &word) override {
+        if (q == SG_Q_VALUE || q == SG_Q_PARENT || q == SG_Q_BUFFER) return prog_.word_source(*this, q, base, stride, word);
+        Engine::word_source(q, qi, base, stride, word);
+    }
+
+    void gather(uint32_t q, void *out, size_t bytes) override {
+        if (q == SG_Q_VALUE || q == SG_Q_BUFFER || q == SG_Q_PARENT) return prog_.gather(*this, q, out, bytes);
+        if (q == SG_Q_IN_DEGREE || q == SG_Q_IN_OFFSETS || q == SG_Q_IN_SLOTS) {
+            if (!in_) fail(SG_EINVAL, "this engine keeps no in-edge store (SG_FLAG_IN_EDGES)");
+            sync();
+            return in_->gather_graph(q - SG_Q_IN_DEGREE + SG_Q_DEGREE, out, bytes);
+        }
+        gather_graph(q, out, bytes);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // PageRank
 // ---------------------------------------------------------------------------
-class PageRankEngine final : public Engine {
+class PageRankEngine : public Engine {
 public:
     using Engine::Engine;
     bool reads_old_lists() const override { return true; }
@@ -2203,20 +2282,12 @@ public:
     uint64_t aux_host_bytes() const override { return rev_ ? rev_->host_bytes_ : 0; }
     uint64_t in_capacity() const override { return rev_ ? rev_->cap_ : 0; }
 
+    // Whether the engine keeps an in-edge store.
+    virtual bool wants_in_store() const { return cfg_.pull_divisor != 0xFFFFFFFFu; }
     void after_load(const uint64_t *row, const uint32_t *col) override {
-        if (cfg_.pull_divisor == 0xFFFFFFFFu) return;
-        std::vector<uint64_t> trow;
-        std::vector<uint32_t> tcol;
-        transpose_csr(nv_, ne_, row, col, trow, tcol);
-        sg_config rc = cfg_;
-        rc.flags &= SG_FLAG_HOST_EDGES;  // no timing, graphs or deferral of its own
-        rc.cache_level = 0;
-        rev_.reset(new GraphStore(rc));
-        rev_->swap_pairs_ = true;
-        rev_->budget_peer_ = this;
-        rev_->init_device(stream_);
-        rev_->load_csr(nv_, ne_, trow.data(), tcol.data());
-        rev_->refresh_tiles();
+        if (!wants_in_store()) return;
+        rev_ = make_in_store(*this, row, col);
+        if (cfg_.pull_divisor == 0xFFFFFFFFu) return;  // the store serves another program only
         xb_ = dalloc<unsigned long long>(nv_);
         accg_ = dalloc<float>(nv_);
         fill<unsigned long long>(xb_, nv_, ~0ull);  // tag 0xFFFFFFFF: no round's
@@ -2229,8 +2300,9 @@ public:
         const uint64_t div = cfg_.pull_divisor ? cfg_.pull_divisor : 24;
         return std::max<unsigned long long>(1ull, ne_ / div);
     }
+    bool pulls() const { return xb_ != nullptr; }
     PassIO pr_io(PassIO io) const {
-        if (!rev_) return io;
+        if (!pulls()) return io;
         const uint32_t R = io.round;
         io.ecnt_in = ecnt_ + (R % 3);
         io.ecnt_out = ecnt_ + ((R + 1) % 3);
@@ -2290,17 +2362,17 @@ public:
         const uint32_t R = io.round;
         launch_pass(io, VPageRank{op_, value_, buffer_, xb_, R}, ERelax<api::PageRank>{op_, buffer_, stamp_, R + 1, 0, 0},
                     do_time);
-        if (rev_) pull_resident(io, 0, do_time && timing(SG_KCLASS_PULL));
+        if (pulls()) pull_resident(io, 0, do_time && timing(SG_KCLASS_PULL));
     }
 
     // Rounds from the frontier of round R_ to convergence.
     uint32_t pr_rounds(bool timed, int phase = -1) {
-        if (rev_) {  // out-edges of the seeded frontier: the first round's direction
+        if (pulls()) {  // out-edges of the seeded frontier: the first round's direction
             hipLaunchKernelGGL(k_list_degsum, dim3(256), dim3(kBlock), 0, stream_, (const uint32_t *)q_[R_ & 1],
                                (const uint32_t *)(cnt_ + (R_ % 3)), nv_, (const uint32_t *)deg_, ecnt_ + (R_ % 3));
             SG_LAUNCHED();
         }
-        if (rev_ && rev_->host_edges_) return pr_rounds_host(timed);
+        if (pulls() && rev_->host_edges_) return pr_rounds_host(timed);
         const bool do_time = timed && timing(SG_KCLASS_RELAX);
         const uint64_t p0 = st_.pull_rounds;
         const uint32_t r = drive_rounds([&](const PassIO &io) { pr_round(io, do_time); }, timed, phase);
@@ -2400,10 +2472,15 @@ public:
         SG_LAUNCHED();
         st_.rounds = pr_rounds(false);
         residual();
+        init_others();
         cache_rebuild();
         inited_ = true;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
+    // Programs that share this engine's stores: their static solve, and their
+    // recompute after each committed batch (PageRank's rounds done).
+    virtual void init_others() {}
+    virtual void recompute_others() {}
 
     // Cancelation + Compensate (framework.cuh:1110-1301) restricted to the sources
     // whose out-lists change: push -alpha*value/deg over the old list, update the
@@ -2441,6 +2518,7 @@ public:
         } else {
             st_.rounds = pr_rounds(true, kPhaseRelax);
         }
+        recompute_others();
         finish_batch();
         st_.snapshots += 1;
         st_.last_phase_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2516,6 +2594,46 @@ public:
             return;
         }
         gather_graph(q, out, bytes);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// PageRank and CC labels on one graph: one forward store and one in-edge
+// store serve both programs, so each batch updates each store once; CC's
+// recompute after deletions reads the in-lists of its reset vertices instead
+// of streaming every live edge. The reference runs HybridPageRank and
+// HybridCC as separate processes, each with its own PMA
+// (samples/hybrid_pr/hybrid_pr.cu:154-201, samples/hybrid_cc/hybrid_cc.cu:172-208).
+// ---------------------------------------------------------------------------
+class PrCcEngine final : public PageRankEngine {
+public:
+    using PageRankEngine::PageRankEngine;
+    TravProgram<api::CC> cc_;
+
+    bool wants_in_store() const override { return true; }
+    void init_others() override {
+        cc_.phase_inval = kPhaseInval2;
+        cc_.phase_relax = kPhaseRelax2;
+        cc_.solve(*this);
+    }
+    void recompute_others() override { cc_.recompute(*this, rev_.get()); }
+
+    void state_arrays(std::vector<std::pair<void *, size_t>> &, uint32_t &) override {
+        fail(SG_EINVAL, "checkpoints are for single-program engines (PR, SSSP, BFS, CC)");
+    }
+    void gather_query(uint32_t qi, uint32_t what, void *out, size_t bytes) override {
+        if (!inited_) fail(SG_EINVAL, "no vertex state before sg_init");
+        if (qi == 0) return PageRankEngine::gather(what, out, bytes);
+        if (qi == 1) {
+            if (what != SG_Q_VALUE && what != SG_Q_PARENT && what != SG_Q_BUFFER) fail(SG_EINVAL, "CC gives values, parents, buffers");
+            return cc_.gather(*this, what, out, bytes);
+        }
+        fail(SG_EINVAL, "PR + CC engines: qi 0 = PageRank, 1 = CC");
+    }
+    void word_source(uint32_t q, uint32_t qi, const uint32_t *&base, uint64_t &stride, uint32_t &word) override {
+        if (qi == 1 && (q == SG_Q_VALUE || q == SG_Q_PARENT || q == SG_Q_BUFFER))
+            return cc_.word_source(*this, q, base, stride, word);
+        PageRankEngine::word_source(q, qi, base, stride, word);
     }
 };
 
@@ -2942,10 +3060,11 @@ sg_status sg_create(const sg_config *cfg, sg_engine **out) {
             case SG_ALGO_BFS: impl.reset(new sg::TraversalEngine<sg::api::BFS>(*cfg)); break;
             case SG_ALGO_CC: impl.reset(new sg::TraversalEngine<sg::api::CC>(*cfg)); break;
             case SG_ALGO_MULTI: impl.reset(new sg::MultiEngine(*cfg)); break;
+            case SG_ALGO_PR_CC: impl.reset(new sg::PrCcEngine(*cfg)); break;
             default: sg::g_create_error = "unknown algo"; return SG_EINVAL;
         }
         if (cfg->weight_mode > SG_WEIGHT_ONE) { sg::g_create_error = "unknown weight mode"; return SG_EINVAL; }
-        if (cfg->algo == SG_ALGO_PR && !(cfg->pr_alpha > 0 && cfg->pr_alpha < 1)) {
+        if ((cfg->algo == SG_ALGO_PR || cfg->algo == SG_ALGO_PR_CC) && !(cfg->pr_alpha > 0 && cfg->pr_alpha < 1)) {
             sg::g_create_error = "pr_alpha must be in (0, 1)";
             return SG_EINVAL;
         }
@@ -2998,6 +3117,7 @@ sg_status sg_init_queries(sg_engine *eng, const uint32_t *sources, const uint32_
 sg_status sg_gather_query(sg_engine *eng, uint32_t qi, uint32_t what, void *out, size_t bytes) {
     return guarded(eng, [&](sg::Engine &e) {
         if (!out) sg::fail(SG_EINVAL, "null output");
+        e.converge();  // deferred PageRank snapshots of a PR + CC engine
         e.gather_query(qi, what, out, bytes);
     });
 }

@@ -324,7 +324,7 @@ __device__ __forceinline__ void walk_staged(const GraphView &g, const PassIO &io
 #pragma unroll
         for (int j = 0; j < kIlp; ++j) enq[j] = v[j] != kGap && efn.apply(src[j], v[j], push[j], peek[j]);
 #pragma unroll
-        for (int j = 0; j < kIlp; ++j) stage_push(enq[j], v[j], st, io);
+        for (int j = 0; j < kIlp; ++j) stage_push(enq[j], EFn::kAppendSource ? src[j] : v[j], st, io);
     }
 }
 
@@ -365,7 +365,10 @@ __device__ __forceinline__ void light_body(const GraphView &g, const PassIO &io,
             sched = vfn(u, du, push, self);
             if constexpr (VFn::kDense) {
                 if (dense) {
-                    if (sched && du) vfn.take(u, push);
+                    if (sched && du) {
+                        vfn.take(u, push);
+                        count_hot(g, u);  // a pusher: its list is hot for the HBM edge cache
+                    }
                     sched = false;
                 }
             }
@@ -439,7 +442,7 @@ __device__ __forceinline__ void heavy_body(const GraphView &g, const PassIO &io,
 #pragma unroll
             for (int j = 0; j < kIlp; ++j) enq[j] = v[j] != kGap && efn.apply(hd.u, v[j], hd.push, peek[j]);
 #pragma unroll
-            for (int j = 0; j < kIlp; ++j) stage_push(enq[j], v[j], st, io);
+            for (int j = 0; j < kIlp; ++j) stage_push(enq[j], EFn::kAppendSource ? hd.u : v[j], st, io);
         }
         ws.hedges += hlen;
         if (hstart >= g.cap) ws.cedges += hlen;
@@ -571,10 +574,12 @@ struct VAll {
 
 // ---- edge functors -------------------------------------------------------------
 // load(v) peeks the destination state (issued for kIlp edges at once);
-// apply(u, v, push, peek) updates it and returns whether v joins the output.
+// apply(u, v, push, peek) updates it and returns whether v joins the output
+// (u, the walked list's owner, when kAppendSource: a pull over in-lists).
 
 template <class Op>
 struct ERelax {
+    static constexpr bool kAppendSource = false;
     Op op;
     typename Op::buffer_t *buffer;
     uint32_t *stamp;
@@ -606,6 +611,7 @@ struct ERelax {
 // L1) before the fine one, so most edges never touch L2 for the test.
 template <class Op>
 struct ERelaxInval {
+    static constexpr bool kAppendSource = false;
     ERelax<Op> r;
     const uint32_t *inval;
     const uint32_t *coarse;  // coarse filter of inval (the sweep tests it first, in LDS)
@@ -625,6 +631,7 @@ struct ERelaxInval {
 // push_functor.h:960-1034), claiming each child once through the bitmap.
 template <class Op>
 struct EInvalidate {
+    static constexpr bool kAppendSource = false;
     Op op;
     typename Op::value_t *value;
     typename Op::buffer_t *buffer;

@@ -176,6 +176,61 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_fix(const unsigned long long
     }
 }
 
+// ---- traversal recompute from the in-lists of invalidated vertices -------------
+// After deletions, every invalidated vertex v must see the best value any
+// in-neighbour u offers: PushValue(best(u)) + w(u, v) (for CC the label of u).
+// The forward form sweeps every live edge and relaxes those whose target is
+// invalidated (the reference re-pushes from every vertex:
+// RelaxCTADB_all_vertices, sync_push_dd.cuh:1470-1557); with an in-edge store
+// only the in-lists of the invalidated vertices are read. A round kernel walks
+// those lists (in-store view, the invalidated vertices as items) with this
+// functor; whatever improves v appends v itself.
+template <class Op>
+struct EPullSeed {
+    static constexpr bool kAppendSource = true;
+    Op op;
+    typename Op::buffer_t *buffer;
+    uint32_t *stamp;
+    uint32_t stamp_val;
+    uint32_t wmode;
+    uint64_t wseed;
+    __device__ uint64_t load(uint32_t u) const { return ld_peek(&buffer[u]); }
+    // v: the invalidated vertex whose in-list is walked; u: one in-neighbour
+    __device__ bool apply(uint32_t v, uint32_t u, uint32_t, uint64_t bu) const {
+        const uint32_t w = Op::kWeighted ? sg_weight(wmode, wseed, u, v) : 0u;
+        const auto p = op.PushValue(bu);
+        if (!op.MayImprove(u, w, p, ld_peek(&buffer[v]))) return false;
+        if (!op.AccumulateBuffer(u, v, w, &buffer[v], p)) return false;
+        return atomicExch(&stamp[v], stamp_val) != stamp_val;
+    }
+};
+
+// The vertices of a bitmap as a list (one thread per word; one atomic per wave).
+__global__ void __launch_bounds__(kBlock) k_bits_to_list(uint32_t nv, const uint32_t *bits, uint32_t *list, uint32_t *cnt,
+                                                         uint32_t cap, unsigned long long *overflow) {
+    const uint32_t nw = (nv + 31) / 32;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nw; base += stride) {
+        const uint64_t wd = base + threadIdx.x;
+        uint32_t b = wd < nw ? bits[wd] : 0u;
+        if (wd + 1 == nw && (nv & 31)) b &= (1u << (nv & 31)) - 1;
+        const uint32_t c = __popc(b);
+        const uint32_t incl = wave_incl_scan(c);
+        const uint32_t total = __shfl(incl, kWave - 1, kWave);
+        if (total == 0) continue;
+        uint32_t at = 0;
+        if (lane_id() == 0) at = atomicAdd(cnt, total);
+        at = __shfl(at, 0, kWave) + incl - c;
+        while (b) {
+            const uint32_t v = (uint32_t)wd * 32 + __builtin_ctz(b);
+            b &= b - 1;
+            if (at < cap) list[at] = v;
+            else atomicOr(overflow, 1ull);
+            ++at;
+        }
+    }
+}
+
 // Out-edges of a frontier list (the first round's direction choice).
 __global__ void __launch_bounds__(kBlock) k_list_degsum(const uint32_t *list, const uint32_t *cnt, uint32_t cap,
                                                         const uint32_t *deg, unsigned long long *out) {

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libsgraph.so")
 
 SG_OK, SG_EINVAL, SG_ENOMEM, SG_EBUDGET, SG_EDEVICE, SG_EIO = range(6)
 STATUS_NAMES = {0: "SG_OK", 1: "SG_EINVAL", 2: "SG_ENOMEM", 3: "SG_EBUDGET", 4: "SG_EDEVICE", 5: "SG_EIO"}
-ALGO = {"pr": 0, "sssp": 1, "bfs": 2, "cc": 3, "multi": 4}
+ALGO = {"pr": 0, "sssp": 1, "bfs": 2, "cc": 3, "multi": 4, "prcc": 5}
 WEIGHT = {"hash128": 0, "seeded": 1, "one": 2}
 Q_VALUE, Q_BUFFER, Q_PARENT, Q_DEGREE, Q_OFFSETS, Q_SLOTS, Q_SEGMENTS, Q_IN_DEGREE, Q_IN_OFFSETS, Q_IN_SLOTS = range(10)
 PULL_NEVER = 0xFFFFFFFF      # pull_divisor: push rounds only, no in-edge store
@@ -27,6 +27,7 @@ FLAG_HOST_EDGES = 0x2
 FLAG_DEFER_CONVERGE = 0x4
 FLAG_GRAPH_ROUNDS = 0x8
 FLAG_TIME_SWEEP = 0x10
+FLAG_IN_EDGES = 0x20
 KCLASS = ("relax", "sweep", "merge", "pull")
 INF = 0xFFFFFFFF
 DEFAULT_SEED = 27491095  # kRandSeed of the reference (groute/graphs/util.h:68)
@@ -346,8 +347,9 @@ class Engine:
         _check(lib().sg_init_queries(self.h, _ptr(src), _ptr(k), len(src)), self.h)
 
     def query(self, qi, q=None):
-        """Distances (or parents with q=Q_PARENT) of query qi of a batched engine."""
-        out = np.empty(self.nv, dtype=np.uint32)
+        """Distances (or parents with q=Q_PARENT) of query qi of a batched engine; for
+        "prcc" engines qi 0 = PageRank ranks (f32), qi 1 = CC labels."""
+        out = np.empty(self.nv, dtype=np.float32 if (self.algo == "prcc" and qi == 0) else np.uint32)
         _check(lib().sg_gather_query(self.h, qi, Q_VALUE if q is None else q, _ptr(out), out.nbytes), self.h)
         return out
 
@@ -406,7 +408,7 @@ class Engine:
             out = np.empty(self.stats()["in_capacity"], dtype=np.uint32)
         elif q == Q_SEGMENTS:
             out = np.empty(max(self.cfg.segments, 1) + 1, dtype=np.uint32)
-        elif q in (Q_VALUE, Q_BUFFER) and self.algo == "pr":
+        elif q in (Q_VALUE, Q_BUFFER) and self.algo in ("pr", "prcc"):
             out = np.empty(self.nv, dtype=np.float32)
         else:
             out = np.empty(self.nv, dtype=np.uint32)

@@ -39,9 +39,17 @@ typedef enum sg_algo {
     SG_ALGO_SSSP = 1, /* operator SSSP, samples/hybrid_sssp/hybrid_sssp.cu:28-194   */
     SG_ALGO_BFS = 2,  /* operator BFS, samples/hybrid_bfs/hybrid_bfs.cu:26-167      */
     SG_ALGO_CC = 3,   /* operator CC, samples/hybrid_cc/hybrid_cc.cu:26-170         */
-    SG_ALGO_MULTI = 4 /* batched BFS/SSSP queries on one graph (sg_init_queries); no
-                         reference counterpart: the reference answers one source per
-                         process (samples/hybrid_sssp/main.cu, samples/hybrid_bfs/main.cu) */
+    SG_ALGO_MULTI = 4, /* batched BFS/SSSP queries on one graph (sg_init_queries); no
+                          reference counterpart: the reference answers one source per
+                          process (samples/hybrid_sssp/main.cu, samples/hybrid_bfs/main.cu) */
+    SG_ALGO_PR_CC = 5  /* PageRank and CC labels kept together on one graph: one forward
+                          store and one in-edge store serve both programs, each batch is
+                          applied to them once, and CC's recompute after deletions reads
+                          the in-lists of the reset vertices instead of sweeping every
+                          edge. Results: sg_gather_query / sg_gather_at with qi 0 =
+                          PageRank (f32 ranks), qi 1 = CC (u32 labels, parents). No
+                          reference counterpart (HybridPageRank and HybridCC run in
+                          separate processes on separate graphs) */
 } sg_algo;
 
 /* Edge-weight function of SSSP. Weights are computed from (u, v) on the fly
@@ -106,6 +114,12 @@ typedef struct sg_config {
  * launches and their harvest after each round batch cost the timed snapshot
  * several percent, which this keeps out of the measurement. */
 #define SG_FLAG_TIME_SWEEP 0x10u
+/* SSSP / BFS / CC: keep an in-edge store (the reversed graph as a second PMA,
+ * updated by every batch) and seed the recompute after deletions from the
+ * in-lists of the reset vertices instead of a sweep over every live edge
+ * (the reference re-pushes from every vertex, sync_push_dd.cuh:1470-1557).
+ * Results are identical; the sweep's cost moves to the second PMA update. */
+#define SG_FLAG_IN_EDGES 0x20u
 
 typedef struct sg_stats {
     uint32_t nv;                  /* vertices                                          */
@@ -230,6 +244,9 @@ sg_status sg_init_queries(sg_engine *eng, const uint32_t *sources, const uint32_
 
 /* Per-query result of an SG_ALGO_MULTI engine: what = SG_Q_VALUE (u32
  * distance or level, UINT32_MAX = unreached) or SG_Q_PARENT; bytes >= 4 * nv.
+ * For an SG_ALGO_PR_CC engine qi selects the program: 0 = PageRank
+ * (SG_Q_VALUE / SG_Q_BUFFER, f32; deferred snapshots are converged first),
+ * 1 = CC (SG_Q_VALUE labels, SG_Q_PARENT, SG_Q_BUFFER).
  * GatherValue / GatherParent (graph_datum.cuh:297-335) for query qi. */
 sg_status sg_gather_query(sg_engine *eng, uint32_t qi, uint32_t what, void *out, size_t bytes);
 
