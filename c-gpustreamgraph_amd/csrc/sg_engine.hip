@@ -23,6 +23,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -31,6 +32,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <type_traits>
 #include <unordered_map>
@@ -627,6 +629,17 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// fn(lo, hi) over [0, n) split among up to 16 host threads (large loads:
+// validation and layout of edge arrays with billions of entries).
+template <class Fn>
+static void parallel_for(uint64_t n, Fn fn) {
+    const uint64_t nt = std::max<uint64_t>(1, std::min<uint64_t>({16, std::thread::hardware_concurrency(), n / (1u << 20) + 1}));
+    if (nt == 1) { fn((uint64_t)0, n); return; }
+    std::vector<std::thread> th;
+    for (uint64_t t = 0; t < nt; ++t) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+    for (auto &x : th) x.join();
+}
+
 class Engine {
 public:
     explicit Engine(const sg_config &cfg) : cfg_(cfg) {}
@@ -940,8 +953,18 @@ public:
         if (row[0] != 0 || row[nv] != ne) fail(SG_EINVAL, "row[0] must be 0 and row[nv] must equal ne");
         for (uint32_t v = 0; v < nv; ++v)
             if (row[v + 1] < row[v]) fail(SG_EINVAL, "row offsets must be non-decreasing");
-        for (uint64_t e = 0; e < ne; ++e)
-            if (col[e] >= nv) fail(SG_EINVAL, "col entry >= nv at " + std::to_string(e));
+        {
+            std::atomic<uint64_t> bad{~0ull};
+            parallel_for(ne, [&](uint64_t lo, uint64_t hi) {
+                for (uint64_t e = lo; e < hi; ++e)
+                    if (col[e] >= nv) {
+                        uint64_t cur = bad.load();
+                        while (e < cur && !bad.compare_exchange_weak(cur, e)) {}
+                        return;
+                    }
+            });
+            if (bad.load() != ~0ull) fail(SG_EINVAL, "col entry >= nv at " + std::to_string(bad.load()));
+        }
         nv_ = nv;
         ne_ = ne;
         const double dens = (cfg_.pma_density > 0 && cfg_.pma_density <= 1) ? cfg_.pma_density : 0.25;
@@ -1016,25 +1039,31 @@ public:
 
         uint64_t *d_row = dalloc<uint64_t>((size_t)nv + 1);
         SG_HIP(hipMemcpyAsync(d_row, row, ((size_t)nv + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
-        const uint32_t *csr_src = nullptr;
-        uint32_t *csr_pinned = nullptr;
-        if (host_edges_) {
-            // the CSR is staged in pinned host memory and laid out over the host link
-            csr_pinned = halloc<uint32_t>(std::max<uint64_t>(ne, 1));
-            if (ne) std::memcpy(csr_pinned, col, ne * sizeof(uint32_t));
-            csr_src = csr_pinned;
-        } else {
-            ensure(tmp_, tmp_cap_, std::max<uint64_t>(ne, 1));
-            if (ne) SG_HIP(hipMemcpyAsync(tmp_, col, ne * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
-            csr_src = tmp_;
-        }
-        fill<uint32_t>(col_, cap_, kGap);
         hipLaunchKernelGGL(k_layout_offsets, dim3(grid_for((uint64_t)nv + 1, 256, 16384)), dim3(256), 0, stream_, nv, d_row,
                            cap_, off_, deg_);
         SG_LAUNCHED();
-        hipLaunchKernelGGL(k_layout_copy, dim3(grid_for(nv, kWavesPerBlock, 16384)), dim3(kBlock), 0, stream_, nv, d_row,
-                           csr_src, off_, col_);
-        SG_LAUNCHED();
+        if (host_edges_) {
+            // the edge array is pinned host memory: lay the lists out on the host
+            // (no staging copy of the CSR, no pass over the host link)
+            std::vector<uint64_t> hoff((size_t)nv + 1);
+            SG_HIP(hipMemcpyAsync(hoff.data(), off_, hoff.size() * 8, hipMemcpyDeviceToHost, stream_));
+            sync();
+            uint32_t *hc = col_;
+            parallel_for(nv, [&](uint64_t a, uint64_t b) {
+                for (uint64_t v = a; v < b; ++v) {
+                    const uint64_t o = hoff[v], d = row[v + 1] - row[v], span = hoff[v + 1] - o;
+                    if (d) std::memcpy(hc + o, col + row[v], d * 4);
+                    if (span > d) std::memset(hc + o + d, 0xFF, (span - d) * 4);
+                }
+            });
+        } else {
+            ensure(tmp_, tmp_cap_, std::max<uint64_t>(ne, 1));
+            if (ne) SG_HIP(hipMemcpyAsync(tmp_, col, ne * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+            fill<uint32_t>(col_, cap_, kGap);
+            hipLaunchKernelGGL(k_layout_copy, dim3(grid_for(nv, kWavesPerBlock, 16384)), dim3(kBlock), 0, stream_, nv, d_row,
+                               (const uint32_t *)tmp_, off_, col_);
+            SG_LAUNCHED();
+        }
         hipLaunchKernelGGL(k_leaf_init, dim3(grid_for(nleaves_ + 1, 256, 16384)), dim3(256), 0, stream_, nv, L_, nleaves_,
                            d_row, leaf_cnt_);
         SG_LAUNCHED();
@@ -1057,7 +1086,6 @@ public:
         }
         sync();
         dfree(d_row);
-        hfree(csr_pinned);
         seg_dirty_ = true;
         loaded_ = true;
         st_.nv = nv_;
@@ -2059,12 +2087,84 @@ static void transpose_csr(uint32_t nv, uint64_t ne, const uint64_t *row, const u
         for (uint64_t e = row[u]; e < row[(size_t)u + 1]; ++e) tcol[pos[col[e]]++] = u;
 }
 
+// keys[e] = target << 32 | source of every edge e (one wave per source).
+__global__ void __launch_bounds__(kBlock) k_edge_keys_t(uint32_t nv, const uint64_t *row, const uint32_t *col,
+                                                        unsigned long long *keys) {
+    const uint64_t gw = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
+    for (uint64_t v = gw; v < nv; v += nw) {
+        const uint64_t b = row[v], d = row[v + 1] - b;
+        for (uint64_t k = lane_id(); k < d; k += kWave) keys[b + k] = ((unsigned long long)col[b + k] << 32) | v;
+    }
+}
+// Sorted keys (hi << 32 | lo) as a CSR: row[v] = first key with hi >= v, col[i] = lo.
+__global__ void k_keys_to_csr(uint32_t nv, uint64_t ne, const unsigned long long *keys, uint64_t *row, uint32_t *col) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ne; i += stride) col[i] = (uint32_t)keys[i];
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= nv; v += stride) {
+        const unsigned long long key = v << 32;
+        uint64_t lo = 0, hi = ne;
+        while (lo < hi) {
+            const uint64_t m = (lo + hi) >> 1;
+            if (keys[m] < key) lo = m + 1;
+            else hi = m;
+        }
+        row[v] = lo;
+    }
+}
+
+// In-lists of a large CSR on the device: a radix sort of (target, source)
+// keys, then the CSR of the sorted keys (sources ascending in each list, as
+// transpose_csr gives). Transient device memory of about 20 bytes per edge,
+// outside the engine's budget (a load-time scratch, freed before returning).
+static void transpose_csr_device(hipStream_t st, uint32_t nv, uint64_t ne, const uint64_t *row, const uint32_t *col,
+                                 std::vector<uint64_t> &trow, std::vector<uint32_t> &tcol) {
+    void *d_row = nullptr, *d_col = nullptr, *d_k0 = nullptr, *d_k1 = nullptr, *d_tmp = nullptr;
+    auto release = [&]() {
+        for (void *p : {d_row, d_col, d_k0, d_k1, d_tmp})
+            if (p) (void)hipFree(p);
+    };
+    try {
+        SG_HIP(hipMalloc(&d_row, ((size_t)nv + 1) * 8));
+        SG_HIP(hipMalloc(&d_col, (size_t)ne * 4));
+        SG_HIP(hipMalloc(&d_k0, (size_t)ne * 8));
+        SG_HIP(hipMalloc(&d_k1, (size_t)ne * 8));
+        SG_HIP(hipMemcpyAsync(d_row, row, ((size_t)nv + 1) * 8, hipMemcpyHostToDevice, st));
+        SG_HIP(hipMemcpyAsync(d_col, col, (size_t)ne * 4, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_edge_keys_t, dim3(grid_for(nv, kWavesPerBlock, 16384)), dim3(kBlock), 0, st, nv,
+                           (const uint64_t *)d_row, (const uint32_t *)d_col, (unsigned long long *)d_k0);
+        SG_LAUNCHED();
+        const int kb = 32 - __builtin_clz(std::max<uint32_t>(nv, 1));
+        size_t tb = 0;
+        SG_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, (const unsigned long long *)d_k0, (unsigned long long *)d_k1,
+                                                 ne, 0, 32 + kb, st));
+        SG_HIP(hipMalloc(&d_tmp, tb + 256));
+        size_t bb = tb + 256;
+        SG_HIP(hipcub::DeviceRadixSort::SortKeys(d_tmp, bb, (const unsigned long long *)d_k0, (unsigned long long *)d_k1, ne,
+                                                 0, 32 + kb, st));
+        hipLaunchKernelGGL(k_keys_to_csr, dim3(4096), dim3(256), 0, st, nv, ne, (const unsigned long long *)d_k1,
+                           (uint64_t *)d_row, (uint32_t *)d_col);
+        SG_LAUNCHED();
+        trow.resize((size_t)nv + 1);
+        tcol.resize(std::max<uint64_t>(ne, 1));
+        SG_HIP(hipMemcpyAsync(trow.data(), d_row, ((size_t)nv + 1) * 8, hipMemcpyDeviceToHost, st));
+        SG_HIP(hipMemcpyAsync(tcol.data(), d_col, (size_t)ne * 4, hipMemcpyDeviceToHost, st));
+        SG_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+}
+
 // The in-edge store of an engine, built from the CSR it loads (its own device
-// budget counted with the owner's, its work on the owner's stream).
+// budget counted with the owner's, its work on the owner's stream). In-lists
+// are built on the host for small graphs, on the device from 2^24 edges.
 static std::unique_ptr<GraphStore> make_in_store(Engine &owner, const uint64_t *row, const uint32_t *col) {
     std::vector<uint64_t> trow;
     std::vector<uint32_t> tcol;
-    transpose_csr(owner.nv_, owner.ne_, row, col, trow, tcol);
+    if (owner.ne_ >= (1ull << 24)) transpose_csr_device(owner.stream_, owner.nv_, owner.ne_, row, col, trow, tcol);
+    else transpose_csr(owner.nv_, owner.ne_, row, col, trow, tcol);
     sg_config rc = owner.cfg_;
     rc.flags &= SG_FLAG_HOST_EDGES;  // no timing, graphs or deferral of its own
     rc.cache_level = 0;
@@ -2295,9 +2395,13 @@ public:
         sync();
     }
 
-    // Frontier out-edges above which a round is a pull.
+    // Frontier out-edges above which a round is a pull. A resident pull reads
+    // every in-edge at HBM speed without atomics (about 40x cheaper per edge
+    // than an atomic push): it pays above E / 24. Over the host link both
+    // directions move about the same bytes per edge read, so a pull pays only
+    // when the frontier holds a large share of the edges (E / 4).
     unsigned long long dense_thr() const {
-        const uint64_t div = cfg_.pull_divisor ? cfg_.pull_divisor : 24;
+        const uint64_t div = cfg_.pull_divisor ? cfg_.pull_divisor : (rev_ && rev_->host_edges_ ? 4 : 24);
         return std::max<unsigned long long>(1ull, ne_ / div);
     }
     bool pulls() const { return xb_ != nullptr; }
@@ -2372,7 +2476,7 @@ public:
                                (const uint32_t *)(cnt_ + (R_ % 3)), nv_, (const uint32_t *)deg_, ecnt_ + (R_ % 3));
             SG_LAUNCHED();
         }
-        if (pulls() && rev_->host_edges_) return pr_rounds_host(timed);
+        if (pulls() && rev_->host_edges_ && (cfg_.flags & SG_FLAG_PULL_STREAM)) return pr_rounds_host(timed);
         const bool do_time = timed && timing(SG_KCLASS_RELAX);
         const uint64_t p0 = st_.pull_rounds;
         const uint32_t r = drive_rounds([&](const PassIO &io) { pr_round(io, do_time); }, timed, phase);

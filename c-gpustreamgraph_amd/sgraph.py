@@ -28,6 +28,7 @@ FLAG_DEFER_CONVERGE = 0x4
 FLAG_GRAPH_ROUNDS = 0x8
 FLAG_TIME_SWEEP = 0x10
 FLAG_IN_EDGES = 0x20
+FLAG_PULL_STREAM = 0x40
 KCLASS = ("relax", "sweep", "merge", "pull")
 INF = 0xFFFFFFFF
 DEFAULT_SEED = 27491095  # kRandSeed of the reference (groute/graphs/util.h:68)
@@ -182,9 +183,9 @@ def _take(p, count, dtype):
     """Copy a malloc'd array returned by the library into numpy and free it."""
     if count == 0:
         out = np.zeros(0, dtype=dtype)
-    else:
+    else:  # one copy (arrays of billions of entries)
         buf = (C.c_char * (count * np.dtype(dtype).itemsize)).from_address(p.value)
-        out = np.frombuffer(bytes(buf), dtype=dtype).copy()
+        out = np.frombuffer(buf, dtype=dtype).copy()
     lib().sg_free(p)
     return out
 

@@ -72,7 +72,8 @@ typedef struct sg_config {
                                  (staging buffers) of a streamed sweep, 2..8               */
     uint32_t pull_divisor;    /* PageRank: a round whose frontier's out-edges exceed
                                  E / pull_divisor is a pull round over the in-edge store
-                                 (0 = 24); UINT32_MAX = push rounds only, no in-edge store
+                                 (0 = 24, or 4 with the store in host memory);
+                                 UINT32_MAX = push rounds only, no in-edge store
                                  (pull_functor.h:22-369, sync_pull_dd.cuh:1-357)          */
     uint32_t max_rounds;      /* cap on convergence rounds per phase (0 = 1000000)        */
     uint32_t rounds_per_sync; /* rounds enqueued between host convergence checks; 0 =
@@ -120,6 +121,12 @@ typedef struct sg_config {
  * (the reference re-pushes from every vertex, sync_push_dd.cuh:1470-1557).
  * Results are identical; the sweep's cost moves to the second PMA update. */
 #define SG_FLAG_IN_EDGES 0x20u
+/* PageRank with its in-edge store in pinned host memory: a pull round
+ * streams every SEGMENT's slot range (gaps included) through HBM staging
+ * buffers, the direction being chosen on the host before each round. Without
+ * the flag a pull round reads the live slots in place over the host link and
+ * rounds stay queued on the device. */
+#define SG_FLAG_PULL_STREAM 0x40u
 
 typedef struct sg_stats {
     uint32_t nv;                  /* vertices                                          */

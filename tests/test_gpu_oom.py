@@ -46,10 +46,11 @@ def run_host(algo, scale, ef, nsnap, nupd, segments, cache_level, seed=31, **cfg
 @pytest.mark.parametrize("algo", ["sssp", "bfs", "cc", "pr"])
 def test_host_edges_streamed_with_cache(algo):
     st = run_host(algo, 12, 8, 3, 500, segments=7, cache_level=1, flags=sgraph.FLAG_HOST_EDGES)
-    # traversals stream the boundary sweep, PageRank its in-edge store in pull
-    # rounds; cache refills over the link are counted apart
+    # traversals stream the boundary sweep (PageRank's pull rounds read its
+    # in-edge store in place unless SG_FLAG_PULL_STREAM); cache refills over
+    # the link are counted apart
     assert st["cache_slots"] > 0 and (algo == "pr" or st["cache_fill_bytes"] > 0)
-    assert st["streamed_bytes"] > 0 and st["stream_ms"] > 0
+    assert algo == "pr" or (st["streamed_bytes"] > 0 and st["stream_ms"] > 0)
 
 
 @pytest.mark.parametrize("algo", ["sssp", "cc"])

@@ -111,15 +111,36 @@ def test_pr_pull_modes(mode):
 
 
 @pytest.mark.parametrize("mode", ["always", "default"])
-def test_pr_pull_host_edges_streamed(mode):
+@pytest.mark.parametrize("stream", [True, False])
+def test_pr_pull_host_edges(mode, stream):
     # forward and in-edge stores in pinned host memory: pull rounds stream the
-    # in-store's SEGMENT slot ranges through HBM staging buffers
+    # in-store's SEGMENT slot ranges through HBM staging buffers, or read its
+    # live slots in place over the host link
     div = {"always": sgraph.PULL_ALWAYS, "default": 0}[mode]
     nv, row, col = sgraph.gen_rmat(12, 16, seed=57)
     ins, dele = sgraph.gen_updates(row, col, 12, 3, 2000, 2000, seed=58, hub_frac=0.8, hub_top=0.01)
-    st = pr_stream(row, col, list(zip(ins, dele)), pull_divisor=div, flags=sgraph.FLAG_HOST_EDGES, segments=7,
-                   cache_level=1)
-    assert st["host_edges"] == 1 and st["pull_rounds"] > 0 and st["streamed_bytes"] > 0
+    flags = sgraph.FLAG_HOST_EDGES | (sgraph.FLAG_PULL_STREAM if stream else 0)
+    st = pr_stream(row, col, list(zip(ins, dele)), pull_divisor=div, flags=flags, segments=7, cache_level=1)
+    assert st["host_edges"] == 1 and st["pull_rounds"] > 0
+    assert (st["streamed_bytes"] > 0) == stream
+
+
+def test_in_store_built_on_the_device():
+    # from 2^24 edges the in-lists are sorted on the device: the same multisets
+    nv, row, col = sgraph.gen_rmat_device(21, 16)
+    assert len(col) >= (1 << 24)
+    ins, dele = sgraph.gen_updates(row, col, 21, 1, 20000, 20000, seed=63)
+    eng = sgraph.Engine("cc", flags=sgraph.FLAG_IN_EDGES)
+    try:
+        eng.load_csr(row, col)
+        check_in_store(eng, row, col, "load")
+        eng.init()
+        eng.apply(ins[0], dele[0])
+        r, c = O.apply_updates(row, col, ins[0], dele[0])
+        check_in_store(eng, r, c, "G1")
+        assert np.array_equal(eng.values(), O.cc(r, c))
+    finally:
+        eng.close()
 
 
 def test_pr_deferred_with_pull():
