@@ -47,7 +47,7 @@
 #include "sg_pull.h"
 
 static_assert(sizeof(sg_config) == 104, "sg_config layout");
-static_assert(sizeof(sg_stats) == 224, "sg_stats layout");
+static_assert(sizeof(sg_stats) == 240, "sg_stats layout");
 
 namespace sg {
 
@@ -1102,6 +1102,7 @@ public:
     virtual uint64_t aux_device_bytes() const { return 0; }
     virtual uint64_t aux_host_bytes() const { return 0; }
     virtual uint64_t in_capacity() const { return 0; }
+    virtual void in_mirror(uint64_t &slots, uint64_t &live) { slots = live = 0; }
 
     void ensure_heavy(uint64_t edges) {
         // every list past a wave's budget becomes >= 1 descriptor: at most one per
@@ -1121,8 +1122,12 @@ public:
         }
     }
 
+    // HBM mirror of the leading slots of a host-resident edge array (see
+    // GraphStore): read paths take those slots from HBM.
+    uint32_t *mirror_ = nullptr;
+    uint64_t mirror_slots_ = 0;
     GraphView gview() const {
-        GraphView g{nv_, off_, deg_, col_, nullptr, nullptr, cap_, nullptr};
+        GraphView g{nv_, off_, deg_, col_, nullptr, nullptr, cap_, nullptr, mirror_, std::min(mirror_slots_, cap_)};
         if (cache_slots_) {
             g.ccol = ccol_[ccur_];
             g.coff = coff_;
@@ -2069,6 +2074,44 @@ public:
         finish_batch();
         st_.snapshots += 1;
         refresh_tiles();
+        refresh_mirror();
+    }
+    // With the store in pinned host memory, the device edge budget holds an
+    // HBM copy of its leading slots (every pull round reads the whole store;
+    // those slots then come from HBM instead of over the host link). Re-copied
+    // after every batch.
+    void refresh_mirror() {
+        if (!host_edges_ || cfg_.edge_budget_gb <= 0) return;
+        const uint64_t want = std::min<uint64_t>(cap_, (uint64_t)(cfg_.edge_budget_gb * 1073741824.0 / 4.0));
+        if (want == 0) return;
+        if (want > mirror_cap_) {
+            dfree(mirror_);
+            mirror_ = nullptr;
+            mirror_ = dalloc<uint32_t>(want);
+            mirror_cap_ = want;
+        }
+        mirror_slots_ = want;
+        mirror_live_dirty_ = true;
+        SG_HIP(hipMemcpyAsync(mirror_, col_, want * 4, hipMemcpyHostToDevice, stream_));
+    }
+    uint64_t mirror_cap_ = 0;
+    // Mirrored slots and the live edges among them (the first vertex whose
+    // range ends past the mirror splits its list; its mirrored part counts).
+    uint64_t mirror_live_ = 0;
+    bool mirror_live_dirty_ = true;
+    void mirror_counts(uint64_t &slots, uint64_t &live) {
+        slots = std::min(mirror_slots_, cap_);
+        live = 0;
+        if (!slots) return;
+        if (!mirror_live_dirty_) { live = mirror_live_; return; }
+        std::vector<uint64_t> off((size_t)nv_ + 1);
+        std::vector<uint32_t> deg(nv_);
+        SG_HIP(hipMemcpyAsync(off.data(), off_, off.size() * 8, hipMemcpyDeviceToHost, stream_));
+        SG_HIP(hipMemcpyAsync(deg.data(), deg_, deg.size() * 4, hipMemcpyDeviceToHost, stream_));
+        sync();
+        for (uint32_t v = 0; v < nv_ && off[v] < slots; ++v) live += std::min<uint64_t>(deg[v], slots - off[v]);
+        mirror_live_ = live;
+        mirror_live_dirty_ = false;
     }
     void refresh_tiles() { tiles_ = sweep_tiles(); }
     uint64_t ntiles() const { return ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile; }
@@ -2174,6 +2217,7 @@ static std::unique_ptr<GraphStore> make_in_store(Engine &owner, const uint64_t *
     in->init_device(owner.stream_);
     in->load_csr(owner.nv_, owner.ne_, trow.data(), tcol.data());
     in->refresh_tiles();
+    in->refresh_mirror();
     return in;
 }
 
@@ -2307,6 +2351,10 @@ public:
     uint64_t aux_device_bytes() const override { return in_ ? in_->dev_bytes_ : 0; }
     uint64_t aux_host_bytes() const override { return in_ ? in_->host_bytes_ : 0; }
     uint64_t in_capacity() const override { return in_ ? in_->cap_ : 0; }
+    void in_mirror(uint64_t &slots, uint64_t &live) override {
+        slots = live = 0;
+        if (in_) in_->mirror_counts(slots, live);
+    }
 
     void state_arrays(std::vector<std::pair<void *, size_t>> &a, uint32_t &source) override {
         prog_.alloc(*this);
@@ -2381,6 +2429,10 @@ public:
     uint64_t aux_device_bytes() const override { return rev_ ? rev_->dev_bytes_ : 0; }
     uint64_t aux_host_bytes() const override { return rev_ ? rev_->host_bytes_ : 0; }
     uint64_t in_capacity() const override { return rev_ ? rev_->cap_ : 0; }
+    void in_mirror(uint64_t &slots, uint64_t &live) override {
+        slots = live = 0;
+        if (rev_) rev_->mirror_counts(slots, live);
+    }
 
     // Whether the engine keeps an in-edge store.
     virtual bool wants_in_store() const { return cfg_.pull_divisor != 0xFFFFFFFFu; }
@@ -3315,6 +3367,7 @@ sg_status sg_get_stats(sg_engine *eng, sg_stats *out) {
         e.st_.device_bytes = e.dev_bytes_ + e.aux_device_bytes();
         e.st_.host_bytes = e.host_bytes_ + e.aux_host_bytes();
         e.st_.in_capacity = e.in_capacity();
+        e.in_mirror(e.st_.in_mirror_slots, e.st_.in_mirror_live);
         *out = e.st_;
     });
 }

@@ -63,7 +63,7 @@ class Stats(C.Structure):
         ("cache_slots", C.c_uint64), ("cached_vertices", C.c_uint64), ("edges_from_cache", C.c_uint64),
         ("streamed_bytes", C.c_uint64), ("stream_ms", C.c_double), ("host_bytes", C.c_uint64),
         ("cache_fill_bytes", C.c_uint64), ("cache_fill_ms", C.c_double), ("pull_rounds", C.c_uint64),
-        ("in_capacity", C.c_uint64),
+        ("in_capacity", C.c_uint64), ("in_mirror_slots", C.c_uint64), ("in_mirror_live", C.c_uint64),
     ]
 
     def as_dict(self):

@@ -40,9 +40,9 @@ def test_config_defaults_are_the_references():
 
 
 def test_struct_layouts_match_the_header():
-    # the ctypes mirrors must have the C structs' sizes (sg_config 104 bytes, sg_stats 224)
+    # the ctypes mirrors must have the C structs' sizes (sg_config 104 bytes, sg_stats 240)
     assert C.sizeof(sgraph.Config) == 104
-    assert C.sizeof(sgraph.Stats) == 224
+    assert C.sizeof(sgraph.Stats) == 240
 
 
 def test_server_config_defaults():

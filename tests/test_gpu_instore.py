@@ -111,6 +111,13 @@ def test_prcc_host_edges():
     assert st["host_edges"] == 1 and st["in_capacity"] > 0
 
 
+def test_prcc_host_edges_with_hbm_mirror():
+    # the edge budget (below the edge array) holds an HBM copy of part of the
+    # in-store: CC's in-list seeds and PageRank's pulls read those slots there
+    st = prcc_stream(12, 16, 3, 2000, seed=93, edge_budget_gb=1e-4, segments=7, cache_level=1)
+    assert st["host_edges"] == 1
+
+
 def test_prcc_deferred_pagerank():
     # PageRank deferred, CC recomputed every snapshot
     nv, row, col = sgraph.gen_rmat(12, 16, seed=91)

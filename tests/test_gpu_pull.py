@@ -125,6 +125,18 @@ def test_pr_pull_host_edges(mode, stream):
     assert (st["streamed_bytes"] > 0) == stream
 
 
+@pytest.mark.parametrize("budget", [1e-4, 1.0])
+def test_pr_pull_host_edges_with_hbm_mirror(budget):
+    # an edge budget below the edge array: both stores live in pinned host
+    # memory and the budget holds an HBM copy of the in-store's leading slots
+    # (a quarter of them, or all), refreshed after every batch
+    nv, row, col = sgraph.gen_rmat(12, 16, seed=65)
+    ins, dele = sgraph.gen_updates(row, col, 12, 3, 2000, 2000, seed=66, hub_frac=0.8, hub_top=0.01)
+    st = pr_stream(row, col, list(zip(ins, dele)), check_store=True, edge_budget_gb=budget,
+                   flags=sgraph.FLAG_HOST_EDGES, segments=7, pull_divisor=sgraph.PULL_ALWAYS)
+    assert st["host_edges"] == 1 and st["pull_rounds"] > 0
+
+
 def test_in_store_built_on_the_device():
     # from 2^24 edges the in-lists are sorted on the device: the same multisets
     nv, row, col = sgraph.gen_rmat_device(21, 16)
