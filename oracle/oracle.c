@@ -25,6 +25,19 @@
  *                          framework.cuh:1001-1023 (all inserts, then all deletes)
  * or_pma_geometry          csr_graph.cuh:364-382 compute_capacity                 hand-computed values
  * or_segment_partition     framework.cuh:310-362 LoadGraph greedy split          hand-computed values
+ *
+ * Certificates (checks of a result in O(V + E), for graphs where the from-
+ * scratch oracles above are too slow; each returns the number of violations):
+ * or_sp_certificate        d(src) = 0; d(v) <= d(u) + w(u, v) on every edge;     oracle outputs pass, planted
+ *                          every reached v != src has a parent edge (p, v) with   mistakes (a raised or lowered
+ *                          d(v) = d(p) + w(p, v): with w >= 1 d is the distance   distance, a wrong parent) fail
+ * or_cc_certificate        label(v) <= v; label(v) <= label(u) on every edge      oracle outputs pass, planted
+ *                          u -> v; parent edges exist, keep the label, and every  mistakes fail
+ *                          parent chain ends at the vertex label(v) (so label(v)
+ *                          reaches v and is below every ancestor: the min ancestor)
+ * or_pr_residual           FP64 residual of r = (1-a) + a A^T D^-1 r at a given  zero (to rounding) at the
+ *                          rank vector: sum and max |res|. With M = a A^T D^-1     or_pr_fp64 fixed point; the
+ *                          (column sums <= a), ||r - r*||_1 <= ||res||_1/(1 - a)  bound holds on random vectors
  */
 #include <math.h>
 #include <stdint.h>
@@ -395,4 +408,98 @@ uint32_t or_segment_partition(uint32_t nv, const uint64_t *off, uint32_t nseg, u
     }
     seg_snode[s] = nv;
     return s;
+}
+
+/* ---- certificates ------------------------------------------------------ */
+/* Shortest-path certificate: d = dist[] and parent[] of a single-source run
+ * (UINT32_MAX = unreached / no parent) with weights w(mode, seed) >= 1. */
+uint64_t or_sp_certificate(uint32_t nv, const uint64_t *row, const uint32_t *col, uint32_t src, uint32_t wmode,
+                           uint64_t seed, const uint32_t *dist, const uint32_t *parent) {
+    uint64_t bad = 0;
+    uint8_t *found;
+    if (src >= nv) return 1;
+    if (dist[src] != 0) ++bad;
+    found = (uint8_t *)calloc((size_t)(nv ? nv : 1), 1);
+    if (!found) return ~0ull;
+    for (uint32_t u = 0; u < nv; ++u) {
+        if (dist[u] == OR_INF) continue;
+        for (uint64_t e = row[u]; e < row[u + 1]; ++e) {
+            const uint32_t v = col[e];
+            const uint64_t nd = (uint64_t)dist[u] + or_weight(wmode, seed, u, v);
+            if ((uint64_t)dist[v] > nd) ++bad;                   /* an edge still relaxes */
+            if (parent[v] == u && (uint64_t)dist[v] == nd) found[v] = 1;  /* a tight parent edge */
+        }
+    }
+    for (uint32_t v = 0; v < nv; ++v)
+        if (v != src && dist[v] != OR_INF && !found[v]) ++bad;  /* reached without a tight parent edge */
+    free(found);
+    return bad;
+}
+
+/* Min-ancestor label certificate: label[] and parent[] (UINT32_MAX = none). */
+uint64_t or_cc_certificate(uint32_t nv, const uint64_t *row, const uint32_t *col, const uint32_t *label,
+                           const uint32_t *parent) {
+    uint64_t bad = 0;
+    uint8_t *ok, *state;
+    uint32_t *stack;
+    for (uint32_t v = 0; v < nv; ++v)
+        if (label[v] > v) ++bad;
+    ok = (uint8_t *)calloc((size_t)(nv ? nv : 1), 1);     /* parent edge (parent[v], v) exists */
+    state = (uint8_t *)calloc((size_t)(nv ? nv : 1), 1);  /* chain check: 0 new, 1 on stack, 2 good, 3 bad */
+    stack = (uint32_t *)malloc((size_t)(nv ? nv : 1) * sizeof(uint32_t));
+    if (!ok || !state || !stack) { free(ok); free(state); free(stack); return ~0ull; }
+    for (uint32_t u = 0; u < nv; ++u)
+        for (uint64_t e = row[u]; e < row[u + 1]; ++e) {
+            const uint32_t v = col[e];
+            if (label[v] > label[u]) ++bad;  /* the label did not reach v along this edge */
+            if (parent[v] == u) ok[v] = 1;
+        }
+    /* every chain v -> parent[v] -> ... stays on one label and ends at the
+     * vertex label[v] (its own label, no parent); chains are walked once each */
+    for (uint32_t v = 0; v < nv; ++v) {
+        uint64_t n = 0;
+        uint32_t x = v;
+        while (state[x] == 0) {
+            state[x] = 1;
+            stack[n++] = x;
+            if (parent[x] == OR_INF) break;
+            if (parent[x] >= nv || !ok[x] || label[parent[x]] != label[x]) { state[x] = 3; break; }
+            x = parent[x];
+        }
+        uint8_t r;
+        if (state[x] == 1 && parent[x] == OR_INF) r = (label[x] == x) ? 2 : 3;  /* a root */
+        else if (state[x] == 1) r = 3;                                          /* a cycle */
+        else r = state[x];
+        while (n) state[stack[--n]] = r;
+        if (r == 3) ++bad;
+    }
+    free(ok);
+    free(state);
+    free(stack);
+    return bad;
+}
+
+/* FP64 residual res_v = (1 - a) + a * sum_{u -> v} r_u / deg(u) - r_v of a
+ * rank vector (float32 ranks as the engine keeps them). */
+void or_pr_residual(uint32_t nv, const uint64_t *row, const uint32_t *col, double alpha, const float *rank,
+                    double *l1_out, double *max_out, double *sum_rank_out) {
+    double *acc = (double *)calloc((size_t)(nv ? nv : 1), sizeof(double));
+    double l1 = 0, mx = 0, sr = 0;
+    if (!acc) { *l1_out = *max_out = *sum_rank_out = -1; return; }
+    for (uint32_t u = 0; u < nv; ++u) {
+        const uint64_t d = row[u + 1] - row[u];
+        if (!d) continue;
+        const double c = alpha * (double)rank[u] / (double)d;
+        for (uint64_t e = row[u]; e < row[u + 1]; ++e) acc[col[e]] += c;
+    }
+    for (uint32_t v = 0; v < nv; ++v) {
+        const double r = (1.0 - alpha) + acc[v] - (double)rank[v];
+        l1 += fabs(r);
+        if (fabs(r) > mx) mx = fabs(r);
+        sr += (double)rank[v];
+    }
+    free(acc);
+    *l1_out = l1;
+    *max_out = mx;
+    *sum_rank_out = sr;
 }

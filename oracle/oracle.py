@@ -39,6 +39,10 @@ def lib():
             "or_free": (None, [_P]),
             "or_pma_geometry": (None, [C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
             "or_segment_partition": (C.c_uint32, [C.c_uint32, _P, C.c_uint32, _P]),
+            "or_sp_certificate": (C.c_uint64, [C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint64, _P, _P]),
+            "or_cc_certificate": (C.c_uint64, [C.c_uint32, _P, _P, _P, _P]),
+            "or_pr_residual": (None, [C.c_uint32, _P, _P, C.c_double, _P, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -141,3 +145,30 @@ def segment_partition(off, nseg):
     out = np.zeros(nseg + 1, dtype=np.uint32)
     n = lib().or_segment_partition(len(off) - 1, _p(off), nseg, _p(out))
     return out[: n + 1]
+
+
+def sp_certificate(row, col, src, dist, parent, wmode="hash128", seed=0):
+    """Violations of the shortest-path certificate (0 = dist are the distances from src)."""
+    row, col = _csr(row, col)
+    dist = np.ascontiguousarray(dist, dtype=np.uint32)
+    parent = np.ascontiguousarray(parent, dtype=np.uint32)
+    return int(lib().or_sp_certificate(len(row) - 1, _p(row), _p(col), src, WMODE.get(wmode, wmode), seed, _p(dist),
+                                       _p(parent)))
+
+
+def cc_certificate(row, col, label, parent):
+    """Violations of the min-ancestor label certificate (0 = label is the min-ancestor labelling)."""
+    row, col = _csr(row, col)
+    label = np.ascontiguousarray(label, dtype=np.uint32)
+    parent = np.ascontiguousarray(parent, dtype=np.uint32)
+    return int(lib().or_cc_certificate(len(row) - 1, _p(row), _p(col), _p(label), _p(parent)))
+
+
+def pr_residual(row, col, rank, alpha=0.85):
+    """(sum |res|, max |res|, sum rank) of the FP64 fixed-point residual of a rank vector.
+    ||rank - r*||_1 <= sum|res| / (1 - alpha)."""
+    row, col = _csr(row, col)
+    rank = np.ascontiguousarray(rank, dtype=np.float32)
+    l1, mx, sr = C.c_double(), C.c_double(), C.c_double()
+    lib().or_pr_residual(len(row) - 1, _p(row), _p(col), alpha, _p(rank), C.byref(l1), C.byref(mx), C.byref(sr))
+    return l1.value, mx.value, sr.value

@@ -1,0 +1,187 @@
+"""Parity at the configurations' own workloads.
+
+* C4: 64 BFS + 64 SSSP queries from seeded uniform sources on R-MAT scale 24
+  EF16, under update batches of log-uniform sizes in [1e3, 1e6] (50/50
+  insert/delete). After the last batch every query passes the shortest-path
+  certificate (d(src) = 0, no edge relaxes, every reached vertex has a tight
+  parent edge: exact distances for weights >= 1); two sampled queries equal
+  the oracle and single-source engines (distances and parents).
+* C3 shape (PageRank + CC on one engine, edge arrays in pinned host memory
+  under an edge budget with the HBM in-store copy, an HBM edge cache, SEGMENT
+  512, 1M + 1M hub-biased updates): CC bit-exact against the oracle's
+  min-ancestor labels and PageRank within 5e-5 relative / 1e-6 max-abs
+  normalised of the FP64 fixed point at scale 20 EF32; at scale 24 EF32 and
+  at the full C3 size (scale 27 EF32, 4.2 billion edges, 8 GiB budget,
+  -cache 4) the CC labels pass the min-ancestor certificate, every PageRank
+  FP64 residual is at most epsilon, and the residual bounds the L1-normalised
+  error below 1e-5.
+* C5 shape: the request loop at R-MAT scale 18 EF16, every answer equal to
+  the oracle on the snapshot it reports.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sgraph
+
+pytestmark = pytest.mark.gpu
+
+ALPHA = 0.85
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_c4_scale24_batched_queries():
+    scale, nq, nbatch = 24, 128, 6
+    nv, row, col = sgraph.gen_rmat_device(scale, 16, seed=sgraph.DEFAULT_SEED)
+    rng = np.random.default_rng(5)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(1000000), nbatch)).astype(np.int64)
+    ins, dele = sgraph.gen_updates(row, col, scale, nbatch, int(sizes.max()) // 2, int(sizes.max()) // 2, seed=6)
+    batches = [(ins[i][: sizes[i] // 2], dele[i][: sizes[i] - sizes[i] // 2]) for i in range(nbatch)]
+    sources = rng.integers(0, nv, nq).astype(np.uint32)
+    kinds = ["bfs"] * (nq // 2) + ["sssp"] * (nq - nq // 2)
+    say(f"[gpu] C4 nv={nv} ne={len(col)} queries={nq} batch sizes={sizes.tolist()}")
+    e = sgraph.Engine("multi", weight_mode="seeded", weight_seed=7)
+    e.load_csr(row, col)
+    e.init_queries(sources, kinds)
+    r, c = row, col
+    for b in batches:
+        e.apply(*b)
+        r, c = O.apply_updates(r, c, *b)
+    bad = 0
+    for q in range(nq):
+        d, p = e.query(q), e.query(q, sgraph.Q_PARENT)
+        n = O.sp_certificate(r, c, int(sources[q]), d, p, "seeded" if kinds[q] == "sssp" else "one", 7)
+        bad += n
+        if n:
+            say(f"[gpu] C4 query {q} ({kinds[q]} from {sources[q]}): {n} certificate violations")
+    assert bad == 0
+    for q in (0, nq - 1):  # one BFS, one SSSP: the oracle and a single-source engine
+        k = kinds[q]
+        exp = O.bfs(r, c, int(sources[q])) if k == "bfs" else O.sssp(r, c, int(sources[q]), "seeded", 7)
+        assert np.array_equal(e.query(q), exp), f"query {q} differs from the oracle"
+        s = sgraph.Engine(k, weight_mode="seeded", weight_seed=7)
+        s.load_csr(row, col)
+        s.init(int(sources[q]))
+        for b in batches:
+            s.apply(*b)
+        assert np.array_equal(s.values(), e.query(q)) and np.array_equal(s.gather(sgraph.Q_PARENT), e.query(q, sgraph.Q_PARENT))
+        s.close()
+    e.close()
+
+
+def c3_shape(scale, budget, cache, nsnap, batch, exact):
+    nv, row, col = sgraph.gen_rmat_device(scale, 32, seed=sgraph.DEFAULT_SEED)
+    ins, dele = sgraph.gen_updates(row, col, scale, nsnap, batch, batch, seed=sgraph.DEFAULT_SEED + 1, hub_frac=0.8,
+                                   hub_top=0.01)
+    say(f"[gpu] C3 shape scale {scale}: nv={nv} ne={len(col)} budget={budget} GiB cache={cache} batch={batch}")
+    e = sgraph.Engine("prcc", edge_budget_gb=budget, cache_level=cache, segments=512, pma_density=0.6)
+    try:
+        e.load_csr(row, col)
+        e.init()
+        st = e.stats()
+        assert st["host_edges"] == 1 and st["in_mirror_slots"] > 0
+        r, c = row, col
+        for t in range(nsnap):
+            e.apply(ins[t], dele[t])
+            r, c = O.apply_updates(r, c, ins[t], dele[t])
+            lab, par = e.query(1), e.query(1, sgraph.Q_PARENT)
+            assert O.cc_certificate(r, c, lab, par) == 0, f"G{t + 1}: CC certificate"
+            ranks = e.query(0)
+            # FP64 residual of the f32 ranks: its max is the stopping rule
+            # (every |residual| <= epsilon = 1e-6) checked on the host, and
+            # ||r - r*||_1 <= ||res||_1 / (1 - alpha) bounds the L1 error of the
+            # normalised ranks (and so their max-abs error); the 1e-6 max-abs
+            # bound itself is checked against the FP64 oracle where it runs
+            l1, mx, sr = O.pr_residual(r, c, ranks, ALPHA)
+            bound = l1 / (1 - ALPHA) / sr
+            say(f"[gpu] C3 shape G{t + 1}: PR residual max {mx:.2e} L1 {l1:.3e} -> normalised L1 error <= {bound:.2e}")
+            assert mx <= 1.1e-6 and bound <= 1e-5
+            if exact:
+                assert np.array_equal(lab, O.cc(r, c)), f"G{t + 1}: CC labels differ from the oracle"
+                ex = O.pagerank(r, c)
+                rel = float(np.max(np.abs(ranks - ex) / ex))
+                gn, en = ranks.astype(np.float64) / ranks.sum(), ex / ex.sum()
+                say(f"[gpu] C3 shape G{t + 1}: PR rel {rel:.2e} max-abs(norm) {np.max(np.abs(gn - en)):.2e} "
+                    f"L1(norm) {np.abs(gn - en).sum():.2e}")
+                assert rel <= 5e-5 and np.max(np.abs(gn - en)) <= 1e-6
+        return e.stats()
+    finally:
+        e.close()
+
+
+def test_c3_shape_scale20_exact():
+    c3_shape(20, 0.05, 1, 2, 100000, exact=True)
+
+
+def test_c3_shape_scale24_certificates():
+    c3_shape(24, 0.5, 2, 2, 1000000, exact=False)
+
+
+@pytest.mark.timeout(900)
+def test_c3_full_size_certificates():
+    # C3 itself: scale 27 EF32, 8 GiB edge budget, -cache 4, SEGMENT 512, 1M + 1M hub-biased
+    c3_shape(27, 8.0, 4, 1, 1000000, exact=False)
+
+
+def test_c5_shape_serving_scale18():
+    scale, nsnap = 18, 3
+    nv, row, col = sgraph.gen_rmat(scale, 16, seed=95)
+    ins, dele = sgraph.gen_updates(row, col, scale, nsnap, 20000, 20000, seed=96)
+    graphs = [(row, col)]
+    for t in range(nsnap):
+        graphs.append(O.apply_updates(*graphs[-1], ins[t], dele[t]))
+    srv = sgraph.Server(row, col, query_slots=64, weight_mode="seeded", weight_seed=9)
+    results, lock = [], threading.Lock()
+
+    def client(cid):
+        r = np.random.default_rng(200 + cid)
+        mine = []
+        for _ in range(10):
+            x = r.random()
+            kind = sgraph.REQ_BFS if x < 0.4 else sgraph.REQ_SSSP if x < 0.7 else sgraph.REQ_CC if x < 0.9 else sgraph.REQ_PR_TOPK
+            arg = int(r.integers(0, nv)) if kind != sgraph.REQ_PR_TOPK else 10
+            tgt = int(r.integers(0, nv)) if kind in (sgraph.REQ_BFS, sgraph.REQ_SSSP) else sgraph.INF
+            mine.append((kind, arg, tgt, srv.submit(kind, arg, target=tgt)))
+        for kind, arg, tgt, t in mine:
+            out, info = srv.wait(t, kind, arg, tgt)
+            with lock:
+                results.append((kind, arg, tgt, out, info))
+
+    th = [threading.Thread(target=client, args=(i,)) for i in range(4)]
+    for x in th:
+        x.start()
+    for t in range(nsnap):
+        _, info = srv.wait(srv.submit(sgraph.REQ_UPDATE, ins=ins[t], dele=dele[t]))
+        assert info["snapshot"] == t + 1
+    for x in th:
+        x.join()
+    st = srv.stats()
+    srv.close()
+    say(f"[gpu] C5 shape: {len(results)} answers, p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms")
+    cache = {}
+
+    def oracle(kind, snap, arg):
+        key = (kind, snap, arg)
+        if key not in cache:
+            r, c = graphs[snap]
+            cache[key] = (O.bfs(r, c, arg) if kind == sgraph.REQ_BFS else O.sssp(r, c, arg, "seeded", 9)
+                          if kind == sgraph.REQ_SSSP else O.cc(r, c) if kind == sgraph.REQ_CC else O.pagerank(r, c))
+        return cache[key]
+
+    for kind, arg, tgt, out, info in results:
+        s = info["snapshot"]
+        tag = f"kind {kind} arg {arg} snapshot {s}"
+        if kind in (sgraph.REQ_BFS, sgraph.REQ_SSSP):
+            assert out[0] == oracle(kind, s, arg)[tgt], tag
+        elif kind == sgraph.REQ_CC:
+            assert out[0] == oracle(kind, s, 0)[arg], tag
+        else:
+            ids, ranks = out
+            ex = oracle(kind, s, 0)
+            np.testing.assert_allclose(ranks, ex[ids], rtol=5e-5, err_msg=tag)
+            assert np.all(ex[ids] >= np.sort(ex)[-len(ids)] * (1 - 1e-4)), tag

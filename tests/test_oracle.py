@@ -191,3 +191,100 @@ def test_segment_partition_hand_values():
     off = np.array([0, 10, 11, 12, 13], dtype=np.uint64)
     # per = ceil(13/3) = 5: vertex 0 alone fills segment 0, then 1..3 (3 slots) end the walk
     assert O.segment_partition(off, 3).tolist() == [0, 1, 4]
+
+
+# ---- certificates: pass on the oracles' own answers, fail on planted mistakes ----
+def _rand_csr(nv, ne, seed):
+    rng = np.random.default_rng(seed)
+    u = rng.integers(0, nv, ne)
+    v = (rng.pareto(1.2, ne) * 3).astype(np.int64) % nv  # skewed targets, hubs included
+    keep = u != v
+    k = np.unique(u[keep].astype(np.int64) << 32 | v[keep])
+    src, dst = (k >> 32).astype(np.int64), (k & 0xFFFFFFFF).astype(np.uint32)
+    row = np.zeros(nv + 1, dtype=np.uint64)
+    np.cumsum(np.bincount(src, minlength=nv), out=row[1:])
+    return nv, row, dst
+
+
+def _sp_parents(row, col, dist, wmode, seed):
+    nv = len(row) - 1
+    par = np.full(nv, O.INF, dtype=np.uint32)
+    for u in range(nv):
+        if dist[u] == O.INF:
+            continue
+        for e in range(int(row[u]), int(row[u + 1])):
+            v = int(col[e])
+            if par[v] == O.INF and v != 0 and dist[v] != O.INF and int(dist[u]) + O.weight(wmode, seed, u, v) == int(dist[v]):
+                par[v] = u
+    return par
+
+
+def _cc_parents(row, col, label):
+    # BFS from every root (label == own id) inside its label class
+    nv = len(row) - 1
+    par = np.full(nv, O.INF, dtype=np.uint32)
+    seen = np.zeros(nv, dtype=bool)
+    for r in range(nv):
+        if label[r] != r:
+            continue
+        seen[r] = True
+        q = [r]
+        while q:
+            u = q.pop(0)
+            for e in range(int(row[u]), int(row[u + 1])):
+                v = int(col[e])
+                if not seen[v] and label[v] == r:
+                    seen[v] = True
+                    par[v] = u
+                    q.append(v)
+    return par
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_sp_certificate_accepts_distances_and_rejects_mistakes(seed):
+    nv, row, col = _rand_csr(256, 1500, seed)
+    for wmode in ("seeded", "one"):
+        d = O.sssp(row, col, 0, wmode, 11) if wmode != "one" else O.bfs(row, col, 0)
+        p = _sp_parents(row, col, d, wmode, 11)
+        assert O.sp_certificate(row, col, 0, d, p, wmode, 11) == 0
+        reached = np.nonzero((d != O.INF) & (np.arange(nv) != 0))[0]
+        v = int(reached[len(reached) // 2])
+        for delta in (-1, +1):
+            bad = d.copy()
+            bad[v] = int(bad[v]) + delta
+            assert O.sp_certificate(row, col, 0, bad, p, wmode, 11) > 0
+        badp = p.copy()
+        badp[v] = (int(p[v]) + 1) % nv
+        assert O.sp_certificate(row, col, 0, d, badp, wmode, 11) > 0
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_cc_certificate_accepts_labels_and_rejects_mistakes(seed):
+    nv, row, col = _rand_csr(256, 500, seed)
+    lab = O.cc(row, col)
+    par = _cc_parents(row, col, lab)
+    assert O.cc_certificate(row, col, lab, par) == 0
+    movers = np.nonzero(lab != np.arange(nv))[0]
+    v = int(movers[0])
+    low = lab.copy()
+    low[v] = 0 if lab[v] != 0 else 1  # below its min ancestor (or a non-ancestor)
+    assert O.cc_certificate(row, col, low, par) > 0
+    high = lab.copy()
+    high[v] = v  # its own id: an in-edge from a smaller label now violates
+    assert O.cc_certificate(row, col, high, par) > 0
+    cyc = par.copy()
+    cyc[int(par[v])] = v  # a parent cycle
+    assert O.cc_certificate(row, col, lab, cyc) > 0
+
+
+def test_pr_residual_bounds_the_error():
+    nv, row, col = _rand_csr(512, 4000, 6)
+    exact = O.pagerank(row, col)
+    l1, mx, _ = O.pr_residual(row, col, exact.astype(np.float32))
+    assert mx < 1e-5 and l1 < 1e-3
+    rng = np.random.default_rng(7)
+    for _ in range(5):
+        r = (exact * (1 + 0.05 * rng.standard_normal(nv))).astype(np.float32)
+        l1, mx, _ = O.pr_residual(row, col, r)
+        err = float(np.abs(r.astype(np.float64) - exact).sum())
+        assert err <= l1 / (1 - 0.85) * (1 + 1e-9)
