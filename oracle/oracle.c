@@ -421,7 +421,11 @@ uint64_t or_sp_certificate(uint32_t nv, const uint64_t *row, const uint32_t *col
     if (dist[src] != 0) ++bad;
     found = (uint8_t *)calloc((size_t)(nv ? nv : 1), 1);
     if (!found) return ~0ull;
-    for (uint32_t u = 0; u < nv; ++u) {
+    /* found[v] is written only by the thread that scans v's parent's list (the
+     * one source u with parent[v] == u), so the parallel scan has no races */
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : bad)
+    for (int64_t ui = 0; ui < (int64_t)nv; ++ui) {
+        const uint32_t u = (uint32_t)ui;
         if (dist[u] == OR_INF) continue;
         for (uint64_t e = row[u]; e < row[u + 1]; ++e) {
             const uint32_t v = col[e];
@@ -448,12 +452,15 @@ uint64_t or_cc_certificate(uint32_t nv, const uint64_t *row, const uint32_t *col
     state = (uint8_t *)calloc((size_t)(nv ? nv : 1), 1);  /* chain check: 0 new, 1 on stack, 2 good, 3 bad */
     stack = (uint32_t *)malloc((size_t)(nv ? nv : 1) * sizeof(uint32_t));
     if (!ok || !state || !stack) { free(ok); free(state); free(stack); return ~0ull; }
-    for (uint32_t u = 0; u < nv; ++u)
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : bad)
+    for (int64_t ui = 0; ui < (int64_t)nv; ++ui) {  /* ok[v]: written by the scan of parent[v]'s list only */
+        const uint32_t u = (uint32_t)ui;
         for (uint64_t e = row[u]; e < row[u + 1]; ++e) {
             const uint32_t v = col[e];
             if (label[v] > label[u]) ++bad;  /* the label did not reach v along this edge */
             if (parent[v] == u) ok[v] = 1;
         }
+    }
     /* every chain v -> parent[v] -> ... stays on one label and ends at the
      * vertex label[v] (its own label, no parent); chains are walked once each */
     for (uint32_t v = 0; v < nv; ++v) {
