@@ -12,9 +12,8 @@
   min-ancestor labels and PageRank within 5e-5 relative / 1e-6 max-abs
   normalised of the FP64 fixed point at scale 20 EF32; at scale 24 EF32 and
   at the full C3 size (scale 27 EF32, 4.2 billion edges, 8 GiB budget,
-  -cache 4) the CC labels pass the min-ancestor certificate, every PageRank
-  FP64 residual is at most epsilon, and the residual bounds the L1-normalised
-  error below 1e-5.
+  -cache 4) the CC labels pass the min-ancestor certificate and PageRank's FP64
+  residual bounds the L1-normalised error below 1e-5.
 * C5 shape: the request loop at R-MAT scale 18 EF16, every answer equal to
   the oracle on the snapshot it reports.
 """
@@ -99,15 +98,16 @@ def c3_shape(scale, budget, cache, nsnap, batch, exact):
             assert O.cc_certificate(r, c, lab, par) == 0, f"G{t + 1}: CC certificate"
             say(f"[gpu] C3 shape G{t + 1}: CC certificate holds")
             ranks = e.query(0)
-            # FP64 residual of the f32 ranks: its max is the stopping rule
-            # (every |residual| <= epsilon = 1e-6) checked on the host, and
-            # ||r - r*||_1 <= ||res||_1 / (1 - alpha) bounds the L1 error of the
-            # normalised ranks (and so their max-abs error); the 1e-6 max-abs
-            # bound itself is checked against the FP64 oracle where it runs
+            # FP64 residual of the f32 ranks: ||r - r*||_1 <= ||res||_1 / (1 - alpha)
+            # bounds the L1 error of the normalised ranks (and so their max-abs
+            # error). The residual includes the ranks' own f32 rounding (a hub
+            # of rank 1e3 carries ulp 6e-5), so it is larger than the engine's
+            # stopping threshold; the 1e-6 max-abs bound itself is checked
+            # against the FP64 oracle where that runs
             l1, mx, sr = O.pr_residual(r, c, ranks, ALPHA)
             bound = l1 / (1 - ALPHA) / sr
             say(f"[gpu] C3 shape G{t + 1}: PR residual max {mx:.2e} L1 {l1:.3e} -> normalised L1 error <= {bound:.2e}")
-            assert mx <= 1.1e-6 and bound <= 1e-5
+            assert bound <= 1e-5
             if exact:
                 assert np.array_equal(lab, O.cc(r, c)), f"G{t + 1}: CC labels differ from the oracle"
                 ex = O.pagerank(r, c)
