@@ -1425,8 +1425,12 @@ public:
     // batch is the rounds it took last time plus one, so a phase whose round
     // count is steady costs one wait and at most one empty round; a phase that
     // runs longer continues with batches of 4, 8, ...
+    // limit > 0: stop after about `limit` rounds (whole batches) even if work
+    // remains (stopped_ tells which); the frontier and spills stay for the next call.
+    bool stopped_ = false;
     template <class Launch>
-    uint32_t drive_rounds(Launch launch, bool timed, int phase = -1) {
+    uint32_t drive_rounds(Launch launch, bool timed, int phase = -1, uint32_t limit = 0) {
+        stopped_ = false;
         uint32_t k0 = cfg_.rounds_per_sync ? cfg_.rounds_per_sync : 4;
         const bool adaptive = !cfg_.rounds_per_sync && phase >= 0 && last_rounds_[phase];
         if (adaptive) k0 = last_rounds_[phase] + 1;
@@ -1441,6 +1445,7 @@ public:
         std::vector<GraphLaunch> batch;
         for (;;) {
             const uint32_t R0 = R_;
+            if (limit) k = std::max<uint32_t>(1, std::min<uint32_t>(k, limit - std::min(limit - 1, rounds)));
             if (graphs) {
                 batch.clear();
                 batch_ = &batch;
@@ -1482,6 +1487,11 @@ public:
                     if (phase >= 0) last_rounds_[phase] = rounds;
                     return rounds;
                 }
+            }
+            if (limit && rounds >= limit) {
+                finish_rounds(timed);
+                stopped_ = true;
+                return rounds;
             }
             if (rounds >= maxr) {
                 finish_rounds(timed);
@@ -1873,6 +1883,16 @@ public:
     }
     virtual void gather_query(uint32_t, uint32_t, void *, size_t) {
         fail(SG_EINVAL, "sg_gather_query needs an engine created with SG_ALGO_MULTI");
+    }
+    virtual void open_slots(uint32_t) { fail(SG_EINVAL, "query slots need an engine created with SG_ALGO_MULTI"); }
+    virtual void add_queries(const uint32_t *, const uint32_t *, uint32_t, uint32_t *) {
+        fail(SG_EINVAL, "query slots need an engine created with SG_ALGO_MULTI");
+    }
+    virtual void step_queries(uint32_t, uint32_t *, uint32_t, uint32_t *, uint32_t *) {
+        fail(SG_EINVAL, "query slots need an engine created with SG_ALGO_MULTI");
+    }
+    virtual void release_queries(const uint32_t *, uint32_t) {
+        fail(SG_EINVAL, "query slots need an engine created with SG_ALGO_MULTI");
     }
     // Where the 32-bit result word of (query, qi) for vertex v lives:
     // base[v * stride + word].
@@ -2848,9 +2868,145 @@ public:
     }
 
     template <int kMode>
-    uint32_t mrounds(bool timed, int phase = -1) {
+    uint32_t mrounds(bool timed, int phase = -1, uint32_t limit = 0) {
         const bool do_time = timed && timing(SG_KCLASS_RELAX);
-        return drive_rounds([&](const PassIO &io) { mround<kMode>(io, do_time); }, timed, phase);
+        return drive_rounds([&](const PassIO &io) { mround<kMode>(io, do_time); }, timed, phase, limit);
+    }
+
+    // Slot layout for nq queries: groups of QS slots (QS = 64, or the next
+    // power of two >= nq below 64); the state only grows.
+    void layout(uint32_t nq) {
+        QS_ = 1;
+        while (QS_ < std::min<uint32_t>(nq, 64)) QS_ <<= 1;
+        qs_log_ = 0;
+        while ((1u << qs_log_) < QS_) ++qs_log_;
+        G_ = (nq + QS_ - 1) / QS_;
+        KP_ = G_ * QS_;
+        if (state_nv_ != nv_ || state_kp_ < KP_) {
+            dfree(best_); dfree(act_); dfree(minv_); dfree(ninv_); dfree(kind_); dfree(src_);
+            best_ = act_ = minv_ = ninv_ = nullptr;
+            kind_ = nullptr;
+            src_ = nullptr;
+            const uint32_t G = (KP_ + 63) / 64 > G_ ? (KP_ + 63) / 64 : G_;
+            best_ = dalloc<unsigned long long>((size_t)nv_ * KP_);
+            act_ = dalloc<unsigned long long>((size_t)nv_ * G);
+            minv_ = dalloc<unsigned long long>((size_t)nv_ * G);
+            ninv_ = dalloc<unsigned long long>((size_t)nv_ * G);
+            kind_ = dalloc<uint8_t>(KP_);
+            src_ = dalloc<uint32_t>(std::max<uint32_t>(KP_, 2 * 4096));
+            state_nv_ = nv_;
+            state_kp_ = KP_;
+        }
+    }
+
+    // ---- continuous batching (sg_query_open / _add / _step / _release) ----
+    // A fixed set of KP slots; queries join free slots between round batches
+    // (their sources seeded into the frontier the next round reads), run
+    // together, and leave when no work is pending for them.
+    bool cont_ = false;
+    std::vector<uint8_t> live_, done_, h_kind_;
+    unsigned long long *pend_ = nullptr;
+    size_t pend_cap_ = 0;
+
+    void open_slots(uint32_t k) override {
+        if (!loaded_) fail(SG_EINVAL, "sg_query_open before sg_load_csr");
+        if (k == 0 || k > 4096) fail(SG_EINVAL, "slots must be in [1, 4096]");
+        layout(k);
+        nq_ = KP_;
+        h_kind_.assign(KP_, kQNone);
+        live_.assign(KP_, 0);
+        done_.assign(KP_, 0);
+        SG_HIP(hipMemcpyAsync(kind_, h_kind_.data(), KP_, hipMemcpyHostToDevice, stream_));
+        fill<unsigned long long>(best_, (uint64_t)nv_ * KP_, ~0ull);
+        fill<unsigned long long>(act_, (uint64_t)nv_ * G_, 0ull);
+        fill<unsigned long long>(minv_, (uint64_t)nv_ * G_, 0ull);
+        fill<unsigned long long>(ninv_, (uint64_t)nv_ * G_, 0ull);
+        fill<uint32_t>(inval_, ((uint64_t)nv_ + 31) / 32, 0u);
+        fill<uint32_t>(coarse_, coarse_words(nv_, cshift_), 0u);
+        ensure(pend_, pend_cap_, G_);
+        nins_ = 0;
+        ensure_mspill();
+        frontier_reset();
+        sync();
+        cont_ = true;
+        inited_ = true;
+    }
+    uint32_t live_count(bool only_running) const {
+        uint32_t n = 0;
+        for (size_t i = 0; i < live_.size(); ++i) n += live_[i] && !(only_running && done_[i]);
+        return n;
+    }
+    void add_queries(const uint32_t *sources, const uint32_t *kinds, uint32_t n, uint32_t *slots) override {
+        if (!cont_) fail(SG_EINVAL, "sg_query_add needs sg_query_open first");
+        if (n == 0) return;
+        if (!sources || !kinds || !slots) fail(SG_EINVAL, "null sources, kinds or slots");
+        std::vector<uint32_t> hs, hsrc;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (sources[i] >= nv_) fail(SG_EINVAL, "query " + std::to_string(i) + ": source id >= nv");
+            if (kinds[i] != SG_ALGO_SSSP && kinds[i] != SG_ALGO_BFS)
+                fail(SG_EINVAL, "query " + std::to_string(i) + ": kind must be SG_ALGO_SSSP or SG_ALGO_BFS");
+        }
+        for (uint32_t q = 0; q < KP_ && hs.size() < n; ++q)
+            if (!live_[q]) hs.push_back(q);
+        if (hs.size() < n) fail(SG_EINVAL, "not enough free query slots");
+        for (uint32_t i = 0; i < n; ++i) {
+            h_kind_[hs[i]] = kinds[i] == SG_ALGO_BFS ? kQUnit : kQWeighted;
+            hsrc.push_back(sources[i]);
+        }
+        uint32_t *d = src_;  // n slots then n sources (n <= KP <= 4096)
+        SG_HIP(hipMemcpyAsync(kind_, h_kind_.data(), KP_, hipMemcpyHostToDevice, stream_));
+        SG_HIP(hipMemcpyAsync(d, hs.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream_));
+        SG_HIP(hipMemcpyAsync(d + n, hsrc.data(), (size_t)n * 4, hipMemcpyHostToDevice, stream_));
+        hipLaunchKernelGGL(k_slot_reset, dim3(grid_for(nv_, 256, 16384)), dim3(256), 0, stream_, mstate(), nv_,
+                           (const uint32_t *)d, n);
+        SG_LAUNCHED();
+        hipLaunchKernelGGL(k_slot_seed, dim3(grid_for(n, 256, 64)), dim3(256), 0, stream_, mstate(), (const uint32_t *)d,
+                           (const uint32_t *)(d + n), n, stamp_, R_, q_[R_ & 1], cnt_ + (R_ % 3), nv_, stats_ + 3);
+        SG_LAUNCHED();
+        sync();  // the host vectors above were read by the copies
+        for (uint32_t i = 0; i < n; ++i) {
+            live_[hs[i]] = 1;
+            done_[hs[i]] = 0;
+            slots[i] = hs[i];
+        }
+    }
+    // Up to max_rounds rounds, then the slots whose work is done (converged
+    // since the last call) in done[0..*ndone); *nactive = slots still running.
+    void step_queries(uint32_t max_rounds, uint32_t *done, uint32_t cap, uint32_t *ndone, uint32_t *nactive) override {
+        if (!cont_) fail(SG_EINVAL, "sg_query_step needs sg_query_open first");
+        *ndone = 0;
+        if (live_count(true)) mrounds<0>(false, -1, std::max<uint32_t>(1, max_rounds));
+        fill<unsigned long long>(pend_, G_, 0ull);
+        const uint32_t R = R_;
+        hipLaunchKernelGGL(k_slot_pending, dim3(64), dim3(kBlock), 0, stream_, mstate(), (const uint32_t *)q_[R & 1],
+                           (const uint32_t *)(cnt_ + (R % 3)), nv_, (const MDesc *)mspill_[(R + 2) % 3],
+                           (const uint32_t *)(hcnt_ + ((R + 2) % 3)), (uint32_t)std::min<size_t>(mspill_cap_, 0xFFFFFFFFu),
+                           pend_);
+        SG_LAUNCHED();
+        std::vector<unsigned long long> pend(G_);
+        SG_HIP(hipMemcpyAsync(pend.data(), pend_, (size_t)G_ * 8, hipMemcpyDeviceToHost, stream_));
+        sync();
+        uint32_t act = 0;
+        for (uint32_t q = 0; q < KP_; ++q) {
+            if (!live_[q] || done_[q]) continue;
+            if ((pend[q / QS_] >> (q % QS_)) & 1ull) { ++act; continue; }
+            done_[q] = 1;
+            if (*ndone < cap) done[(*ndone)++] = q;
+        }
+        *nactive = act;
+    }
+    void release_queries(const uint32_t *slots, uint32_t n) override {
+        if (!cont_) fail(SG_EINVAL, "sg_query_release needs sg_query_open first");
+        for (uint32_t i = 0; i < n; ++i) {
+            if (slots[i] >= KP_ || !live_[slots[i]]) fail(SG_EINVAL, "slot " + std::to_string(slots[i]) + " holds no query");
+            if (!done_[slots[i]]) fail(SG_EINVAL, "slot " + std::to_string(slots[i]) + " is still converging");
+        }
+        for (uint32_t i = 0; i < n; ++i) {
+            live_[slots[i]] = done_[slots[i]] = 0;
+            h_kind_[slots[i]] = kQNone;
+        }
+        SG_HIP(hipMemcpyAsync(kind_, h_kind_.data(), KP_, hipMemcpyHostToDevice, stream_));
+        sync();
     }
 
     template <int kMode>
@@ -2872,6 +3028,7 @@ public:
         if (!loaded_) fail(SG_EINVAL, "sg_init before sg_load_csr");
         if (nq == 0) {  // drop the queries: later snapshots only update the graph
             nq_ = 0;
+            cont_ = false;
             inited_ = true;
             return;
         }
@@ -2883,29 +3040,9 @@ public:
                 fail(SG_EINVAL, "query " + std::to_string(i) + ": kind must be SG_ALGO_SSSP or SG_ALGO_BFS");
         }
         nq_ = nq;
-        QS_ = 1;
-        while (QS_ < std::min<uint32_t>(nq, 64)) QS_ <<= 1;
-        qs_log_ = 0;
-        while ((1u << qs_log_) < QS_) ++qs_log_;
-        G_ = (nq + QS_ - 1) / QS_;
-        KP_ = G_ * QS_;
-        // state for KP slots; the allocation only grows (a request loop changes
-        // the query count every step), a smaller KP uses a prefix of it
-        if (state_nv_ != nv_ || state_kp_ < KP_) {
-            dfree(best_); dfree(act_); dfree(minv_); dfree(ninv_); dfree(kind_); dfree(src_);
-            best_ = act_ = minv_ = ninv_ = nullptr;
-            kind_ = nullptr;
-            src_ = nullptr;
-            const uint32_t G = (KP_ + 63) / 64 > G_ ? (KP_ + 63) / 64 : G_;
-            best_ = dalloc<unsigned long long>((size_t)nv_ * KP_);
-            act_ = dalloc<unsigned long long>((size_t)nv_ * G);
-            minv_ = dalloc<unsigned long long>((size_t)nv_ * G);
-            ninv_ = dalloc<unsigned long long>((size_t)nv_ * G);
-            kind_ = dalloc<uint8_t>(KP_);
-            src_ = dalloc<uint32_t>(KP_);
-            state_nv_ = nv_;
-            state_kp_ = KP_;
-        }
+        cont_ = false;
+        // state for KP slots; the allocation only grows, a smaller KP uses a prefix of it
+        layout(nq);
         std::vector<uint8_t> hk(KP_, kQNone);
         for (uint32_t i = 0; i < nq; ++i) hk[i] = kinds[i] == SG_ALGO_BFS ? kQUnit : kQWeighted;
         auto t0 = std::chrono::steady_clock::now();
@@ -2933,9 +3070,11 @@ public:
     void apply(const uint32_t *d_ins, uint64_t nins, const uint32_t *d_del, uint64_t ndel) override {
         if (!inited_) fail(SG_EINVAL, "sg_apply before sg_init");
         auto t0 = std::chrono::steady_clock::now();
+        if (cont_ && live_count(true))
+            fail(SG_EINVAL, "queries are still converging: step them to convergence before an update");
         const bool overflow = prepare(d_ins, nins, d_del, ndel);
         commit(overflow);
-        if (nq_ == 0) {  // no queries: the PMA update is the whole snapshot
+        if (nq_ == 0 || (cont_ && live_count(false) == 0)) {  // no queries: the PMA update is the whole snapshot
             st_.invalidated = 0;
             st_.rounds = 0;
             finish_batch();
@@ -2979,6 +3118,7 @@ public:
     void gather_query(uint32_t qi, uint32_t what, void *out, size_t bytes) override {
         if (!inited_) fail(SG_EINVAL, "no queries before sg_init_queries");
         if (qi >= nq_) fail(SG_EINVAL, "query index out of range");
+        if (cont_ && !live_[qi]) fail(SG_EINVAL, "slot holds no query");
         if (what != SG_Q_VALUE && what != SG_Q_PARENT) fail(SG_EINVAL, "sg_gather_query returns SG_Q_VALUE or SG_Q_PARENT");
         if (bytes < (size_t)nv_ * 4) fail(SG_EINVAL, "output too small");
         ensure(tmp_, tmp_cap_, nv_);
@@ -2992,6 +3132,7 @@ public:
     void word_source(uint32_t q, uint32_t qi, const uint32_t *&base, uint64_t &stride, uint32_t &word) override {
         if (q == SG_Q_VALUE || q == SG_Q_PARENT) {
             if (qi >= nq_) fail(SG_EINVAL, "query index out of range");
+            if (cont_ && !live_[qi]) fail(SG_EINVAL, "slot holds no query");
             base = (const uint32_t *)best_;
             stride = 2ull * KP_;
             word = 2 * qi + (q == SG_Q_VALUE ? 1 : 0);
@@ -3268,6 +3409,29 @@ sg_status sg_restore_state(sg_engine *eng, const char *path) {
 
 sg_status sg_init_queries(sg_engine *eng, const uint32_t *sources, const uint32_t *kinds, uint32_t nq) {
     return guarded(eng, [&](sg::Engine &e) { e.init_queries(sources, kinds, nq); });
+}
+
+sg_status sg_query_open(sg_engine *eng, uint32_t slots) {
+    return guarded(eng, [&](sg::Engine &e) { e.open_slots(slots); });
+}
+
+sg_status sg_query_add(sg_engine *eng, const uint32_t *sources, const uint32_t *kinds, uint32_t n, uint32_t *slots) {
+    return guarded(eng, [&](sg::Engine &e) { e.add_queries(sources, kinds, n, slots); });
+}
+
+sg_status sg_query_step(sg_engine *eng, uint32_t max_rounds, uint32_t *done, uint32_t cap, uint32_t *ndone,
+                        uint32_t *nactive) {
+    return guarded(eng, [&](sg::Engine &e) {
+        if (!ndone || !nactive || (cap && !done)) sg::fail(SG_EINVAL, "null output");
+        e.step_queries(max_rounds, done, cap, ndone, nactive);
+    });
+}
+
+sg_status sg_query_release(sg_engine *eng, const uint32_t *slots, uint32_t n) {
+    return guarded(eng, [&](sg::Engine &e) {
+        if (n && !slots) sg::fail(SG_EINVAL, "null slot list");
+        e.release_queries(slots, n);
+    });
 }
 
 sg_status sg_gather_query(sg_engine *eng, uint32_t qi, uint32_t what, void *out, size_t bytes) {

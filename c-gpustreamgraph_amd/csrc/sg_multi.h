@@ -759,6 +759,51 @@ __global__ void k_minit_sources(MultiState ms, const uint32_t *src, uint32_t nq,
     }
 }
 
+// ---- continuous batching: queries join and leave fixed slots between round batches ----
+// New slots: every vertex's word unreached, no pending or reset bits.
+__global__ void k_slot_reset(MultiState ms, uint32_t nv, const uint32_t *slots, uint32_t n) {
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t s = slots[i], grp = s / ms.QS;
+            const unsigned long long keep = ~(1ull << (s % ms.QS));
+            ms.best[v * ms.KP + s] = ~0ull;
+            if (ms.act[v * ms.G + grp] & ~keep) ms.act[v * ms.G + grp] &= keep;
+        }
+    }
+}
+// Each new query's source at distance 0 and active, appended once to the
+// frontier the next round reads (stamp_val = that round).
+__global__ void k_slot_seed(MultiState ms, const uint32_t *slots, const uint32_t *src, uint32_t n, uint32_t *stamp,
+                            uint32_t stamp_val, uint32_t *list, uint32_t *cnt, uint32_t cap, unsigned long long *overflow) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t s = src[i], q = slots[i], grp = q / ms.QS, slot = q % ms.QS;
+        ms.best[(uint64_t)s * ms.KP + q] = pack(0, kNone);
+        atomicOr(&ms.act[(uint64_t)s * ms.G + grp], 1ull << slot);
+        if (atomicExch(&stamp[s], stamp_val) != stamp_val) {
+            const uint32_t pos = atomicAdd(cnt, 1u);
+            if (pos < cap) list[pos] = s;
+            else atomicOr(overflow, 1ull);
+        }
+    }
+}
+// Slots with work still pending (per group): the masks of the frontier the
+// next round reads and of the descriptors the last round spilled.
+__global__ void __launch_bounds__(kBlock) k_slot_pending(MultiState ms, const uint32_t *list, const uint32_t *cnt,
+                                                         uint32_t cap, const MDesc *spill, const uint32_t *scnt,
+                                                         uint32_t scap, unsigned long long *out) {
+    const uint32_t n = min(*cnt, cap), ns = min(*scnt, scap);
+    for (uint32_t grp = 0; grp < ms.G; ++grp) {
+        unsigned long long m = 0;
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+            m |= ms.act[(uint64_t)list[i] * ms.G + grp];
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += (uint64_t)gridDim.x * blockDim.x)
+            if (spill[i].grp == grp) m |= spill[i].mask;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) m |= __shfl_xor(m, d, kWave);
+        if (lane_id() == 0 && m) atomicOr(&out[grp], m);
+    }
+}
+
 // Column q of the state: distance (what = 0) or parent (what = 1).
 __global__ void k_mextract(uint32_t nv, const unsigned long long *best, uint32_t KP, uint32_t q, uint32_t *out, int what) {
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (uint64_t)gridDim.x * blockDim.x) {

@@ -31,6 +31,7 @@ struct Request {
     sg_status status = SG_OK;
     std::string error;
     bool done = false;
+    bool claimed = false;  // a thread waits for it (one waiter per ticket)
     Clock::time_point t_submit;
 };
 
@@ -59,8 +60,7 @@ struct sg_server {
     sg_server_config cfg{};
     uint32_t nv = 0;
     sg_engine *multi = nullptr, *cc = nullptr, *pr = nullptr;
-    uint32_t slots = 64;
-    uint32_t kp_allocated = 0;  // query slots the batched engine already holds state for
+    uint32_t slots = 0;  // query slots opened on the batched engine (0: no HBM for them)
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -69,6 +69,11 @@ struct sg_server {
     uint64_t next_id = 1;
     bool stop = false;
     std::thread worker;
+
+    // After an engine fails to apply a snapshot the engines may hold different
+    // graphs: every later request fails with the saved error.
+    sg_status failed_status = SG_OK;
+    std::string failed;
 
     // PageRank thread: applies the worker's snapshots in order with deferred
     // convergence and answers top-k requests on the snapshot it has reached
@@ -80,6 +85,9 @@ struct sg_server {
     bool pr_stop = false;
     uint64_t pr_snapshot = 0;
     std::string pr_error;
+
+    // queries in the batched engine's slots (worker thread only)
+    std::unordered_map<uint32_t, uint64_t> running;  // slot -> request id
 
     uint64_t snapshot = 0, step = 0;
     std::vector<double> lat_all, lat_q, lat_u;
@@ -93,8 +101,14 @@ struct sg_server {
 
     std::string engine_error(sg_engine *e) { return std::string(sg_last_error(e)); }
 
+    Request *req(uint64_t id) {
+        std::lock_guard<std::mutex> lk(mu);
+        return reqs[id].get();
+    }
+
     // Complete requests (under the lock) and wake waiters.
     void complete(const std::vector<std::pair<uint64_t, Request *>> &done) {
+        if (done.empty()) return;
         const auto now = Clock::now();
         std::lock_guard<std::mutex> lk(mu);
         for (auto &d : done) {
@@ -109,10 +123,10 @@ struct sg_server {
         cv_done.notify_all();
     }
 
-    // How many queries the batched engine may take now: the state of K slots
-    // is about nv * K * 8 + 3 * nv * ceil(K / 64) * 8 bytes; only growth
-    // beyond what it already holds needs free HBM.
-    uint32_t admissible(uint32_t want) {
+    // Query slots the free HBM allows beside what the engines hold, keeping
+    // cfg.hbm_reserve_gb free: a slot's state is about nv * 8 bytes plus three
+    // 64-bit masks per vertex per 64 slots.
+    uint32_t affordable_slots(uint32_t want) {
         uint64_t fr = 0, tot = 0;
         if (sg_device_memory(cfg.engine.device, &fr, &tot) != SG_OK) return want;
         const double reserve = cfg.hbm_reserve_gb * 1073741824.0;
@@ -123,52 +137,142 @@ struct sg_server {
             return (double)nv * (double)kp * 8.0 + 3.0 * (double)nv * (double)g * 8.0;
         };
         uint32_t k = want;
-        while (k > 0) {
-            const double grow = bytes(k) > bytes(kp_allocated) ? bytes(k) : 0.0;  // a new allocation replaces the old
-            if (grow == 0.0 || (double)fr + (kp_allocated ? bytes(kp_allocated) : 0.0) - grow >= reserve) break;
-            k /= 2;
-        }
+        while (k > 0 && (double)fr - bytes(k) < reserve) k /= 2;
         return k;
     }
 
-    void run_step(std::vector<uint64_t> &U, std::vector<uint64_t> &Q, std::vector<uint64_t> &L) {
-        std::vector<std::pair<uint64_t, Request *>> done;
-        auto req = [&](uint64_t id) {
-            std::lock_guard<std::mutex> lk(mu);
-            return reqs[id].get();
-        };
-        // 1. updates, in submission order, one snapshot each, to every engine
-        for (uint64_t id : U) {
-            Request *r = req(id);
-            const uint64_t ni = r->ins.size() / 2, nd = r->del.size() / 2;
-            for (sg_engine *e : {multi, cc}) {
-                if (!e) continue;
-                const sg_status s = sg_apply_snapshot(e, r->ins.data(), ni, r->del.data(), nd);
-                if (s != SG_OK) { fail_request(r, s, engine_error(e)); break; }
+    // Answer the queries whose slots converged, then free the slots.
+    void finish_slots(const uint32_t *done, uint32_t n, uint64_t ne) {
+        std::vector<std::pair<uint64_t, Request *>> out;
+        std::vector<uint32_t> rel;
+        for (uint32_t i = 0; i < n; ++i) {
+            auto it = running.find(done[i]);
+            if (it == running.end()) continue;
+            Request *r = req(it->second);
+            sg_status g;
+            if (r->target == 0xFFFFFFFFu) {
+                r->result.resize((size_t)nv * 4);
+                g = sg_gather_query(multi, done[i], SG_Q_VALUE, r->result.data(), r->result.size());
+            } else {
+                r->result.resize(4);
+                g = sg_gather_at(multi, SG_Q_VALUE, done[i], &r->target, 1, (uint32_t *)r->result.data());
             }
-            ++snapshot;
-            st.edge_updates += ni + nd;
+            if (g != SG_OK) fail_request(r, g, engine_error(multi));
             r->info.snapshot = snapshot;
-            if (pr) {
-                PrUpdate pu;
-                pu.snapshot = snapshot;
-                pu.ins = std::move(r->ins);
-                pu.del = std::move(r->del);
-                std::lock_guard<std::mutex> lk(mu);
-                q_pr.push_back(std::move(pu));
-                cv_pr.notify_one();
-            }
-            r->ins.clear();
-            r->ins.shrink_to_fit();
-            r->del.clear();
-            r->del.shrink_to_fit();
-            done.push_back({id, r});
+            out.push_back({it->second, r});
+            rel.push_back(done[i]);
+            running.erase(it);
         }
-        st.snapshots = snapshot;
+        if (!rel.empty() && sg_query_release(multi, rel.data(), (uint32_t)rel.size()) != SG_OK)
+            set_failed(SG_EDEVICE, engine_error(multi));
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            st.queries += out.size();
+            st.edges_answered += ne * out.size();
+        }
+        complete(out);
+    }
+
+    // Rounds for the running queries (all of them when `drain`), answering
+    // each as it converges.
+    void run_queries(bool drain, uint64_t ne) {
+        std::vector<uint32_t> done(std::max<uint32_t>(slots, 1));
+        do {
+            if (running.empty()) return;
+            uint32_t nd = 0, na = 0;
+            if (sg_query_step(multi, drain ? 64 : kRoundsPerStep, done.data(), (uint32_t)done.size(), &nd, &na) != SG_OK) {
+                set_failed(SG_EDEVICE, engine_error(multi));
+                fail_running();
+                return;
+            }
+            finish_slots(done.data(), nd, ne);
+        } while (drain);
+    }
+    static constexpr uint32_t kRoundsPerStep = 8;  // rounds between admissions
+
+    void set_failed(sg_status code, const std::string &msg) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (failed_status == SG_OK) {
+            failed_status = code;
+            failed = msg;
+        }
+    }
+    void fail_running() {
+        std::vector<std::pair<uint64_t, Request *>> out;
+        for (auto &kv : running) {
+            Request *r = req(kv.second);
+            fail_request(r, failed_status, failed);
+            out.push_back({kv.second, r});
+        }
+        running.clear();
+        complete(out);
+    }
+
+    uint64_t graph_edges() {
         sg_stats es{};
-        uint64_t ne = 0;
-        if (sg_get_stats(multi, &es) == SG_OK) ne = es.ne;
-        // 2. CC label lookups on the current snapshot
+        return sg_get_stats(multi, &es) == SG_OK ? es.ne : 0;
+    }
+
+    // One step of the worker: update batches (at a point where no query is
+    // converging), CC lookups, admission of waiting queries into free slots,
+    // and a few rounds for every running query.
+    void run_step(std::vector<uint64_t> &U, std::vector<uint64_t> &L) {
+        std::vector<std::pair<uint64_t, Request *>> done;
+        uint64_t ne = graph_edges();
+        if (!U.empty()) {
+            run_queries(true, ne);  // running queries finish on the snapshot they started on
+            for (uint64_t id : U) {
+                Request *r = req(id);
+                const uint64_t ni = r->ins.size() / 2, nd = r->del.size() / 2;
+                sg_status fs;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    fs = failed_status;
+                }
+                if (fs != SG_OK) {
+                    fail_request(r, fs, failed);
+                    done.push_back({id, r});
+                    continue;
+                }
+                for (sg_engine *e : {multi, cc}) {
+                    if (!e) continue;
+                    const sg_status s = sg_apply_snapshot(e, r->ins.data(), ni, r->del.data(), nd);
+                    if (s != SG_OK) {
+                        // this engine (and the ones after it) missed the batch: no
+                        // later answer can name one snapshot for every program
+                        set_failed(s, engine_error(e));
+                        fail_request(r, s, engine_error(e));
+                        break;
+                    }
+                }
+                if (r->status == SG_OK) {
+                    ++snapshot;
+                    r->info.snapshot = snapshot;
+                    if (pr) {
+                        PrUpdate pu;
+                        pu.snapshot = snapshot;
+                        pu.ins = std::move(r->ins);
+                        pu.del = std::move(r->del);
+                        std::lock_guard<std::mutex> lk(mu);
+                        q_pr.push_back(std::move(pu));
+                        cv_pr.notify_one();
+                    }
+                    std::lock_guard<std::mutex> lk(mu);
+                    st.edge_updates += ni + nd;
+                }
+                r->ins.clear();
+                r->ins.shrink_to_fit();
+                r->del.clear();
+                r->del.shrink_to_fit();
+                done.push_back({id, r});
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                st.snapshots = snapshot;
+            }
+            ne = graph_edges();
+        }
+        // CC label lookups on the current snapshot
         std::vector<uint32_t> verts;
         std::vector<Request *> ccr;
         for (uint64_t id : L) {
@@ -180,53 +284,96 @@ struct sg_server {
         }
         if (!ccr.empty()) {
             std::vector<uint32_t> out(verts.size());
-            const sg_status s = sg_gather_at(cc, SG_Q_VALUE, 0, verts.data(), verts.size(), out.data());
+            sg_status fs;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                fs = failed_status;
+            }
+            const sg_status s = fs != SG_OK ? fs : sg_gather_at(cc, SG_Q_VALUE, 0, verts.data(), verts.size(), out.data());
             for (size_t i = 0; i < ccr.size(); ++i) {
-                if (s != SG_OK) { fail_request(ccr[i], s, engine_error(cc)); continue; }
+                if (s != SG_OK) { fail_request(ccr[i], s, fs != SG_OK ? failed : engine_error(cc)); continue; }
                 ccr[i]->result.resize(4);
                 std::memcpy(ccr[i]->result.data(), &out[i], 4);
             }
-        }
-        {
             std::lock_guard<std::mutex> lk(mu);
             st.lookups += L.size();
         }
-        // 3. queries admitted this step, converged together
+        complete(done);
+        // admission: waiting queries into free slots, as the free HBM allows
+        std::vector<uint64_t> Q;
+        uint32_t deferred = 0;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            const uint32_t free_slots = slots - (uint32_t)running.size();
+            const uint32_t want = (uint32_t)std::min<size_t>(q_queries.size(), free_slots);
+            uint32_t k = want;
+            if (want && reserve_short()) k = 0;
+            deferred = want - k;
+            for (uint32_t i = 0; i < k; ++i) {
+                Q.push_back(q_queries.front());
+                q_queries.pop_front();
+            }
+            st.deferred_for_memory += deferred;
+            if (failed_status != SG_OK || slots == 0 || (deferred && running.empty())) {
+                // nothing can run them: a failed server, no slots at all, or HBM
+                // short with no running query whose end would free any
+                std::vector<uint64_t> all(q_queries.begin(), q_queries.end());
+                q_queries.clear();
+                for (uint64_t id : all) Q.push_back(id);
+                if (failed_status == SG_OK) st.deferred_for_memory += all.size();  // refused for memory
+                refuse_ = true;
+            }
+        }
+        if (refuse_) {
+            std::vector<std::pair<uint64_t, Request *>> out;
+            sg_status code;
+            std::string msg;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                code = failed_status != SG_OK ? failed_status : SG_EBUDGET;
+                msg = failed_status != SG_OK ? failed
+                                             : "no device memory for query state under the HBM reserve (hbm_reserve_gb)";
+            }
+            for (uint64_t id : Q) {
+                Request *r = req(id);
+                r->info.snapshot = snapshot;
+                fail_request(r, code, msg);
+                out.push_back({id, r});
+            }
+            refuse_ = false;
+            complete(out);
+            Q.clear();
+        }
         if (!Q.empty()) {
-            std::vector<uint32_t> src, kinds;
-            std::vector<Request *> qr;
+            std::vector<uint32_t> src, kinds, slot(Q.size());
             for (uint64_t id : Q) {
                 Request *r = req(id);
                 src.push_back(r->arg);
                 kinds.push_back(r->kind == SG_REQ_BFS ? (uint32_t)SG_ALGO_BFS : (uint32_t)SG_ALGO_SSSP);
-                qr.push_back(r);
             }
-            const sg_status s = sg_init_queries(multi, src.data(), kinds.data(), (uint32_t)src.size());
-            kp_allocated = std::max<uint32_t>(kp_allocated, (uint32_t)src.size());
-            for (size_t i = 0; i < qr.size(); ++i) {
-                Request *r = qr[i];
-                r->info.snapshot = snapshot;
-                if (s != SG_OK) { fail_request(r, s, engine_error(multi)); continue; }
-                sg_status g;
-                if (r->target == 0xFFFFFFFFu) {
-                    r->result.resize((size_t)nv * 4);
-                    g = sg_gather_query(multi, (uint32_t)i, SG_Q_VALUE, r->result.data(), r->result.size());
-                } else {
-                    r->result.resize(4);
-                    g = sg_gather_at(multi, SG_Q_VALUE, (uint32_t)i, &r->target, 1, (uint32_t *)r->result.data());
+            const sg_status s = sg_query_add(multi, src.data(), kinds.data(), (uint32_t)src.size(), slot.data());
+            if (s != SG_OK) {
+                std::vector<std::pair<uint64_t, Request *>> out;
+                for (uint64_t id : Q) {
+                    Request *r = req(id);
+                    fail_request(r, s, engine_error(multi));
+                    out.push_back({id, r});
                 }
-                if (g != SG_OK) fail_request(r, g, engine_error(multi));
+                complete(out);
+            } else {
+                for (size_t i = 0; i < Q.size(); ++i) running[slot[i]] = Q[i];
+                std::lock_guard<std::mutex> lk(mu);
+                st.max_batch = std::max<uint64_t>(st.max_batch, running.size());
             }
-            // the answers are copied out: drop the queries so that later
-            // snapshots do not maintain them (dropping makes no device call
-            // and cannot fail on a loaded engine)
-            if (s == SG_OK) (void)sg_init_queries(multi, nullptr, nullptr, 0);
-            for (size_t i = 0; i < Q.size(); ++i) done.push_back({Q[i], qr[i]});
-            st.queries += Q.size();
-            st.edges_answered += ne * Q.size();
-            st.max_batch = std::max<uint64_t>(st.max_batch, Q.size());
         }
-        complete(done);
+        run_queries(false, ne);
+    }
+    bool refuse_ = false;
+    // Free HBM below the reserve (the other engines' batch scratch can grow).
+    bool reserve_short() {
+        uint64_t fr = 0, tot = 0;
+        if (sg_device_memory(cfg.engine.device, &fr, &tot) != SG_OK) return false;
+        return (double)fr < cfg.hbm_reserve_gb * 1073741824.0;
     }
 
     void pr_loop() {
@@ -245,8 +392,12 @@ struct sg_server {
             for (auto &u : U) {
                 if (!pr_error.empty()) break;
                 const sg_status s = sg_apply_snapshot(pr, u.ins.data(), u.ins.size() / 2, u.del.data(), u.del.size() / 2);
-                if (s != SG_OK) pr_error = engine_error(pr);
-                else pr_snapshot = u.snapshot;
+                if (s != SG_OK) {
+                    pr_error = engine_error(pr);
+                    set_failed(s, pr_error);
+                } else {
+                    pr_snapshot = u.snapshot;
+                }
             }
             if (K.empty()) continue;
             std::vector<std::pair<uint64_t, Request *>> done;
@@ -264,7 +415,10 @@ struct sg_server {
             sg_status s = SG_EDEVICE;
             if (pr_error.empty()) {
                 s = sg_pr_topk(pr, kmax, ids.data(), rk.data());  // converges the deferred snapshots
-                if (s != SG_OK) pr_error = engine_error(pr);
+                if (s != SG_OK) {
+                    pr_error = engine_error(pr);
+                    set_failed(s, pr_error);
+                }
             }
             for (auto &d : done) {
                 Request *r = d.second;
@@ -285,32 +439,23 @@ struct sg_server {
 
     void loop() {
         for (;;) {
-            std::vector<uint64_t> U, Q, L;
-            uint32_t deferred = 0;
+            std::vector<uint64_t> U, L;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] { return stop || !q_updates.empty() || !q_queries.empty() || !q_lookups.empty(); });
-                if (stop && q_updates.empty() && q_queries.empty() && q_lookups.empty()) break;
+                cv_work.wait(lk, [&] {
+                    return stop || !q_updates.empty() || !q_queries.empty() || !q_lookups.empty() || !running.empty();
+                });
+                if (stop && q_updates.empty() && q_queries.empty() && q_lookups.empty() && running.empty()) break;
                 U.assign(q_updates.begin(), q_updates.end());
                 q_updates.clear();
                 L.assign(q_lookups.begin(), q_lookups.end());
                 q_lookups.clear();
-                const uint32_t want = (uint32_t)std::min<size_t>(q_queries.size(), slots);
-                lk.unlock();
-                const uint32_t k = want ? std::max<uint32_t>(1, admissible(want)) : 0;
-                lk.lock();
-                deferred = want - k;
-                for (uint32_t i = 0; i < k && !q_queries.empty(); ++i) {
-                    Q.push_back(q_queries.front());
-                    q_queries.pop_front();
-                }
                 ++step;
             }
             const auto t0 = Clock::now();
-            run_step(U, Q, L);
+            run_step(U, L);
             std::lock_guard<std::mutex> lk(mu);
             st.busy_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
-            st.deferred_for_memory += deferred;
             st.steps = step;
         }
     }
@@ -347,18 +492,25 @@ sg_status sg_server_create(const sg_server_config *cfg, uint32_t nv, uint64_t ne
     if (!s) return SG_ENOMEM;
     s->cfg = *cfg;
     s->nv = nv;
-    s->slots = cfg->query_slots ? std::min<uint32_t>(cfg->query_slots, 4096) : 64;
+    s->slots = cfg->query_slots ? std::min<uint32_t>(cfg->query_slots, 4096) : 64;  // asked; capped by HBM below
     std::string err;
-    sg_status st = make_engine(cfg, SG_ALGO_MULTI, nv, ne, row, col, &s->multi, err);
-    if (st == SG_OK) {  // no queries yet: updates before the first one only change the graph
-        st = sg_init_queries(s->multi, nullptr, nullptr, 0);
-        if (st != SG_OK) err = sg_last_error(s->multi);
-    }
-    if (st == SG_OK && (cfg->standing & SG_SERVE_CC)) st = make_engine(cfg, SG_ALGO_CC, nv, ne, row, col, &s->cc, err);
+    sg_status st = SG_OK;
+    if (cfg->standing & SG_SERVE_CC) st = make_engine(cfg, SG_ALGO_CC, nv, ne, row, col, &s->cc, err);
     if (st == SG_OK && (cfg->standing & SG_SERVE_PR)) {
         sg_server_config pc = *cfg;
         pc.engine.flags |= SG_FLAG_DEFER_CONVERGE;  // one convergence per top-k answer
         st = make_engine(&pc, SG_ALGO_PR, nv, ne, row, col, &s->pr, err);
+    }
+    if (st == SG_OK) st = make_engine(cfg, SG_ALGO_MULTI, nv, ne, row, col, &s->multi, err);
+    if (st == SG_OK) {  // query slots: as many as asked and as the free HBM allows over the reserve
+        s->slots = s->affordable_slots(s->slots);
+        if (s->slots) {
+            st = sg_query_open(s->multi, s->slots);
+            if (st != SG_OK) err = sg_last_error(s->multi);
+        } else {
+            st = sg_init_queries(s->multi, nullptr, nullptr, 0);  // updates only change the graph
+            if (st != SG_OK) err = sg_last_error(s->multi);
+        }
     }
     if (st != SG_OK) {
         t_error = err;
@@ -427,6 +579,7 @@ sg_status sg_server_submit(sg_server *s, uint32_t kind, uint32_t arg, uint32_t t
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) { t_error = "server is stopping"; return SG_EINVAL; }
+    if (s->failed_status != SG_OK) { t_error = "server failed earlier: " + s->failed; return s->failed_status; }
     const uint64_t id = s->next_id++;
     s->reqs[id] = std::move(r);
     *ticket = id;
@@ -448,6 +601,8 @@ sg_status sg_server_wait(sg_server *s, uint64_t ticket, void *out, size_t bytes,
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) { t_error = "unknown ticket"; return SG_EINVAL; }
     Request *r = it->second.get();
+    if (r->claimed) { t_error = "another thread is waiting for this ticket"; return SG_EINVAL; }
+    r->claimed = true;  // the request lives until this waiter releases it
     s->cv_done.wait(lk, [&] { return r->done; });
     sg_status st = r->status;
     if (st != SG_OK) t_error = r->error;
@@ -461,6 +616,7 @@ sg_status sg_server_wait(sg_server *s, uint64_t ticket, void *out, size_t bytes,
     }
     if (info) *info = r->info;
     if (st != SG_EINVAL || r->status != SG_OK) s->reqs.erase(it);  // a too-small buffer may retry
+    else r->claimed = false;
     return st;
 }
 

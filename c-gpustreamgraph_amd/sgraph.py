@@ -125,6 +125,10 @@ SIGNATURES = [
     ("sg_server_last_error", C.c_char_p, [_P]),
     ("sg_server_destroy", None, [_P]),
     ("sg_gather_query", C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.c_size_t]),
+    ("sg_query_open", C.c_int, [_P, C.c_uint32]),
+    ("sg_query_add", C.c_int, [_P, _P, _P, C.c_uint32, _P]),
+    ("sg_query_step", C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("sg_query_release", C.c_int, [_P, _P, C.c_uint32]),
     ("sg_get_stats", C.c_int, [_P, C.POINTER(Stats)]),
     ("sg_kernel_times", C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     ("sg_reset_stats", C.c_int, [_P]),
@@ -346,6 +350,29 @@ class Engine:
         if len(k) != len(src):
             raise ValueError("one kind per source")
         _check(lib().sg_init_queries(self.h, _ptr(src), _ptr(k), len(src)), self.h)
+
+    def query_open(self, slots):
+        """Continuous batching: fix `slots` query slots (drops every query)."""
+        _check(lib().sg_query_open(self.h, slots), self.h)
+
+    def query_add(self, sources, kinds):
+        """Add queries into free slots; returns their slot numbers."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        k = np.array([ALGO[x] if isinstance(x, str) else int(x) for x in kinds], dtype=np.uint32)
+        slots = np.empty(len(src), dtype=np.uint32)
+        _check(lib().sg_query_add(self.h, _ptr(src), _ptr(k), len(src), _ptr(slots)), self.h)
+        return slots
+
+    def query_step(self, max_rounds, cap=4096):
+        """Run up to max_rounds rounds: (slots that converged, slots still running)."""
+        done = np.empty(cap, dtype=np.uint32)
+        nd, na = C.c_uint32(), C.c_uint32()
+        _check(lib().sg_query_step(self.h, max_rounds, _ptr(done), cap, C.byref(nd), C.byref(na)), self.h)
+        return done[:nd.value].copy(), na.value
+
+    def query_release(self, slots):
+        s = np.ascontiguousarray(slots, dtype=np.uint32)
+        _check(lib().sg_query_release(self.h, _ptr(s), len(s)), self.h)
 
     def query(self, qi, q=None):
         """Distances (or parents with q=Q_PARENT) of query qi of a batched engine; for

@@ -252,6 +252,32 @@ sg_status sg_restore_state(sg_engine *eng, const char *path);
  * (framework.cuh:389-439, 549-584); no reference counterpart as one call. */
 sg_status sg_init_queries(sg_engine *eng, const uint32_t *sources, const uint32_t *kinds, uint32_t nq);
 
+/* Continuous batching on an SG_ALGO_MULTI engine: queries join free slots
+ * while others are converging, and leave when their own work is done.
+ *   sg_query_open     fixes `slots` query slots (<= 4096) and drops every
+ *                     query (the graph stays);
+ *   sg_query_add      puts n BFS/SSSP queries (kinds as sg_init_queries) into
+ *                     free slots, written to slots[0..n); they start with the
+ *                     next round, beside the queries already running;
+ *   sg_query_step     runs up to max_rounds rounds for all running queries,
+ *                     then lists in done[0..*ndone) (at most cap) the slots
+ *                     whose query has converged since the last step
+ *                     (*nactive = slots still running); a converged slot's
+ *                     result is read with sg_gather_query / sg_gather_at
+ *                     (qi = slot) and stays until released;
+ *   sg_query_release  frees converged slots.
+ * An update (sg_apply_snapshot*) needs every running query converged first
+ * (SG_EINVAL otherwise); with no slot in use it only updates the graph, and
+ * converged queries still held are brought to the new snapshot. Each query's
+ * result equals a single-source engine's (the lexicographic (distance, parent)
+ * minimum has one fixed point whatever joins or leaves meanwhile). No
+ * reference counterpart (one query per process). */
+sg_status sg_query_open(sg_engine *eng, uint32_t slots);
+sg_status sg_query_add(sg_engine *eng, const uint32_t *sources, const uint32_t *kinds, uint32_t n, uint32_t *slots);
+sg_status sg_query_step(sg_engine *eng, uint32_t max_rounds, uint32_t *done, uint32_t cap, uint32_t *ndone,
+                        uint32_t *nactive);
+sg_status sg_query_release(sg_engine *eng, const uint32_t *slots, uint32_t n);
+
 /* Per-query result of an SG_ALGO_MULTI engine: what = SG_Q_VALUE (u32
  * distance or level, UINT32_MAX = unreached) or SG_Q_PARENT; bytes >= 4 * nv.
  * For an SG_ALGO_PR_CC engine qi selects the program: 0 = PageRank

@@ -10,18 +10,27 @@
  *                    engine's sg_apply_snapshot semantics: inserts, then
  *                    deletes);
  *   BFS / SSSP       one-shot queries from a source on the current snapshot,
- *                    answered together by a batched engine (SG_ALGO_MULTI):
- *                    continuous batching, queries that arrive while a step is
- *                    on the device are admitted at the next step, as many as
- *                    the query slots and the free HBM allow;
+ *                    answered by a batched engine (SG_ALGO_MULTI) with
+ *                    continuous batching: a query takes a free slot between
+ *                    round batches (every 8 rounds), runs beside the queries
+ *                    already converging, and is answered as soon as its own
+ *                    work is done; the slots are opened at creation, as many
+ *                    as query_slots asks and the free HBM allows over
+ *                    hbm_reserve_gb (none: BFS/SSSP requests fail with
+ *                    SG_EBUDGET; so do waiting ones when the free HBM falls
+ *                    under the reserve with no query running);
  *   CC label lookup  label of one vertex from a standing CC computation that
  *                    every update keeps at its fixed point;
  *   PR top-k         the k highest ranks of a standing PageRank, answered on
  *                    the latest snapshot its own thread has reached (the
  *                    snapshot is reported with the answer).
  *
- * A worker thread runs steps: apply pending updates, answer lookups, admit
- * queries, converge them in one launch per round, complete their requests.
+ * A worker thread runs steps: apply pending updates (first letting the
+ * running queries converge, so each answer belongs to one snapshot), answer
+ * lookups, admit waiting queries into free slots, run a few rounds, answer the
+ * queries that converged. If an engine fails to apply a snapshot the server
+ * enters a failed state: the engines may hold different graphs, so that
+ * request and every later one fail with the saved error.
  * A second thread applies the same snapshots, in order, to the PageRank
  * engine with deferred convergence (SG_FLAG_DEFER_CONVERGE) and converges
  * once per batch of top-k requests, on its own HIP stream, so PageRank work
@@ -97,7 +106,9 @@ sg_status sg_server_submit(sg_server *s, uint32_t kind, uint32_t arg, uint32_t t
                            uint64_t nins, const uint32_t *del_uv, uint64_t ndel, uint64_t *ticket);
 
 /* Block until the request completes, copy its result to out (bytes at least
- * the result's size; NULL for updates) and release it. */
+ * the result's size; NULL for updates) and release it. One thread may wait
+ * for a ticket (a second waiter gets SG_EINVAL); after SG_EINVAL for a too
+ * small buffer the ticket can be waited for again. */
 sg_status sg_server_wait(sg_server *s, uint64_t ticket, void *out, size_t bytes, sg_result_info *info);
 
 sg_status sg_server_get_stats(sg_server *s, sg_server_stats *out);

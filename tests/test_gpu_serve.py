@@ -160,3 +160,136 @@ def test_deferred_pagerank_converges_once_for_many_snapshots():
     assert np.array_equal(lazy.values(), got)
     lazy.close()
     eager.close()
+
+
+def test_query_slots_join_mid_convergence():
+    # continuous batching on the engine: queries join free slots while others
+    # converge, leave when done, slots are reused; every answer equals the
+    # oracle and a single-source engine (distances and parents)
+    nv, row, col = sgraph.gen_rmat(14, 16, seed=97)
+    ins, dele = sgraph.gen_updates(row, col, 14, 2, 2000, 2000, seed=98)
+    e = sgraph.Engine("multi", weight_mode="seeded", weight_seed=4)
+    e.load_csr(row, col)
+    e.query_open(8)
+    rng = np.random.default_rng(99)
+    src = {}
+    hubs = [int(x) for x in np.argsort(np.diff(row.astype(np.int64)))[-2:]]  # deep enough for one round not to end them
+    first = e.query_add(hubs, ["sssp", "bfs"])
+    src.update({int(s): (q, k) for s, q, k in zip(first, hubs, ["sssp", "bfs"])})
+    done, active = e.query_step(1)  # one round: the first two are mid-convergence
+    assert active == 2 and len(done) == 0
+    more = [int(x) for x in rng.integers(0, nv, 4)]
+    kinds = ["sssp", "bfs", "sssp", "bfs"]
+    slots = e.query_add(more, kinds)
+    src.update({int(s): (q, k) for s, q, k in zip(slots, more, kinds)})
+    with pytest.raises(sgraph.SgError):  # an update waits for the running queries
+        e.apply(ins[0], dele[0])
+    results = {}
+    while True:
+        done, active = e.query_step(2)
+        for s in done:
+            results[int(s)] = (e.query(int(s)), e.query(int(s), sgraph.Q_PARENT))
+        if len(done):
+            e.query_release(done)
+        if active == 0:
+            break
+    assert sorted(results) == sorted(src)
+    for s, (d, p) in results.items():
+        q, k = src[s]
+        exp = O.bfs(row, col, q) if k == "bfs" else O.sssp(row, col, q, "seeded", 4)
+        assert np.array_equal(d, exp), f"slot {s} ({k} from {q})"
+        one = sgraph.Engine(k, weight_mode="seeded", weight_seed=4)
+        one.load_csr(row, col)
+        one.init(q)
+        assert np.array_equal(one.values(), d) and np.array_equal(one.gather(sgraph.Q_PARENT), p)
+        one.close()
+    # no slot in use: an update only changes the graph; freed slots are reused
+    e.apply(ins[0], dele[0])
+    r1, c1 = O.apply_updates(row, col, ins[0], dele[0])
+    s2 = e.query_add([3], ["sssp"])
+    assert int(s2[0]) < 8
+    finished = []
+    while True:
+        done, active = e.query_step(4)
+        finished += [int(x) for x in done]
+        if active == 0:
+            break
+    assert finished == [int(s2[0])]
+    assert np.array_equal(e.query(int(s2[0])), O.sssp(r1, c1, 3, "seeded", 4))
+    # a converged query still held is brought to the next snapshot by the update
+    e.apply(ins[1], dele[1])
+    r2, c2 = O.apply_updates(r1, c1, ins[1], dele[1])
+    assert np.array_equal(e.query(int(s2[0])), O.sssp(r2, c2, 3, "seeded", 4))
+    e.close()
+
+
+def test_server_refuses_queries_without_hbm_for_them():
+    # an HBM reserve larger than the device: no query slot can be opened, so
+    # BFS/SSSP requests fail with SG_EBUDGET (counted), lookups still work
+    nv, row, col = sgraph.gen_rmat(10, 8, seed=101)
+    srv = sgraph.Server(row, col, query_slots=16, hbm_reserve_gb=1e6)
+    with pytest.raises(sgraph.SgError) as e:
+        srv.wait(srv.submit(sgraph.REQ_BFS, 0), sgraph.REQ_BFS, 0)
+    assert e.value.status == sgraph.SG_EBUDGET
+    out, info = srv.wait(srv.submit(sgraph.REQ_CC, 3), sgraph.REQ_CC, 3)
+    assert out[0] == O.cc(row, col)[3]
+    st = srv.stats()
+    srv.close()
+    assert st["deferred_for_memory"] >= 1 and st["queries"] == 0
+
+
+def test_server_enters_failed_state_after_an_engine_error():
+    # the smallest device budget the server can start with, then an update
+    # batch whose scratch does not fit: that update fails with the engine's
+    # status, and every later request fails with the saved error
+    nv, row, col = sgraph.gen_rmat(12, 8, seed=103)
+    srv, b = None, 0.001
+    while srv is None and b < 1.0:
+        try:
+            srv = sgraph.Server(row, col, query_slots=4, standing=sgraph.SERVE_CC, device_budget_gb=b, hbm_reserve_gb=0)
+        except sgraph.SgError as e:
+            assert e.status in (sgraph.SG_EBUDGET, sgraph.SG_ENOMEM)
+            b *= 1.25
+    assert srv is not None
+    rng = np.random.default_rng(104)
+    big = rng.integers(0, nv, (400000, 2)).astype(np.uint32)
+    with pytest.raises(sgraph.SgError) as e1:
+        srv.wait(srv.submit(sgraph.REQ_UPDATE, ins=big, dele=big[:1000]))
+    assert e1.value.status in (sgraph.SG_EBUDGET, sgraph.SG_ENOMEM)
+    with pytest.raises(sgraph.SgError) as e2:
+        srv.wait(srv.submit(sgraph.REQ_CC, 1), sgraph.REQ_CC, 1)
+    assert e2.value.status == e1.value.status
+    srv.close()
+
+
+def test_one_waiter_per_ticket():
+    # a second waiter on a ticket (while the first waits, or after it took the
+    # result) gets SG_EINVAL; the first gets the answer
+    nv, row, col = sgraph.gen_rmat(12, 8, seed=105)
+    srv = sgraph.Server(row, col, query_slots=8, standing=sgraph.SERVE_CC)
+    rng = np.random.default_rng(106)
+    big = rng.integers(0, nv, (200000, 2)).astype(np.uint32)
+    t = srv.submit(sgraph.REQ_UPDATE, ins=big, dele=big[:10])
+    first = {}
+
+    def waiter():
+        try:
+            first["out"] = srv.wait(t)
+        except sgraph.SgError as e:
+            first["err"] = e
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    import time
+    time.sleep(0.05)
+    with pytest.raises(sgraph.SgError) as e:
+        srv.wait(t)
+    assert e.value.status == sgraph.SG_EINVAL
+    th.join()
+    assert "out" in first and first["out"][1]["snapshot"] == 1
+    q = srv.submit(sgraph.REQ_BFS, 0, target=1)
+    srv.wait(q, sgraph.REQ_BFS, 0, 1)
+    with pytest.raises(sgraph.SgError) as e:
+        srv.wait(q, sgraph.REQ_BFS, 0, 1)
+    assert e.value.status == sgraph.SG_EINVAL
+    srv.close()
