@@ -16,6 +16,16 @@
 //   AccumulateBuffer(src,dst,w,&buf_dst,push) -> dst became active     api.cuh:65-78
 //   AccumulateBuffer_del(src,dst,&buf,&val) -> dst was reset           api.cuh:93-103
 //   IsActiveNode(v,buf,val), IsActiveNode_INC(v,buf,val)               api.cuh:111-122
+//   MergeBuffer(dst,&buf_dst,acc) -> dst improved (pull shape)         pull_functor.h:137-175
+//                                    the destination side of a pull round: what
+//                                    dst's in-edges offered, accumulated with
+//                                    AccumulateBuffer into a word of its own that
+//                                    started at GetIdentityElement(), folded into
+//                                    dst's buffer (atomically: the in-list of a
+//                                    vertex can be split between waves)
+//
+// The contract the public operator header (include/sgraph_operator.h) states
+// for a caller's own operator is the one these four follow.
 //
 // Differences from the reference, each deliberate:
 //  * The traversal operators keep parent and pending value in one 64-bit buffer
@@ -131,6 +141,10 @@ struct SSSP : TraversalBase {
     }
     // What v pushes along an edge right now (edges added by an update).
     __device__ push_t PushValue(buffer_t buf) const { return hi32(buf); }
+    // pull shape: the best (distance, parent) the in-edges offered
+    __device__ bool MergeBuffer(uint32_t, buffer_t *p_buffer, buffer_t acc) const {
+        return accumulate_min(p_buffer, lo32(acc), hi32(acc));
+    }
 };
 
 struct BFS : TraversalBase {
@@ -166,6 +180,9 @@ struct BFS : TraversalBase {
         return true;
     }
     __device__ push_t PushValue(buffer_t buf) const { return hi32(buf) == kInf ? kInf : hi32(buf) + 1; }
+    __device__ bool MergeBuffer(uint32_t, buffer_t *p_buffer, buffer_t acc) const {
+        return accumulate_min(p_buffer, lo32(acc), hi32(acc));
+    }
 };
 
 // Directed min-ancestor labels: label(v) = min{u : u reaches v}.
@@ -190,10 +207,14 @@ struct CC : TraversalBase {
     // on a strict label decrease (a CAS on the packed word), so parent edges
     // always point to a vertex that held the label earlier: a forest.
     __device__ bool AccumulateBuffer(uint32_t src, uint32_t, uint32_t, buffer_t *p_buffer, push_t pushed) const {
+        return MergeBuffer(0, p_buffer, pack(pushed, src));
+    }
+    // pull shape: the smallest label an in-edge offered, with that in-neighbour
+    // as the parent, on a strict decrease only (as above)
+    __device__ bool MergeBuffer(uint32_t, buffer_t *p_buffer, buffer_t acc) const {
         unsigned long long cur = __hip_atomic_load(p_buffer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long nv = pack(pushed, src);
-        while (pushed < hi32(cur)) {
-            const unsigned long long prev = atomicCAS(p_buffer, cur, nv);
+        while (hi32(acc) < hi32(cur)) {
+            const unsigned long long prev = atomicCAS(p_buffer, cur, acc);
             if (prev == cur) return true;
             cur = prev;
         }

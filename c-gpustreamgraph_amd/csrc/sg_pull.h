@@ -176,6 +176,114 @@ __global__ void __launch_bounds__(kBlock) k_pr_pull_fix(const unsigned long long
     }
 }
 
+// ---- pull rounds of traversal programs -------------------------------------------
+// The same direction choice for min-type programs (SSSP, BFS, CC and a
+// caller's operator with a MergeBuffer method). In a dense round R the round
+// kernel commits every frontier vertex and records its push tagged with R
+// (VCombinePull::take) instead of walking its out-list; k_trav_pull then walks
+// every in-list: per destination, the offers of the in-neighbours tagged R are
+// accumulated in LDS with the operator's AccumulateBuffer (from
+// GetIdentityElement) and folded into the destination's buffer with
+// MergeBuffer. The fold is atomic, so a destination whose in-list is split by
+// a tile boundary needs no fix-up pass: each tile folds its part. Every offer
+// a push round makes along u -> v is made here along the same edge, and both
+// folds keep the minimum, so both directions reach the same buffers and
+// schedule the same vertices.
+template <class Op>
+struct VCombinePull {
+    static constexpr bool kDense = true;
+    Op op;
+    typename Op::value_t *value;
+    typename Op::buffer_t *buffer;
+    unsigned long long *xb;  // [nv] tagged pushes
+    uint32_t round;
+    __device__ bool operator()(uint32_t u, uint32_t deg, uint32_t &push, bool &self) const {
+        self = false;
+        auto p = op.CombineValueBuffer(u, &value[u], &buffer[u], deg);
+        push = to_bits<typename Op::push_t>(p.value);
+        return p.scheduled;
+    }
+    __device__ void take(uint32_t u, uint32_t push) const { xb[u] = ((unsigned long long)round << 32) | push; }
+};
+
+template <class Op>
+struct PullTrav {
+    Op op;
+    const unsigned long long *xb;  // tagged pushes of this round's frontier
+    typename Op::buffer_t *buffer;
+    uint32_t *stamp;               // round stamps (frontier de-duplication)
+    uint32_t round;                // R
+    uint32_t wmode;
+    uint64_t wseed;
+};
+
+// One wave per tile of the in-edge store (as k_pr_pull); colp/slot_lo as there.
+template <class Op>
+__global__ void __launch_bounds__(kSweepBlock) k_trav_pull(GraphView rg, const unsigned long long *dpre, SweepTiles tiles,
+                                                           const uint32_t *colp, uint64_t slot_lo, PullTrav<Op> p,
+                                                           PassIO io, int force) {
+    using buf_t = typename Op::buffer_t;
+    if (!pull_dense(io, force)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(io.stats + 4, 1ull);  // a pull round
+    __shared__ SweepWave s_wave[kSweepWaves];
+    __shared__ buf_t s_acc[kSweepWaves][kSweepTile + 1];
+    __shared__ unsigned long long s_red[kSweepWaves];
+    const unsigned long long E = dpre[rg.nv];
+    const unsigned long long ntiles = (rg.nv + E + kSweepTile - 1) / kSweepTile;
+    const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+    SweepWave &sh = s_wave[w];
+    buf_t *const acc = s_acc[w];
+    const buf_t ident = p.op.GetIdentityElement();
+    OutStage st{sh.out, 0, kSweepOutStage};
+    uint32_t n_edges = 0;
+    unsigned long long n_done = 0;
+    TilePipe tp;
+    tp.tend = min_u64(ntiles, tiles.cap - 1);
+    tp.nw = (unsigned long long)gridDim.x * kSweepWaves;
+    tp.t = (unsigned long long)blockIdx.x * kSweepWaves + w;
+    tp.start(rg, dpre, tiles);
+    while (tp.t < tp.tend) {
+        uint32_t vv[kSweepPerLane];
+        const TileHead h = tp.h;
+        sweep_tile(rg, dpre, h, tp.pre, colp, slot_lo, 0u, rg.nv, sh, vv, n_edges);
+        tp.advance(rg, dpre, tiles);
+        unsigned long long xw[kSweepPerLane];
+#pragma unroll
+        for (int j = 0; j < kSweepPerLane; ++j) xw[j] = vv[j] != kGap ? p.xb[vv[j]] : 0ull;
+        for (uint32_t i = lane; i < h.span; i += kWave) acc[i] = ident;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int j = 0; j < kSweepPerLane; ++j) {
+            if (vv[j] != kGap && (uint32_t)(xw[j] >> 32) == p.round) {
+                const uint32_t own = sh.own[j * kWave + lane];
+                const uint32_t u = vv[j], v = h.v0 + own;
+                const uint32_t wt = Op::kWeighted ? sg_weight(p.wmode, p.wseed, u, v) : 0u;
+                p.op.AccumulateBuffer(u, v, wt, &acc[own], from_bits<typename Op::push_t>((uint32_t)xw[j]));
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i0 = 0; i0 < h.span; i0 += kWave) {
+            const uint32_t i = i0 + lane;
+            const uint32_t v = h.v0 + i;
+            bool act = false;
+            if (i < h.span) {
+                const buf_t a = acc[i];
+                if (a != ident) {
+                    act = p.op.MergeBuffer(v, &p.buffer[v], a) && atomicExch(&p.stamp[v], p.round + 1) != p.round + 1;
+                    ++n_done;
+                }
+            }
+            stage_push(act, v, st, io);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    stage_flush(st, io);
+    stage_count_flush(st, io);
+    block_count<kSweepWaves>((unsigned long long)n_edges, s_red, io.stats + 0);
+    __syncthreads();
+    block_count<kSweepWaves>(n_done, s_red, io.stats + 1);
+}
+
 // ---- traversal recompute from the in-lists of invalidated vertices -------------
 // After deletions, every invalidated vertex v must see the best value any
 // in-neighbour u offers: PushValue(best(u)) + w(u, v) (for CC the label of u).

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libsgraph.so")
 
 SG_OK, SG_EINVAL, SG_ENOMEM, SG_EBUDGET, SG_EDEVICE, SG_EIO = range(6)
 STATUS_NAMES = {0: "SG_OK", 1: "SG_EINVAL", 2: "SG_ENOMEM", 3: "SG_EBUDGET", 4: "SG_EDEVICE", 5: "SG_EIO"}
-ALGO = {"pr": 0, "sssp": 1, "bfs": 2, "cc": 3, "multi": 4, "prcc": 5}
+ALGO = {"pr": 0, "sssp": 1, "bfs": 2, "cc": 3, "multi": 4, "prcc": 5, "custom": 6}
 WEIGHT = {"hash128": 0, "seeded": 1, "one": 2}
 Q_VALUE, Q_BUFFER, Q_PARENT, Q_DEGREE, Q_OFFSETS, Q_SLOTS, Q_SEGMENTS, Q_IN_DEGREE, Q_IN_OFFSETS, Q_IN_SLOTS = range(10)
 PULL_NEVER = 0xFFFFFFFF      # pull_divisor: push rounds only, no in-edge store
@@ -173,6 +173,24 @@ def lib():
     return _lib
 
 
+_operators = {}
+
+
+def load_operator(name, path=None):
+    """Create function of an operator library built with include/sgraph_operator.h:
+    `sg_create_<name>` from `path` (default: lib<...> beside libsgraph.so,
+    libsg_<name>.so). Pass it to Engine(create=...)."""
+    key = (name, path)
+    if key not in _operators:
+        lib()  # libsgraph.so first: the operator library links to it
+        L = C.CDLL(path or os.path.join(HERE, f"libsg_{name}.so"))
+        f = getattr(L, f"sg_create_{name}")
+        f.restype = C.c_int
+        f.argtypes = [C.c_void_p, C.c_void_p]
+        _operators[key] = (L, f)
+    return _operators[key][1]
+
+
 def _check(st, eng=None):
     if st != SG_OK:
         msg = lib().sg_last_error(eng)
@@ -296,11 +314,13 @@ def default_config(**kw):
 class Engine:
     """One engine handle (single-threaded)."""
 
-    def __init__(self, algo, **cfg):
+    def __init__(self, algo, create=None, **cfg):
+        """algo: a built-in program ("pr", "sssp", ...), or a name for an engine
+        made by `create` (an operator library's create function, load_operator)."""
         self.algo = algo
-        self.cfg = default_config(algo=algo, **cfg)
+        self.cfg = default_config(algo=algo if create is None else "custom", **cfg)
         h = C.c_void_p()
-        _check(lib().sg_create(C.byref(self.cfg), C.byref(h)))
+        _check((create or lib().sg_create)(C.byref(self.cfg), C.byref(h)))
         self.h = h
         self.nv = 0
 

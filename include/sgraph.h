@@ -42,7 +42,7 @@ typedef enum sg_algo {
     SG_ALGO_MULTI = 4, /* batched BFS/SSSP queries on one graph (sg_init_queries); no
                           reference counterpart: the reference answers one source per
                           process (samples/hybrid_sssp/main.cu, samples/hybrid_bfs/main.cu) */
-    SG_ALGO_PR_CC = 5  /* PageRank and CC labels kept together on one graph: one forward
+    SG_ALGO_PR_CC = 5, /* PageRank and CC labels kept together on one graph: one forward
                           store and one in-edge store serve both programs, each batch is
                           applied to them once, and CC's recompute after deletions reads
                           the in-lists of the reset vertices instead of sweeping every
@@ -50,6 +50,9 @@ typedef enum sg_algo {
                           PageRank (f32 ranks), qi 1 = CC (u32 labels, parents). No
                           reference counterpart (HybridPageRank and HybridCC run in
                           separate processes on separate graphs) */
+    SG_ALGO_CUSTOM = 6 /* an engine built from a caller's own operator by a create
+                          function of sgraph_operator.h (framework.cuh:77 takes the
+                          operator as a template argument); sg_create refuses it */
 } sg_algo;
 
 /* Edge-weight function of SSSP. Weights are computed from (u, v) on the fly
@@ -70,11 +73,14 @@ typedef struct sg_config {
     uint32_t segments;        /* -SEGMENT vertex segments, framework.cuh:310-384          */
     uint32_t n_stream;        /* -n_stream (app_skeleton.h:89): segment copies in flight
                                  (staging buffers) of a streamed sweep, 2..8               */
-    uint32_t pull_divisor;    /* PageRank: a round whose frontier's out-edges exceed
-                                 E / pull_divisor is a pull round over the in-edge store
-                                 (0 = 24, or 4 with the store in host memory);
-                                 UINT32_MAX = push rounds only, no in-edge store
-                                 (pull_functor.h:22-369, sync_pull_dd.cuh:1-357)          */
+    uint32_t pull_divisor;    /* a round whose frontier's out-edges exceed E /
+                                 pull_divisor is a pull round over the in-edge store
+                                 (pull_functor.h:22-369, sync_pull_dd.cuh:1-357).
+                                 PageRank: 0 = 24, or 4 with the store in host memory;
+                                 UINT32_MAX = push rounds only, no in-edge store.
+                                 SSSP / BFS / CC and custom operators with the pull
+                                 method, with SG_FLAG_IN_EDGES: 0 = 2; UINT32_MAX =
+                                 push rounds only (the store still seeds recomputes) */
     uint32_t max_rounds;      /* cap on convergence rounds per phase (0 = 1000000)        */
     uint32_t rounds_per_sync; /* rounds enqueued between host convergence checks; 0 =
                                  adaptive: a snapshot phase first enqueues the rounds it
@@ -119,7 +125,12 @@ typedef struct sg_config {
  * updated by every batch) and seed the recompute after deletions from the
  * in-lists of the reset vertices instead of a sweep over every live edge
  * (the reference re-pushes from every vertex, sync_push_dd.cuh:1470-1557).
- * Results are identical; the sweep's cost moves to the second PMA update. */
+ * Results are identical; the sweep's cost moves to the second PMA update.
+ * The store also makes rounds direction-optimising (pull_divisor): a round
+ * whose frontier holds a large share of the edges walks every in-list and
+ * folds, per destination, what its in-neighbours in the frontier offer
+ * (the operator's pull method), instead of pushing along every out-edge of
+ * the frontier; both directions give the same values and parents. */
 #define SG_FLAG_IN_EDGES 0x20u
 /* PageRank with its in-edge store in pinned host memory: a pull round
  * streams every SEGMENT's slot range (gaps included) through HBM staging

@@ -38,6 +38,12 @@
  * or_pr_residual           FP64 residual of r = (1-a) + a A^T D^-1 r at a given  zero (to rounding) at the
  *                          rank vector: sum and max |res|. With M = a A^T D^-1     or_pr_fp64 fixed point; the
  *                          (column sums <= a), ||r - r*||_1 <= ||res||_1/(1 - a)  bound holds on random vectors
+ *
+ * Programs with no reference counterpart (operators a caller compiles in):
+ * or_widest_path           widest (maximum-bottleneck) path from a source:       brute force over all simple
+ *                          textbook Dijkstra variant (largest width first);       paths on tiny graphs; an
+ *                          width(src) = UINT32_MAX, unreachable 0                 independent fixed-point
+ *                                                                                 iteration (tests/)
  */
 #include <math.h>
 #include <stdint.h>
@@ -126,6 +132,34 @@ int or_sssp_dijkstra(uint32_t nv, const uint64_t *row, const uint32_t *col, uint
             uint32_t v = col[e];
             uint64_t nd = (uint64_t)d + or_weight(wmode, seed, u, v);
             if (nd < out[v]) { out[v] = (uint32_t)nd; heap_push(h, &n, (nd << 32) | v); }
+        }
+    }
+    free(h);
+    return 0;
+}
+
+/* ---- widest path: Dijkstra on widths, widest first ---------------------
+ * Heap keys (~width) << 32 | vertex pop the widest vertex first; a popped
+ * width is final because widths only shrink along a path. */
+int or_widest_path(uint32_t nv, const uint64_t *row, const uint32_t *col, uint32_t src,
+                   uint32_t wmode, uint64_t seed, uint32_t *out) {
+    uint64_t ne = nv ? row[nv] : 0, n = 0;
+    uint64_t *h;
+    for (uint32_t i = 0; i < nv; ++i) out[i] = 0;
+    if (src >= nv) return 1;
+    h = (uint64_t *)malloc((size_t)(ne + 1) * sizeof(uint64_t));
+    if (!h) return 2;
+    out[src] = OR_INF;
+    heap_push(h, &n, (uint64_t)src); /* key ~UINT32_MAX = 0 */
+    while (n) {
+        uint64_t k = heap_pop(h, &n);
+        uint32_t wu = ~(uint32_t)(k >> 32), u = (uint32_t)k;
+        if (wu != out[u]) continue; /* stale entry */
+        for (uint64_t e = row[u]; e < row[u + 1]; ++e) {
+            uint32_t v = col[e];
+            uint32_t w = or_weight(wmode, seed, u, v);
+            uint32_t nw = w < wu ? w : wu;
+            if (nw > out[v]) { out[v] = nw; heap_push(h, &n, ((uint64_t)(uint32_t)~nw << 32) | v); }
         }
     }
     free(h);

@@ -31,6 +31,7 @@ def lib():
             "or_bfs_levels": (C.c_int, [C.c_uint32, _P, _P, C.c_uint32, _P]),
             "or_sssp_dijkstra": (C.c_int, [C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint64, _P]),
             "or_sssp_fifo": (C.c_int, [C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint64, _P]),
+            "or_widest_path": (C.c_int, [C.c_uint32, _P, _P, C.c_uint32, C.c_uint32, C.c_uint64, _P]),
             "or_cc_min_ancestor": (C.c_int, [C.c_uint32, _P, _P, _P]),
             "or_cc_label_prop": (C.c_int, [C.c_uint32, _P, _P, _P]),
             "or_pr_fp64": (C.c_int, [C.c_uint32, _P, _P, C.c_double, C.c_double, C.c_int, _P]),
@@ -86,6 +87,68 @@ def sssp_fifo(row, col, src, wmode="hash128", seed=0):
     out = np.empty(len(row) - 1, dtype=np.uint32)
     assert lib().or_sssp_fifo(len(row) - 1, _p(row), _p(col), src, WMODE.get(wmode, wmode), seed, _p(out)) == 0
     return out
+
+
+def widest_path(row, col, src, wmode="hash128", seed=0):
+    """Widest (maximum-bottleneck) path widths from src: INF at src, 0 unreached."""
+    row, col = _csr(row, col)
+    out = np.empty(len(row) - 1, dtype=np.uint32)
+    assert lib().or_widest_path(len(row) - 1, _p(row), _p(col), src, WMODE.get(wmode, wmode), seed, _p(out)) == 0
+    return out
+
+
+def edge_weights(row, col, wmode="hash128", seed=0):
+    """Weight of every CSR edge (the engine's weight function)."""
+    row, col = _csr(row, col)
+    src = np.repeat(np.arange(len(row) - 1, dtype=np.uint32), np.diff(row.astype(np.int64)))
+    w = np.fromiter((weight(wmode, seed, int(u), int(v)) for u, v in zip(src, col)), dtype=np.uint32, count=len(col))
+    return src, w
+
+
+def widest_certificate(row, col, src, width, parent, wmode="hash128", seed=0):
+    """Violations of: width(src) = INF; no edge u -> v offers more than width(v)
+    (min(width(u), w) <= width(v)); every reached v != src has a parent edge
+    (p, v) with min(width(p), w(p, v)) = width(v), unreached vertices none; the
+    parent chains are acyclic and end at src. With these, width(v) is the
+    widest-path width (a path of that width exists, and no edge can widen it)."""
+    row, col = _csr(row, col)
+    nv = len(row) - 1
+    width = np.asarray(width, dtype=np.uint64)
+    parent = np.asarray(parent, dtype=np.uint64)
+    s, w = edge_weights(row, col, wmode, seed)
+    bad = int(width[src] != INF)
+    offer = np.minimum(width[s], w.astype(np.uint64))
+    bad += int(np.count_nonzero(offer > width[col]))
+    reached = (width > 0) & (np.arange(nv) != src)
+    bad += int(np.count_nonzero((width == 0) & (parent != INF)))
+    rv = np.nonzero(reached)[0]
+    p = parent[rv]
+    okp = p < nv
+    bad += int(np.count_nonzero(~okp))
+    rv, p = rv[okp], p[okp]
+    # the parent edge must exist with the tight weight (the widest of parallel edges)
+    key_e = s.astype(np.uint64) << np.uint64(32) | col.astype(np.uint64)
+    ukeys = np.unique(key_e)
+    order = np.argsort(key_e, kind="stable")
+    ks, ws = key_e[order], w[order]
+    starts = np.flatnonzero(np.r_[True, ks[1:] != ks[:-1]]) if len(ks) else np.zeros(0, dtype=np.int64)
+    wmax = np.maximum.reduceat(ws, starts) if len(ks) else np.zeros(0, dtype=np.uint32)
+    want = p << np.uint64(32) | rv.astype(np.uint64)
+    pos = np.searchsorted(ukeys, want)
+    found = (pos < len(ukeys)) & (ukeys[np.minimum(pos, max(len(ukeys) - 1, 0))] == want) if len(ukeys) else \
+        np.zeros(len(want), dtype=bool)
+    bad += int(np.count_nonzero(~found))
+    fi = np.nonzero(found)[0]
+    tight = np.minimum(width[p[fi]], wmax[pos[fi]].astype(np.uint64)) == width[rv[fi]]
+    bad += int(np.count_nonzero(~tight))
+    # chains end at src: pointer jumping over the parent array
+    nxt = np.where(reached & (parent < nv), parent, np.uint64(nv)).astype(np.int64)
+    nxt[src] = src
+    nxt = np.append(nxt, nv)  # sink for broken chains
+    for _ in range(64):
+        nxt = nxt[nxt]
+    bad += int(np.count_nonzero(nxt[:nv][reached] != src))
+    return bad
 
 
 def cc(row, col):

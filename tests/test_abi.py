@@ -97,3 +97,24 @@ def test_device_generator_without_device_fails_cleanly():
     with pytest.raises(sgraph.SgError) as e:
         sgraph.gen_rmat_device(6, 4, device=99)
     assert e.value.status == sgraph.SG_EDEVICE
+
+
+def test_operator_library_loads_and_refuses_without_device():
+    # the widest-path operator library (include/sgraph_operator.h) exports its
+    # create function; without a device the create fails with a status and
+    # libsgraph's sg_last_error(NULL) explains it
+    f = sgraph.load_operator("widest_path")
+    cfg = sgraph.default_config(algo="custom", device=99)
+    h = C.c_void_p()
+    assert f(C.byref(cfg), C.byref(h)) == sgraph.SG_EDEVICE and not h.value
+    assert b"device" in sgraph.lib().sg_last_error(None).lower()
+    cfg = sgraph.default_config(algo="custom", weight_mode=7)
+    assert f(C.byref(cfg), C.byref(h)) == sgraph.SG_EINVAL
+    assert f(None, C.byref(h)) == sgraph.SG_EINVAL
+
+
+def test_sg_create_refuses_custom_algo():
+    cfg = sgraph.default_config(algo="custom")
+    h = C.c_void_p()
+    assert sgraph.lib().sg_create(C.byref(cfg), C.byref(h)) == sgraph.SG_EINVAL
+    assert b"operator library" in sgraph.lib().sg_last_error(None)

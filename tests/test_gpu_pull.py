@@ -186,3 +186,57 @@ def test_c1_full_size():
     ins, dele = sgraph.gen_updates(row, col, 16, 2, 10000, 10000, seed=sgraph.DEFAULT_SEED + 1)
     say(f"[gpu] C1 nv={nv} ne={len(col)}")
     pr_stream(row, col, list(zip(ins, dele)), segments=64)
+
+
+def trav_modes(algo, scale, ef, nsnap, nupd, seed, **cfg):
+    nv, row, col = sgraph.gen_rmat(scale, ef, seed=seed)
+    ins, dele = sgraph.gen_updates(row, col, scale, nsnap, nupd, nupd, seed=seed + 1, hub_frac=0.5, hub_top=0.01)
+    flags = sgraph.FLAG_IN_EDGES | cfg.pop("flags", 0)
+    src = 0 if algo != "cc" else None
+    graphs = [(row, col)]
+    for t in range(nsnap):
+        graphs.append(O.apply_updates(*graphs[-1], ins[t], dele[t]))
+    res = {}
+    for mode, div in (("never", sgraph.PULL_NEVER), ("always", sgraph.PULL_ALWAYS), ("default", 0)):
+        e = sgraph.Engine(algo, weight_mode="seeded", weight_seed=7, flags=flags, pull_divisor=div, **cfg)
+        try:
+            e.load_csr(row, col)
+            e.init(src)
+            outs, pulls = [(e.values(), e.gather(sgraph.Q_PARENT))], e.stats()["pull_rounds"]
+            for t in range(nsnap):
+                e.apply(ins[t], dele[t])
+                outs.append((e.values(), e.gather(sgraph.Q_PARENT)))
+                pulls += e.stats()["pull_rounds"]
+        finally:
+            e.close()
+        res[mode] = (outs, pulls)
+        say(f"[gpu] {algo} pull mode {mode}: {pulls} pull rounds {cfg}")
+    for t, (r, c) in enumerate(graphs):
+        exp = O.bfs(r, c, 0) if algo == "bfs" else O.sssp(r, c, 0, "seeded", 7) if algo == "sssp" else O.cc(r, c)
+        for mode in res:
+            d, p = res[mode][0][t]
+            assert np.array_equal(d, exp), f"G{t} {mode}: values differ from the oracle"
+            if algo == "cc":
+                assert O.cc_certificate(r, c, d, p) == 0, f"G{t} {mode}: CC certificate"
+            else:
+                # a unique fixed point: the smallest parent id of the best value
+                assert np.array_equal(p, res["never"][0][t][1]), f"G{t} {mode}: parents differ from push rounds"
+    assert res["never"][1] == 0 and res["always"][1] > 0
+    return res
+
+
+@pytest.mark.parametrize("algo", ["sssp", "bfs", "cc"])
+def test_traversal_pull_rounds(algo):
+    # direction-optimising rounds over the in-edge store: every round pulls,
+    # none does, or the default share; same values (and SSSP/BFS parents)
+    trav_modes(algo, 14, 16, 3, 3000, seed=67)
+
+
+@pytest.mark.parametrize("algo", ["sssp", "cc"])
+def test_traversal_pull_rounds_host_edges(algo):
+    trav_modes(algo, 12, 16, 2, 2000, seed=69, flags=sgraph.FLAG_HOST_EDGES, segments=7, cache_level=1)
+
+
+def test_traversal_pull_rounds_as_graphs():
+    trav_modes("bfs", 13, 16, 2, 2000, seed=79, flags=sgraph.FLAG_GRAPH_ROUNDS)
+

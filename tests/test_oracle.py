@@ -288,3 +288,114 @@ def test_pr_residual_bounds_the_error():
         l1, mx, _ = O.pr_residual(row, col, r)
         err = float(np.abs(r.astype(np.float64) - exact).sum())
         assert err <= l1 / (1 - 0.85) * (1 + 1e-9)
+
+
+# ---- widest path (the operator library's program; no reference counterpart) ----
+def _widest_brute(nv, row, col, src, wmode, seed):
+    """Largest bottleneck over every simple path (depth-first enumeration)."""
+    best = [0] * nv
+    best[src] = O.INF
+    w = {}
+    for u in range(nv):
+        for e in range(int(row[u]), int(row[u + 1])):
+            v = int(col[e])
+            w[(u, v)] = max(w.get((u, v), 0), O.weight(wmode, seed, u, v))
+
+    def walk(u, width, on_path):
+        for (a, b), ww in w.items():
+            if a != u or b in on_path:
+                continue
+            nw = min(width, ww)
+            best[b] = max(best[b], nw)
+            on_path.add(b)
+            walk(b, nw, on_path)
+            on_path.discard(b)
+
+    walk(src, O.INF, {src})
+    return np.array(best, dtype=np.uint32)
+
+
+def _widest_fixed_point(row, col, src, wmode, seed):
+    """width[v] = max(width[v], max over edges u -> v of min(width[u], w)) until stable."""
+    nv = len(row) - 1
+    s, w = O.edge_weights(row, col, wmode, seed)
+    width = np.zeros(nv, dtype=np.uint64)
+    width[src] = O.INF
+    while True:
+        offer = np.minimum(width[s], w.astype(np.uint64))
+        nw = width.copy()
+        np.maximum.at(nw, col, offer)
+        if np.array_equal(nw, width):
+            return width.astype(np.uint32)
+        width = nw
+
+
+def _widest_parents(row, col, src, width, wmode, seed):
+    """A parent tree for the widths from a pure-Python widest-first search
+    (heapq): a vertex's parent is the popped vertex whose edge last widened it.
+    Its widths must equal the given ones."""
+    import heapq
+    nv = len(row) - 1
+    best = [0] * nv
+    par = np.full(nv, O.INF, dtype=np.uint32)
+    best[src] = O.INF
+    heap, done = [(-O.INF, src)], [False] * nv
+    while heap:
+        nw, u = heapq.heappop(heap)
+        if done[u]:
+            continue
+        done[u] = True
+        for e in range(int(row[u]), int(row[u + 1])):
+            v = int(col[e])
+            c = min(best[u], O.weight(wmode, seed, u, v))
+            if c > best[v] and not done[v]:
+                best[v], par[v] = c, u
+                heapq.heappush(heap, (-c, v))
+    assert np.array_equal(np.array(best, dtype=np.uint32), width)
+    return par
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_widest_path_brute_force_tiny(seed):
+    rng = np.random.default_rng(300 + seed)
+    nv = int(rng.integers(2, 8))
+    ne = int(rng.integers(0, 3 * nv))
+    src = rng.integers(0, nv, ne).astype(np.uint32)
+    dst = rng.integers(0, nv, ne).astype(np.uint32)  # self-loops and parallel edges included
+    order = np.lexsort((dst, src))
+    row = np.zeros(nv + 1, dtype=np.uint64)
+    np.cumsum(np.bincount(src, minlength=nv), out=row[1:])
+    col = dst[order]
+    for wmode in ("hash128", "seeded", "one"):
+        for s0 in range(nv):
+            exp = _widest_brute(nv, row, col, s0, wmode, 5)
+            assert np.array_equal(O.widest_path(row, col, s0, wmode, 5), exp), (seed, wmode, s0)
+
+
+def test_widest_path_fixed_point_and_certificate():
+    rng = np.random.default_rng(7)  # uniform random digraph, mean degree 6: most vertices reached
+    nv, ne = 512, 3072
+    k = np.unique(rng.integers(0, nv, ne).astype(np.int64) << 32 | rng.integers(0, nv, ne))
+    row = np.zeros(nv + 1, dtype=np.uint64)
+    np.cumsum(np.bincount(k >> 32, minlength=nv), out=row[1:])
+    col = (k & 0xFFFFFFFF).astype(np.uint32)
+    src = 3
+    for wmode in ("seeded", "hash128"):
+        wd = O.widest_path(row, col, src, wmode, 13)
+        assert np.array_equal(wd, _widest_fixed_point(row, col, src, wmode, 13))
+        assert (wd > 0).sum() > 100
+        par = _widest_parents(row, col, src, wd, wmode, 13)
+        assert O.widest_certificate(row, col, src, wd, par, wmode, 13) == 0
+        # a reached vertex two parent steps below the source
+        deep = [int(v) for v in np.nonzero((wd > 0) & (par != O.INF))[0] if par[v] != src and par[par[v]] != O.INF]
+        v = deep[len(deep) // 2]
+        for delta in (-1, +1):  # a narrower or wider width
+            bad = wd.copy()
+            bad[v] = int(bad[v]) + delta
+            assert O.widest_certificate(row, col, src, bad, par, wmode, 13) > 0
+        cyc = par.copy()
+        cyc[par[v]] = v  # a 2-cycle of parents
+        assert O.widest_certificate(row, col, src, wd, cyc, wmode, 13) > 0
+        badp = par.copy()
+        badp[v] = O.INF  # a reached vertex without a parent
+        assert O.widest_certificate(row, col, src, wd, badp, wmode, 13) > 0
