@@ -23,7 +23,10 @@ each read the 4.2-billion-edge in-store, mostly over the host link), so the
 C3 part times --c3-steps snapshots per path (default 2, after --c3-warmup);
 --steps/--warmup apply to the secondary C2 line (incremental SSSP + BFS from
 source 0 on R-MAT scale 22 EF16, 100k + 100k per snapshot, HBM-resident),
-printed in the same JSON object under "c2".
+printed in the same JSON object under "c2", and a C4 key: 64 BFS + 64 SSSP
+queries on R-MAT scale 24 EF16 under batches of log-uniform size in [1e3, 1e6],
+the batched engine against single-source engines on a stratified sample of
+the same sources (--no-c4 skips it).
 
 Prints one JSON line. `--gpus` above 1 is not supported (one GPU per box).
 """
@@ -351,7 +354,7 @@ def run_c2(args):
     engines = {}
     for a in algos:
         timing = 0 if args.no_timing else sgraph.FLAG_KERNEL_TIMING | (0 if args.all_kernel_timing else sgraph.FLAG_TIME_SWEEP)
-        flags = timing | (sgraph.FLAG_GRAPH_ROUNDS if args.graph_rounds else 0)
+        flags = timing | (sgraph.FLAG_GRAPH_ROUNDS if args.graph_rounds else 0) | (sgraph.FLAG_IN_EDGES if args.in_edges else 0)
         e = sgraph.Engine(a, weight_mode="seeded", weight_seed=wseed, flags=flags, wave_budget=args.wave_budget,
                           heavy_degree=args.heavy_degree, round_blocks=args.round_blocks,
                           rounds_per_sync=args.rounds_per_sync)
@@ -587,6 +590,10 @@ def main():
     ap.add_argument("--rounds-per-sync", type=int, default=0, help="rounds enqueued per host check (0: engine default)")
     ap.add_argument("--graph-rounds", action="store_true", help="submit each batch of rounds as one HIP graph")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched (shared-graph) measurement")
+    ap.add_argument("--no-c4", action="store_true", help="default line: skip the C4 batched-queries key")
+    ap.add_argument("--in-edges", action="store_true",
+                    help="C2 engines keep the in-edge store: the recompute after deletions reads the in-lists of the "
+                         "reset vertices instead of sweeping every live edge")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 256 MiB (5 times) to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE")
     ap.add_argument("--traffic", type=float, default=None,
@@ -716,6 +723,17 @@ def main():
         "c3_wall_s": round(t_c3, 1),
         "c2": c2,
     }
+    if not args.no_c4:
+        # C4: 64 BFS + 64 SSSP queries on R-MAT scale 24 EF16 under batches of
+        # log-uniform size in [1e3, 1e6]: batched engine vs single-source engines
+        a4 = argparse.Namespace(**vars(args))
+        a4.scale, a4.edge_factor, a4.queries, a4.max_update, a4.steps, a4.warmup = 24, 16, 128, 1000000, 4, 1
+        log("[bench] C4 batched queries ...")
+        t_c4 = time.perf_counter()
+        c4 = run_batch(a4)
+        c4["wall_s"] = round(time.perf_counter() - t_c4, 1)
+        out["c4"] = {k: c4[k] for k in ("metric", "value", "unit", "vs_plain", "plain", "batched_reached_mean",
+                                         "latency_p50_ms", "latency_p99_ms", "config", "steps", "warmup", "wall_s")}
     if not args.no_cpu_baseline:
         log("[bench] C3 CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline_prcc(args.seed, 15)

@@ -215,7 +215,21 @@ __global__ void __launch_bounds__(256) k_cache_fill(uint32_t nv, const unsigned 
         const uint64_t old = coff_old[v];
         const uint32_t *src = old != ~0ull ? ccol_old + old : col + off[v];
         if (old == ~0ull) f += d;
-        for (uint32_t e = lane_id(); e < d; e += kWave) ccol_new[dst + e] = src[e];
+        // four loads in flight per lane: over the host link a load takes
+        // microseconds, so one at a time would leave the link idle
+        for (uint32_t e0 = 0; e0 < d; e0 += 4 * kWave) {
+            uint32_t x[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t e = e0 + j * kWave + lane_id();
+                x[j] = e < d ? src[e] : 0u;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t e = e0 + j * kWave + lane_id();
+                if (e < d) ccol_new[dst + e] = x[j];
+            }
+        }
     }
     if (lane_id() == 0 && f) atomicAdd(fetched, f);
 }
@@ -1109,12 +1123,12 @@ public:
         }
     }
 
-    // HBM mirror of the leading slots of a host-resident edge array (see
-    // GraphStore): read paths take those slots from HBM.
+    // HBM copy of the leading live edges of a host-resident edge array (see
+    // GraphStore): the tile sweeps read those edges from HBM.
     uint32_t *mirror_ = nullptr;
-    uint64_t mirror_slots_ = 0;
+    uint64_t mirror_live_ = 0;
     GraphView gview() const {
-        GraphView g{nv_, off_, deg_, col_, nullptr, nullptr, cap_, nullptr, mirror_, std::min(mirror_slots_, cap_)};
+        GraphView g{nv_, off_, deg_, col_, nullptr, nullptr, cap_, nullptr, mirror_, std::min(mirror_live_, ne_)};
         if (cache_slots_) {
             g.ccol = ccol_[ccur_];
             g.coff = coff_;
@@ -2084,41 +2098,36 @@ public:
         refresh_mirror();
     }
     // With the store in pinned host memory, the device edge budget holds an
-    // HBM copy of its leading slots (every pull round reads the whole store;
-    // those slots then come from HBM instead of over the host link). Re-copied
-    // after every batch.
+    // HBM copy of its leading live edges, gaps left out (every pull round
+    // reads the whole store; those edges then come from HBM instead of over
+    // the host link). Rebuilt after every batch from the live slots (the
+    // live-edge order shifts with every insert and delete).
     void refresh_mirror() {
         if (!host_edges_ || cfg_.edge_budget_gb <= 0) return;
-        const uint64_t want = std::min<uint64_t>(cap_, (uint64_t)(cfg_.edge_budget_gb * 1073741824.0 / 4.0));
+        const uint64_t cap = (uint64_t)(cfg_.edge_budget_gb * 1073741824.0 / 4.0);
+        const uint64_t want = std::min<uint64_t>(ne_, cap);
+        mirror_live_ = 0;
         if (want == 0) return;
         if (want > mirror_cap_) {
             dfree(mirror_);
             mirror_ = nullptr;
-            mirror_ = dalloc<uint32_t>(want);
-            mirror_cap_ = want;
+            mirror_cap_ = std::min<uint64_t>(cap, want + want / 8);  // room for growth
+            mirror_ = dalloc<uint32_t>(mirror_cap_);
         }
-        mirror_slots_ = want;
-        mirror_live_dirty_ = true;
-        SG_HIP(hipMemcpyAsync(mirror_, col_, want * 4, hipMemcpyHostToDevice, stream_));
+        GraphView g = gview();  // without a mirror: the fill reads the host store
+        g.mirror = nullptr;
+        g.mlive = 0;
+        const uint64_t nt = ntiles();
+        hipLaunchKernelGGL(k_mirror_fill, dim3(sweep_grid(k_mirror_fill, nt)), dim3(kSweepBlock), 0, stream_, g,
+                           (const unsigned long long *)dpre_, tiles_, mirror_, want);
+        SG_LAUNCHED();
+        mirror_live_ = want;
     }
     uint64_t mirror_cap_ = 0;
-    // Mirrored slots and the live edges among them (the first vertex whose
-    // range ends past the mirror splits its list; its mirrored part counts).
-    uint64_t mirror_live_ = 0;
-    bool mirror_live_dirty_ = true;
+    // Mirror capacity (entries) and the live edges it holds.
     void mirror_counts(uint64_t &slots, uint64_t &live) {
-        slots = std::min(mirror_slots_, cap_);
-        live = 0;
-        if (!slots) return;
-        if (!mirror_live_dirty_) { live = mirror_live_; return; }
-        std::vector<uint64_t> off((size_t)nv_ + 1);
-        std::vector<uint32_t> deg(nv_);
-        SG_HIP(hipMemcpyAsync(off.data(), off_, off.size() * 8, hipMemcpyDeviceToHost, stream_));
-        SG_HIP(hipMemcpyAsync(deg.data(), deg_, deg.size() * 4, hipMemcpyDeviceToHost, stream_));
-        sync();
-        for (uint32_t v = 0; v < nv_ && off[v] < slots; ++v) live += std::min<uint64_t>(deg[v], slots - off[v]);
-        mirror_live_ = live;
-        mirror_live_dirty_ = false;
+        slots = mirror_ ? mirror_cap_ : 0;
+        live = std::min(mirror_live_, ne_);
     }
     void refresh_tiles() { tiles_ = sweep_tiles(); }
     uint64_t ntiles() const { return ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile; }

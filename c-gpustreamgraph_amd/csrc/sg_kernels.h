@@ -51,10 +51,11 @@ struct GraphView {
     const uint64_t *coff;  // [nv] start of v's copy in ccol, ~0 = not cached
     uint64_t cap;
     uint8_t *hot;          // [nv] activations this snapshot (null when unused)
-    // HBM mirror of slots [0, msplit) of a col in pinned host memory (read
-    // paths only; updates write col). msplit = 0: no mirror.
+    // HBM copy of the live edges [0, mlive) of a col in pinned host memory, in
+    // live-edge order (edge k of the merge path, vertex by vertex: no gaps),
+    // read by the tile sweeps; list walks and updates use col. mlive = 0: none.
     const uint32_t *mirror;
-    uint64_t msplit;
+    uint64_t mlive;
 };
 
 // Where a relax walk reads u's list: its cached copy when there is one.
@@ -66,7 +67,6 @@ __device__ __forceinline__ uint64_t list_start(const GraphView &g, uint32_t u) {
     return g.off[u];
 }
 __device__ __forceinline__ uint32_t edge_at(const GraphView &g, uint64_t s) {
-    if (s < g.msplit) return g.mirror[s];
     return s < g.cap ? g.col[s] : g.ccol[s - g.cap];
 }
 // Activation count of the cache's hotness window (non-atomic and saturating,

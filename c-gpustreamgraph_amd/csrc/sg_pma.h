@@ -652,7 +652,7 @@ __device__ __forceinline__ void sweep_tile(const GraphView &g, const unsigned lo
                 ++n_edges;
                 const uint64_t slot = !wide ? off0 + (uint64_t)((long long)sh.base[o] + (long long)p)
                                             : g.off[src] + (e0 + p - dpre[src]);
-                vv[j] = slot < g.msplit ? g.mirror[slot] : colp[slot - slot_lo];
+                vv[j] = e0 + p < g.mlive ? g.mirror[e0 + p] : colp[slot - slot_lo];
             }
         }
     }

@@ -284,6 +284,39 @@ __global__ void __launch_bounds__(kSweepBlock) k_trav_pull(GraphView rg, const u
     block_count<kSweepWaves>(n_done, s_red, io.stats + 1);
 }
 
+// ---- HBM mirror of a host-resident in-edge store ---------------------------------
+// mirror[k] = live edge k (merge-path order) for k < mlive, read from the
+// store in host memory through the tile sweep (live slots only, coalesced).
+// g must carry no mirror itself. Tiles are in edge order, so a wave stops at
+// its first tile past mlive.
+__global__ void __launch_bounds__(kSweepBlock) k_mirror_fill(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
+                                                             uint32_t *mirror, uint64_t mlive) {
+    __shared__ SweepWave s_wave[kSweepWaves];
+    const unsigned long long E = dpre[g.nv];
+    const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
+    const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+    SweepWave &sh = s_wave[w];
+    uint32_t n_edges = 0;
+    TilePipe tp;
+    tp.tend = min_u64(ntiles, tiles.cap - 1);
+    tp.nw = (unsigned long long)gridDim.x * kSweepWaves;
+    tp.t = (unsigned long long)blockIdx.x * kSweepWaves + w;
+    tp.start(g, dpre, tiles);
+    while (tp.t < tp.tend) {
+        const TileHead h = tp.h;
+        if (h.e0 >= mlive) break;
+        uint32_t vv[kSweepPerLane];
+        sweep_tile(g, dpre, h, tp.pre, g.col, 0, 0u, g.nv, sh, vv, n_edges);
+        tp.advance(g, dpre, tiles);
+#pragma unroll
+        for (int j = 0; j < kSweepPerLane; ++j) {
+            const uint32_t p = j * kWave + lane;
+            if (p < h.ne && h.e0 + p < mlive) mirror[h.e0 + p] = vv[j];
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // ---- traversal recompute from the in-lists of invalidated vertices -------------
 // After deletions, every invalidated vertex v must see the best value any
 // in-neighbour u offers: PushValue(best(u)) + w(u, v) (for CC the label of u).

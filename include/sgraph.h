@@ -171,9 +171,10 @@ typedef struct sg_stats {
     uint64_t pull_rounds;         /* PageRank rounds run as pulls over the in-edge
                                      store (cumulative)                                 */
     uint64_t in_capacity;         /* slots of the in-edge store (0 = none)             */
-    uint64_t in_mirror_slots;     /* leading in-store slots copied to HBM (host-resident
-                                     store under an edge budget; 0 = none)           */
-    uint64_t in_mirror_live;      /* live edges among those slots                      */
+    uint64_t in_mirror_slots;     /* entries of the HBM copy of the in-store's leading
+                                     live edges (host-resident store under an edge
+                                     budget; 0 = none)                                 */
+    uint64_t in_mirror_live;      /* live edges it holds (the first ones in vertex order) */
 } sg_stats;
 
 /* What sg_gather copies out. */
