@@ -42,6 +42,12 @@
 #include "sgraph.h"
 #include "sg_engine_core.h"
 
+// Waves of a pull pass, in 64ths, that take the tiles beyond the in-store's
+// HBM mirror (a tuning constant; the host link saturates with few waves).
+#ifndef SG_PULL_HOST_WAVES64
+#define SG_PULL_HOST_WAVES64 12
+#endif
+
 static_assert(sizeof(sg_config) == 104, "sg_config layout");
 static_assert(sizeof(sg_stats) == 240, "sg_stats layout");
 
@@ -54,15 +60,29 @@ thread_local std::string g_create_error = "no error";
 // ---------------------------------------------------------------------------
 class PageRankEngine : public Engine {
 public:
-    using Engine::Engine;
+    // With the edges in host memory, every dense round pulls over all
+    // in-edges, while push rounds touch few lists: the -cache HBM budget then
+    // extends the in-store's HBM copy (read by every pull round) instead of
+    // caching hot out-lists of the forward store (the reference's cache holds
+    // out-lists for its push rounds, graph_datum.cuh:151-163). The HBM spent
+    // on edges stays the edge budget plus the cache.
+    uint64_t in_cache_bytes_ = 0;
+    explicit PageRankEngine(const sg_config &c) : Engine(c) {
+        if (c.pull_divisor != 0xFFFFFFFFu && c.cache_level) {
+            in_cache_bytes_ = 4ull * kCacheSlotsByLevel[std::min<uint32_t>(c.cache_level, 4)];
+            cfg_.cache_level = 0;
+        }
+    }
     bool reads_old_lists() const override { return true; }
     api::PageRank op_{};
     float *value_ = nullptr, *buffer_ = nullptr;
     double *d_sum_ = nullptr;
     // pull rounds (cfg.pull_divisor != UINT32_MAX): the in-edge store, the
-    // tagged pushes of dense rounds and the partial sums of cut in-lists
+    // frontier bitmaps and pushes of dense rounds, the partial sums of cut
+    // in-lists
     std::unique_ptr<GraphStore> rev_;
-    unsigned long long *xb_ = nullptr;
+    uint32_t *fb_[2] = {nullptr, nullptr};
+    uint32_t *pv_ = nullptr;
     float *accg_ = nullptr;
 
     uint64_t aux_device_bytes() const override { return rev_ ? rev_->dev_bytes_ : 0; }
@@ -77,11 +97,15 @@ public:
     virtual bool wants_in_store() const { return cfg_.pull_divisor != 0xFFFFFFFFu; }
     void after_load(const uint64_t *row, const uint32_t *col) override {
         if (!wants_in_store()) return;
-        rev_ = make_in_store(*this, row, col);
+        rev_ = make_in_store(*this, row, col, in_cache_bytes_);
         if (cfg_.pull_divisor == 0xFFFFFFFFu) return;  // the store serves another program only
-        xb_ = dalloc<unsigned long long>(nv_);
+        const uint64_t words = ((uint64_t)nv_ + 31) / 32;
+        for (auto &b : fb_) {
+            b = dalloc<uint32_t>(words);
+            fill<uint32_t>(b, words, 0u);
+        }
+        pv_ = dalloc<uint32_t>(nv_);
         accg_ = dalloc<float>(nv_);
-        fill<unsigned long long>(xb_, nv_, ~0ull);  // tag 0xFFFFFFFF: no round's
         fill<uint32_t>((uint32_t *)accg_, nv_, 0u);
         sync();
     }
@@ -95,7 +119,14 @@ public:
         const uint64_t div = cfg_.pull_divisor ? cfg_.pull_divisor : (rev_ && rev_->host_edges_ ? 4 : 24);
         return std::max<unsigned long long>(1ull, ne_ / div);
     }
-    bool pulls() const { return xb_ != nullptr; }
+    bool pulls() const { return pv_ != nullptr; }
+    PullTake take_state(uint32_t R) const { return PullTake{fb_[R & 1], pv_}; }
+    // Round R's frontier bitmap starts empty (it last served round R - 2).
+    void clear_take(uint32_t R) {
+        if (!pulls()) return;
+        const uint64_t words = ((uint64_t)nv_ + 31) / 32;
+        round_launch(k_fill_u32, dim3(grid_for(words, 256, 2048)), dim3(256), fb_[R & 1], words, 0u);
+    }
     PassIO pr_io(PassIO io) const {
         if (!pulls()) return io;
         const uint32_t R = io.round;
@@ -106,7 +137,14 @@ public:
         io.dense_thr = dense_thr();
         return io;
     }
-    PullPR pull_state(uint32_t R) const { return PullPR{xb_, buffer_, accg_, stamp_, R, (float)cfg_.pr_epsilon}; }
+    // Share of a pull pass's waves that take the tiles beyond the HBM mirror
+    // (64ths): the host link saturates with far fewer waves than the HBM
+    // gathers need.
+    static constexpr uint32_t kHostWaves64 = SG_PULL_HOST_WAVES64;
+    PullPR pull_state(uint32_t R, bool split = true) const {
+        return PullPR{fb_[R & 1], (const float *)pv_, buffer_, accg_, stamp_, R, (float)cfg_.pr_epsilon,
+                      split && rev_ ? rev_->tile_split() : nullptr, kHostWaves64};
+    }
 
     // Pull pass of round R over the whole in-edge store (resident store).
     // force = 0: the kernels return unless the round is dense (device flag).
@@ -125,7 +163,7 @@ public:
     // ranges stream through HBM staging buffers while earlier ones are pulled.
     void pull_streamed(const PassIO &io, bool timed) {
         const uint64_t nt = rev_->ntiles();
-        const PullPR ps = pull_state(io.round);
+        const PullPR ps = pull_state(io.round, false);  // segments are staged in HBM: no split
         const GraphView rg = rev_->gview();
         const unsigned long long *rdpre = rev_->dpre_;
         const SweepTiles rt = rev_->tiles_;
@@ -155,8 +193,9 @@ public:
     void pr_round(const PassIO &io0, bool do_time) {
         const PassIO io = pr_io(io0);
         const uint32_t R = io.round;
-        launch_pass(io, VPageRank{op_, value_, buffer_, xb_, R}, ERelax<api::PageRank>{op_, buffer_, stamp_, R + 1, 0, 0},
-                    do_time);
+        clear_take(R);
+        launch_pass(io, VPageRank{op_, value_, buffer_, take_state(R)},
+                    ERelax<api::PageRank>{op_, buffer_, stamp_, R + 1, 0, 0}, do_time);
         if (pulls()) pull_resident(io, 0, do_time && timing(SG_KCLASS_PULL));
     }
 
@@ -195,8 +234,9 @@ public:
             if (timed) io.stats = stats_ + 8;
             io = pr_io(io);
             const bool dense = h_ecnt()[R % 3] > io.dense_thr;
-            launch_pass(io, VPageRank{op_, value_, buffer_, xb_, R}, ERelax<api::PageRank>{op_, buffer_, stamp_, R + 1, 0, 0},
-                        do_time);
+            clear_take(R);
+            launch_pass(io, VPageRank{op_, value_, buffer_, take_state(R)},
+                        ERelax<api::PageRank>{op_, buffer_, stamp_, R + 1, 0, 0}, do_time);
             if (dense) {
                 pull_streamed(io, do_time && timing(SG_KCLASS_PULL));
                 count_pull_bytes(timed);

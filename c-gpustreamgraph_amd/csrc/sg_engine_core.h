@@ -73,6 +73,9 @@ static std::mutex g_scratch_call_mu[64];        // one call at a time per device
 // ---------------------------------------------------------------------------
 // PMA / batch kernels
 // ---------------------------------------------------------------------------
+// HBM edge-cache slots of -cache levels 0..4 (graph_datum.cuh:151-163).
+constexpr uint64_t kCacheSlotsByLevel[5] = {0ull, 268435455ull, 536870921ull, 805306365ull, 1ull << 30};
+
 struct Ctl {
     unsigned long long overflow;  // some touched vertex outgrows its range
     unsigned long long ne_delta;  // sum of (new - old) degrees (two's complement)
@@ -990,9 +993,8 @@ public:
         seg_v_ = dalloc<uint32_t>(nseg_ + 1);
         seg_aux_ = dalloc<unsigned long long>(2 * ((size_t)nseg_ + 1));
         if (host_edges_) {
-            static const uint64_t kCacheSlots[5] = {0ull, 268435455ull, 536870921ull, 805306365ull, 1ull << 30};
             // no larger than twice the edge array: the rest could never be filled
-            cache_slots_ = std::min<uint64_t>(kCacheSlots[std::min<uint32_t>(cfg_.cache_level, 4)], 2 * cap_);
+            cache_slots_ = std::min<uint64_t>(kCacheSlotsByLevel[std::min<uint32_t>(cfg_.cache_level, 4)], 2 * cap_);
             SG_HIP(hipStreamCreateWithFlags(&cstream_, hipStreamNonBlocking));
             for (int b = 0; b < kMaxStage; ++b) {
                 SG_HIP(hipEventCreateWithFlags(&seg_copied_[b], hipEventDisableTiming));
@@ -2102,11 +2104,13 @@ public:
     // reads the whole store; those edges then come from HBM instead of over
     // the host link). Rebuilt after every batch from the live slots (the
     // live-edge order shifts with every insert and delete).
+    uint64_t mirror_extra_bytes_ = 0;  // HBM beyond the edge budget the owner gives the mirror
     void refresh_mirror() {
-        if (!host_edges_ || cfg_.edge_budget_gb <= 0) return;
-        const uint64_t cap = (uint64_t)(cfg_.edge_budget_gb * 1073741824.0 / 4.0);
-        const uint64_t want = std::min<uint64_t>(ne_, cap);
         mirror_live_ = 0;
+        if (!host_edges_) return;
+        const double bytes = std::max(0.0, cfg_.edge_budget_gb) * 1073741824.0 + (double)mirror_extra_bytes_;
+        const uint64_t cap = (uint64_t)(bytes / 4.0);
+        const uint64_t want = std::min<uint64_t>(ne_, cap);
         if (want == 0) return;
         if (want > mirror_cap_) {
             dfree(mirror_);
@@ -2121,9 +2125,19 @@ public:
         hipLaunchKernelGGL(k_mirror_fill, dim3(sweep_grid(k_mirror_fill, nt)), dim3(kSweepBlock), 0, stream_, g,
                            (const unsigned long long *)dpre_, tiles_, mirror_, want);
         SG_LAUNCHED();
+        if (!tsplit_) tsplit_ = dalloc<unsigned long long>(1);
+        hipLaunchKernelGGL(k_tile_split, dim3(1), dim3(1), 0, stream_, tiles_, (const unsigned long long *)dpre_, nv_,
+                           (unsigned long long)want, tsplit_);
+        SG_LAUNCHED();
         mirror_live_ = want;
     }
     uint64_t mirror_cap_ = 0;
+    unsigned long long *tsplit_ = nullptr;  // first tile past the mirror (k_tile_split)
+    // The tile split for pull passes: null without a mirror, or with every
+    // live edge in it.
+    const unsigned long long *tile_split() const {
+        return host_edges_ && mirror_live_ && mirror_live_ < ne_ ? tsplit_ : nullptr;
+    }
     // Mirror capacity (entries) and the live edges it holds.
     void mirror_counts(uint64_t &slots, uint64_t &live) {
         slots = mirror_ ? mirror_cap_ : 0;
@@ -2219,7 +2233,10 @@ static void transpose_csr_device(hipStream_t st, uint32_t nv, uint64_t ne, const
 // The in-edge store of an engine, built from the CSR it loads (its own device
 // budget counted with the owner's, its work on the owner's stream). In-lists
 // are built on the host for small graphs, on the device from 2^24 edges.
-static std::unique_ptr<GraphStore> make_in_store(Engine &owner, const uint64_t *row, const uint32_t *col) {
+// mirror_extra_bytes: HBM beyond the edge budget for the store's mirror when
+// its edges live in host memory.
+static std::unique_ptr<GraphStore> make_in_store(Engine &owner, const uint64_t *row, const uint32_t *col,
+                                                 uint64_t mirror_extra_bytes = 0) {
     std::vector<uint64_t> trow;
     std::vector<uint32_t> tcol;
     if (owner.ne_ >= (1ull << 24)) transpose_csr_device(owner.stream_, owner.nv_, owner.ne_, row, col, trow, tcol);
@@ -2232,6 +2249,7 @@ static std::unique_ptr<GraphStore> make_in_store(Engine &owner, const uint64_t *
     in->budget_peer_ = &owner;
     in->init_device(owner.stream_);
     in->load_csr(owner.nv_, owner.ne_, trow.data(), tcol.data());
+    in->mirror_extra_bytes_ = mirror_extra_bytes;
     in->refresh_tiles();
     in->refresh_mirror();
     return in;
@@ -2264,20 +2282,25 @@ struct TravProgram {
     unsigned long long *buffer = nullptr;
     int phase_inval = Engine::kPhaseInval, phase_relax = Engine::kPhaseRelax;
     // pull rounds (direction-optimising): the in-edge store they walk, the
-    // tagged pushes of dense rounds; null: push rounds only
+    // frontier bitmaps and pushes of dense rounds; null: push rounds only
     GraphStore *pull_in = nullptr;
-    unsigned long long *xb = nullptr;
+    uint32_t *fb[2] = {nullptr, nullptr};
+    uint32_t *pv = nullptr;
     uint32_t pull_div = kTravPullDivisor;
 
     // Pull rounds over in (cfg.pull_divisor: 0 = default share, UINT32_MAX =
     // never, UINT32_MAX - 1... = a round pulls above E / divisor).
     void enable_pull(Engine &e, GraphStore *in, uint32_t divisor) {
         if constexpr (has_merge<Op>::value) {
-            if (!in || divisor == 0xFFFFFFFFu || xb) return;
+            if (!in || divisor == 0xFFFFFFFFu || pv) return;
             pull_in = in;
             pull_div = divisor ? divisor : kTravPullDivisor;
-            xb = e.dalloc<unsigned long long>(e.nv_);
-            e.template fill<unsigned long long>(xb, e.nv_, ~0ull);  // tag 0xFFFFFFFF: no round's
+            const uint64_t words = ((uint64_t)e.nv_ + 31) / 32;
+            for (auto &b : fb) {
+                b = e.dalloc<uint32_t>(words);
+                e.template fill<uint32_t>(b, words, 0u);
+            }
+            pv = e.dalloc<uint32_t>(e.nv_);
         }
     }
 
@@ -2308,7 +2331,7 @@ struct TravProgram {
     // the frontier's out-edges.
     uint32_t relax_rounds(Engine &e, bool timed, int phase) {
         if constexpr (has_merge<Op>::value) {
-            if (xb) return relax_rounds_pull(e, timed, phase);
+            if (pv) return relax_rounds_pull(e, timed, phase);
         }
         return e.run_rounds([&](uint32_t sv) { return relax_fns(e, sv); }, timed, phase);
     }
@@ -2330,7 +2353,10 @@ struct TravProgram {
             io.ecnt_zero = e.ecnt_ + ((R + 2) % 3);
             io.fdeg = e.deg_;
             io.dense_thr = thr;
-            e.launch_pass(io, VCombinePull<Op>{op, value, buffer, xb, R},
+            // round R's frontier bitmap starts empty (it last served round R - 2)
+            const uint64_t words = ((uint64_t)e.nv_ + 31) / 32;
+            e.round_launch(k_fill_u32, dim3(grid_for(words, 256, 2048)), dim3(256), fb[R & 1], words, 0u);
+            e.launch_pass(io, VCombinePull<Op>{op, value, buffer, PullTake{fb[R & 1], pv}},
                           ERelax<Op>{op, buffer, e.stamp_, R + 1, e.cfg_.weight_mode, e.cfg_.weight_seed}, do_time);
             // the pull kernel returns at once unless the round is dense (device flag)
             const uint64_t nt = pull_in->ntiles();
@@ -2338,12 +2364,13 @@ struct TravProgram {
             e.round_launch(k_trav_pull<Op>, dim3(e.sweep_grid(k_trav_pull<Op>, nt)), dim3(kSweepBlock), pull_in->gview(),
                            (const unsigned long long *)pull_in->dpre_, pull_in->tiles_, (const uint32_t *)pull_in->col_,
                            (uint64_t)0,
-                           PullTrav<Op>{op, xb, buffer, e.stamp_, R, e.cfg_.weight_mode, e.cfg_.weight_seed}, io, 0);
+                           PullTrav<Op>{op, fb[R & 1], pv, buffer, e.stamp_, R, e.cfg_.weight_mode, e.cfg_.weight_seed},
+                           io, 0);
             e.end_timed(time_pull && !e.batch_);
         }, timed, phase);
-        // 4 B per in-edge slot read, 8 B tagged push gathered per in-edge, 8 B
-        // buffer + 4 B stamp per destination
-        if (time_pull) e.kt_bytes_[SG_KCLASS_PULL] += (e.st_.pull_rounds - p0) * (12ull * e.ne_ + 12ull * e.nv_);
+        // 4 B per in-edge slot read, its bitmap word (4 B), 8 B buffer + 4 B
+        // stamp per destination
+        if (time_pull) e.kt_bytes_[SG_KCLASS_PULL] += (e.st_.pull_rounds - p0) * (8ull * e.ne_ + 12ull * e.nv_);
         return r;
     }
 

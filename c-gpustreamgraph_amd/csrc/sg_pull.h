@@ -23,12 +23,21 @@
 
 namespace sg {
 
-// Tag of a take: round R in the high word, the pushed f32 in the low word.
-// A pull pass of round R reads only the pushes tagged R, so the words of
-// earlier rounds never need clearing.
-__device__ __forceinline__ unsigned long long pr_tag(uint32_t round, float p) {
-    return ((unsigned long long)round << 32) | __float_as_uint(p);
-}
+// The pushes a dense round takes: bit u of the round's frontier bitmap and
+// the push bits pv[u]. A pull pass tests the bitmap for every in-edge - one
+// bit per vertex, small enough to stay in the caches (16 MiB at 2^27
+// vertices) - and reads pv only for the in-neighbours in the frontier, so the
+// per-edge random read no longer touches a word per vertex in HBM. Two
+// bitmaps alternate between rounds; round R's is cleared before round R (it
+// last served round R - 2).
+struct PullTake {
+    uint32_t *bits;  // [ceil(nv / 32)] this round's frontier
+    uint32_t *pv;    // [nv] push bits of the frontier's vertices
+    __device__ void take(uint32_t u, uint32_t push) const {
+        pv[u] = push;
+        atomicOr(&bits[u >> 5], 1u << (u & 31));
+    }
+};
 
 // Vertex functor of PageRank rounds (CombineValueBuffer, hybrid_pr.cu:66-85).
 // In a dense round take() records the push instead of walking the list.
@@ -37,25 +46,32 @@ struct VPageRank {
     api::PageRank op;
     float *value;
     float *buffer;
-    unsigned long long *xb;  // [nv] tagged pushes (null: no pull store)
-    uint32_t round;
+    PullTake t;
     __device__ bool operator()(uint32_t u, uint32_t deg, uint32_t &push, bool &self) const {
         self = false;
         auto p = op.CombineValueBuffer(u, &value[u], &buffer[u], deg);
         push = __float_as_uint(p.value);
         return p.scheduled;
     }
-    __device__ void take(uint32_t u, uint32_t push) const { xb[u] = pr_tag(round, __uint_as_float(push)); }
+    __device__ void take(uint32_t u, uint32_t push) const { t.take(u, push); }
 };
 
 // State a pull pass reads and updates.
 struct PullPR {
-    const unsigned long long *xb;  // tagged pushes of this round's frontier
+    const uint32_t *bits;          // this round's frontier bitmap
+    const float *pv;               // pushes of the frontier's vertices
     float *res;                    // residual (the operator's buffer)
     float *accg;                   // partial sums of lists cut by tile boundaries (zero between rounds)
     uint32_t *stamp;               // round stamps (frontier de-duplication)
     uint32_t round;                // R
     float eps;                     // activity threshold (IsActiveNode / IsActiveNode_INC)
+    // Store in host memory with an HBM mirror: the first tile whose edges
+    // reach past the mirror (null: no split). Tiles before it read HBM only,
+    // the rest mostly the host link; host_waves64 / 64 of the waves take the
+    // latter, the others the former, so that both run at once instead of
+    // every wave stalling on the host link in turn.
+    const unsigned long long *tsplit;
+    uint32_t host_waves64;
 };
 
 __device__ __forceinline__ bool pull_dense(const PassIO &io, int force) {
@@ -96,29 +112,44 @@ __global__ void __launch_bounds__(kSweepBlock) k_pr_pull(GraphView rg, const uns
     OutStage st{sh.out, 0, kSweepOutStage};
     uint32_t n_edges = 0;
     unsigned long long n_done = 0;
+    unsigned long long t0 = t_begin, t1 = min_u64(min_u64(t_end, ntiles), tiles.cap - 1);
+    unsigned long long gw = (unsigned long long)blockIdx.x * kSweepWaves + w, nwv = (unsigned long long)gridDim.x * kSweepWaves;
+    if (p.tsplit) {
+        const unsigned long long ts = *p.tsplit;
+        const unsigned long long nh = max(1ull, nwv * p.host_waves64 / 64);
+        if (ts > t0 && ts < t1 && nh < nwv) {
+            if (gw < nh) {
+                t0 = ts;
+                nwv = nh;
+            } else {
+                t1 = ts;
+                gw -= nh;
+                nwv -= nh;
+            }
+        }
+    }
     TilePipe tp;
-    tp.tend = min_u64(min_u64(t_end, ntiles), tiles.cap - 1);
-    tp.nw = (unsigned long long)gridDim.x * kSweepWaves;
-    tp.t = t_begin + (unsigned long long)blockIdx.x * kSweepWaves + w;
+    tp.tend = t1;
+    tp.nw = nwv;
+    tp.t = t0 + gw;
     tp.start(rg, dpre, tiles);
     while (tp.t < tp.tend) {
         uint32_t vv[kSweepPerLane];
         const TileHead h = tp.h;
         sweep_tile(rg, dpre, h, tp.pre, colp, slot_lo, dst_lo, dst_hi, sh, vv, n_edges);
         tp.advance(rg, dpre, tiles);
-        unsigned long long xw[kSweepPerLane];
+        uint32_t bw[kSweepPerLane];
 #pragma unroll
-        for (int j = 0; j < kSweepPerLane; ++j) xw[j] = vv[j] != kGap ? p.xb[vv[j]] : 0ull;
+        for (int j = 0; j < kSweepPerLane; ++j) bw[j] = vv[j] != kGap ? p.bits[vv[j] >> 5] : 0u;
+        float c[kSweepPerLane];
+#pragma unroll
+        for (int j = 0; j < kSweepPerLane; ++j) c[j] = (bw[j] >> (vv[j] & 31)) & 1u ? p.pv[vv[j]] : 0.0f;
         __builtin_amdgcn_wave_barrier();
         for (uint32_t i = lane; i < h.span; i += kWave) acc[i] = 0.0f;
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int j = 0; j < kSweepPerLane; ++j) {
-            if (vv[j] != kGap && (uint32_t)(xw[j] >> 32) == p.round) {
-                const float c = __uint_as_float((uint32_t)xw[j]);
-                if (c != 0.0f) atomicAdd(&acc[sh.own[j * kWave + lane]], c);
-            }
-        }
+        for (int j = 0; j < kSweepPerLane; ++j)
+            if (c[j] != 0.0f) atomicAdd(&acc[sh.own[j * kWave + lane]], c[j]);
         __builtin_amdgcn_wave_barrier();
         for (uint32_t i0 = 0; i0 < h.span; i0 += kWave) {
             const uint32_t i = i0 + lane;
@@ -195,21 +226,21 @@ struct VCombinePull {
     Op op;
     typename Op::value_t *value;
     typename Op::buffer_t *buffer;
-    unsigned long long *xb;  // [nv] tagged pushes
-    uint32_t round;
+    PullTake t;
     __device__ bool operator()(uint32_t u, uint32_t deg, uint32_t &push, bool &self) const {
         self = false;
         auto p = op.CombineValueBuffer(u, &value[u], &buffer[u], deg);
         push = to_bits<typename Op::push_t>(p.value);
         return p.scheduled;
     }
-    __device__ void take(uint32_t u, uint32_t push) const { xb[u] = ((unsigned long long)round << 32) | push; }
+    __device__ void take(uint32_t u, uint32_t push) const { t.take(u, push); }
 };
 
 template <class Op>
 struct PullTrav {
     Op op;
-    const unsigned long long *xb;  // tagged pushes of this round's frontier
+    const uint32_t *bits;          // this round's frontier bitmap
+    const uint32_t *pv;            // pushes of the frontier's vertices
     typename Op::buffer_t *buffer;
     uint32_t *stamp;               // round stamps (frontier de-duplication)
     uint32_t round;                // R
@@ -247,18 +278,24 @@ __global__ void __launch_bounds__(kSweepBlock) k_trav_pull(GraphView rg, const u
         const TileHead h = tp.h;
         sweep_tile(rg, dpre, h, tp.pre, colp, slot_lo, 0u, rg.nv, sh, vv, n_edges);
         tp.advance(rg, dpre, tiles);
-        unsigned long long xw[kSweepPerLane];
+        uint32_t bw[kSweepPerLane];
 #pragma unroll
-        for (int j = 0; j < kSweepPerLane; ++j) xw[j] = vv[j] != kGap ? p.xb[vv[j]] : 0ull;
+        for (int j = 0; j < kSweepPerLane; ++j) bw[j] = vv[j] != kGap ? p.bits[vv[j] >> 5] : 0u;
+        uint32_t pw[kSweepPerLane];
+#pragma unroll
+        for (int j = 0; j < kSweepPerLane; ++j) {
+            bw[j] = (bw[j] >> (vv[j] & 31)) & 1u;
+            pw[j] = bw[j] ? p.pv[vv[j]] : 0u;
+        }
         for (uint32_t i = lane; i < h.span; i += kWave) acc[i] = ident;
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int j = 0; j < kSweepPerLane; ++j) {
-            if (vv[j] != kGap && (uint32_t)(xw[j] >> 32) == p.round) {
+            if (bw[j]) {
                 const uint32_t own = sh.own[j * kWave + lane];
                 const uint32_t u = vv[j], v = h.v0 + own;
                 const uint32_t wt = Op::kWeighted ? sg_weight(p.wmode, p.wseed, u, v) : 0u;
-                p.op.AccumulateBuffer(u, v, wt, &acc[own], from_bits<typename Op::push_t>((uint32_t)xw[j]));
+                p.op.AccumulateBuffer(u, v, wt, &acc[own], from_bits<typename Op::push_t>(pw[j]));
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -315,6 +352,21 @@ __global__ void __launch_bounds__(kSweepBlock) k_mirror_fill(GraphView g, const 
         }
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// First tile whose edges reach edge index mlive (every tile before it lies
+// wholly in the mirror); ntiles when none. One thread.
+__global__ void k_tile_split(SweepTiles tiles, const unsigned long long *dpre, uint32_t nv, unsigned long long mlive,
+                             unsigned long long *out) {
+    const unsigned long long E = dpre[nv];
+    const unsigned long long nt = min_u64((nv + E + kSweepTile - 1) / kSweepTile, tiles.cap - 1);
+    unsigned long long lo = 0, hi = nt;
+    while (lo < hi) {
+        const unsigned long long mid = (lo + hi) / 2;
+        if (tiles.e[mid + 1] > mlive) hi = mid;
+        else lo = mid + 1;
+    }
+    *out = lo;
 }
 
 // ---- traversal recompute from the in-lists of invalidated vertices -------------

@@ -92,7 +92,10 @@ typedef struct sg_config {
     uint32_t heavy_degree;    /* lists longer than this always become chunks (0 = 2048)  */
     uint32_t cache_level;     /* -cache: HBM edge cache of {0, 268435455, 536870921,
                                  805306365, 2^30} u32 slots for levels 0..4
-                                 (graph_datum.cuh:151-163); used with host edges   */
+                                 (graph_datum.cuh:151-163); used with host edges.
+                                 PageRank engines with pull rounds spend it on the
+                                 in-edge store's HBM copy instead (every pull round
+                                 reads all in-edges); the rest cache hot out-lists */
     double edge_budget_gb;    /* device budget of the edge array in GiB
                                  (-device_graph_budget_gb): an edge array larger than
                                  this lives in pinned host memory (0 = no cap)       */

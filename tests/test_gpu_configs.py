@@ -88,7 +88,8 @@ def c3_shape(scale, budget, cache, nsnap, batch, exact):
         e.init()
         st = e.stats()
         say(f"[gpu] C3 shape static solve done: {st['rounds']} rounds")
-        assert st["host_edges"] == 1 and st["in_mirror_slots"] > 0
+        # the in-store's HBM copy (edge budget + cache) holds part of its live edges
+        assert st["host_edges"] == 1 and 0 < st["in_mirror_live"] < st["ne"]
         r, c = row, col
         for t in range(nsnap):
             e.apply(ins[t], dele[t])
@@ -121,12 +122,14 @@ def c3_shape(scale, budget, cache, nsnap, batch, exact):
         e.close()
 
 
+# At C3 the edge budget and the cache (8 + 4 GiB) hold about 70 % of the
+# in-edges in HBM; the scaled-down shapes keep a similar share.
 def test_c3_shape_scale20_exact():
-    c3_shape(20, 0.05, 1, 2, 100000, exact=True)
+    c3_shape(20, 0.07, 0, 2, 100000, exact=True)
 
 
 def test_c3_shape_scale24_certificates():
-    c3_shape(24, 0.5, 2, 2, 1000000, exact=False)
+    c3_shape(24, 0.5, 1, 2, 1000000, exact=False)
 
 
 @pytest.mark.timeout(900)

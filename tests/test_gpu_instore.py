@@ -107,15 +107,15 @@ def test_prcc_resident():
 
 
 def test_prcc_host_edges():
-    st = prcc_stream(12, 16, 3, 2000, seed=89, flags=sgraph.FLAG_HOST_EDGES, segments=7, cache_level=1)
-    assert st["host_edges"] == 1 and st["in_capacity"] > 0
+    st = prcc_stream(12, 16, 3, 2000, seed=89, flags=sgraph.FLAG_HOST_EDGES, segments=7, cache_level=0)
+    assert st["host_edges"] == 1 and st["in_capacity"] > 0 and st["in_mirror_live"] == 0
 
 
 def test_prcc_host_edges_with_hbm_mirror():
     # the edge budget (below the edge array) holds an HBM copy of part of the
     # in-store: CC's in-list seeds and PageRank's pulls read those slots there
-    st = prcc_stream(12, 16, 3, 2000, seed=93, edge_budget_gb=1e-4, segments=7, cache_level=1)
-    assert st["host_edges"] == 1
+    st = prcc_stream(12, 16, 3, 2000, seed=93, edge_budget_gb=1e-4, segments=7, cache_level=0)
+    assert st["host_edges"] == 1 and 0 < st["in_mirror_live"] < st["ne"]
 
 
 def test_prcc_deferred_pagerank():

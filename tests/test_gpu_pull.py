@@ -120,21 +120,24 @@ def test_pr_pull_host_edges(mode, stream):
     nv, row, col = sgraph.gen_rmat(12, 16, seed=57)
     ins, dele = sgraph.gen_updates(row, col, 12, 3, 2000, 2000, seed=58, hub_frac=0.8, hub_top=0.01)
     flags = sgraph.FLAG_HOST_EDGES | (sgraph.FLAG_PULL_STREAM if stream else 0)
-    st = pr_stream(row, col, list(zip(ins, dele)), pull_divisor=div, flags=flags, segments=7, cache_level=1)
-    assert st["host_edges"] == 1 and st["pull_rounds"] > 0
+    # cache_level 0: no HBM copy of the in-store, every pull reads host memory
+    st = pr_stream(row, col, list(zip(ins, dele)), pull_divisor=div, flags=flags, segments=7, cache_level=0)
+    assert st["host_edges"] == 1 and st["pull_rounds"] > 0 and st["in_mirror_live"] == 0
     assert (st["streamed_bytes"] > 0) == stream
 
 
-@pytest.mark.parametrize("budget", [1e-4, 1.0])
-def test_pr_pull_host_edges_with_hbm_mirror(budget):
-    # an edge budget below the edge array: both stores live in pinned host
-    # memory and the budget holds an HBM copy of the in-store's leading slots
-    # (a quarter of them, or all), refreshed after every batch
+@pytest.mark.parametrize("budget,cache", [(1e-4, 0), (1.0, 0), (1e-4, 1)])
+def test_pr_pull_host_edges_with_hbm_mirror(budget, cache):
+    # both stores in pinned host memory; the edge budget (plus, for PageRank,
+    # the -cache budget) holds an HBM copy of the in-store's leading live
+    # edges - part of them, or all - rebuilt after every batch
     nv, row, col = sgraph.gen_rmat(12, 16, seed=65)
     ins, dele = sgraph.gen_updates(row, col, 12, 3, 2000, 2000, seed=66, hub_frac=0.8, hub_top=0.01)
-    st = pr_stream(row, col, list(zip(ins, dele)), check_store=True, edge_budget_gb=budget,
+    st = pr_stream(row, col, list(zip(ins, dele)), check_store=True, edge_budget_gb=budget, cache_level=cache,
                    flags=sgraph.FLAG_HOST_EDGES, segments=7, pull_divisor=sgraph.PULL_ALWAYS)
     assert st["host_edges"] == 1 and st["pull_rounds"] > 0
+    partial = budget * 2**30 / 4 < st["ne"] and cache == 0
+    assert 0 < st["in_mirror_live"] and (st["in_mirror_live"] < st["ne"]) == partial
 
 
 def test_in_store_built_on_the_device():
