@@ -97,7 +97,7 @@ public:
     virtual bool wants_in_store() const { return cfg_.pull_divisor != 0xFFFFFFFFu; }
     void after_load(const uint64_t *row, const uint32_t *col) override {
         if (!wants_in_store()) return;
-        rev_ = make_in_store(*this, row, col, in_cache_bytes_);
+        rev_ = make_in_store(*this, row, col, in_cache_bytes_, cfg_.pull_divisor != 0xFFFFFFFFu);
         if (cfg_.pull_divisor == 0xFFFFFFFFu) return;  // the store serves another program only
         const uint64_t words = ((uint64_t)nv_ + 31) / 32;
         for (auto &b : fb_) {
@@ -164,7 +164,8 @@ public:
     void pull_streamed(const PassIO &io, bool timed) {
         const uint64_t nt = rev_->ntiles();
         const PullPR ps = pull_state(io.round, false);  // segments are staged in HBM: no split
-        const GraphView rg = rev_->gview();
+        GraphView rg = rev_->gview();
+        rg.tail = nullptr;  // the staged segments stand in for the host copy
         const unsigned long long *rdpre = rev_->dpre_;
         const SweepTiles rt = rev_->tiles_;
         begin_timed(timed, SG_KCLASS_PULL);

@@ -1129,8 +1129,9 @@ public:
     // GraphStore): the tile sweeps read those edges from HBM.
     uint32_t *mirror_ = nullptr;
     uint64_t mirror_live_ = 0;
+    uint32_t *tail_ = nullptr;  // gap-free host copy of the live edges past the mirror (GraphStore)
     GraphView gview() const {
-        GraphView g{nv_, off_, deg_, col_, nullptr, nullptr, cap_, nullptr, mirror_, std::min(mirror_live_, ne_)};
+        GraphView g{nv_, off_, deg_, col_, nullptr, nullptr, cap_, nullptr, mirror_, std::min(mirror_live_, ne_), tail_};
         if (cache_slots_) {
             g.ccol = ccol_[ccur_];
             g.coff = coff_;
@@ -2105,25 +2106,39 @@ public:
     // the host link). Rebuilt after every batch from the live slots (the
     // live-edge order shifts with every insert and delete).
     uint64_t mirror_extra_bytes_ = 0;  // HBM beyond the edge budget the owner gives the mirror
+    // With want_tail_ (an owner whose pull rounds read every in-edge), the
+    // live edges past the mirror also get a gap-free copy in pinned host
+    // memory: a pull then reads them as contiguous runs over the host link
+    // instead of short lists scattered between gaps. Rebuilt with the mirror.
+    bool want_tail_ = false;
+    uint64_t tail_cap_ = 0;
     void refresh_mirror() {
         mirror_live_ = 0;
         if (!host_edges_) return;
         const double bytes = std::max(0.0, cfg_.edge_budget_gb) * 1073741824.0 + (double)mirror_extra_bytes_;
         const uint64_t cap = (uint64_t)(bytes / 4.0);
         const uint64_t want = std::min<uint64_t>(ne_, cap);
-        if (want == 0) return;
+        const uint64_t rest = want_tail_ ? ne_ - want : 0;
+        if (want == 0 && rest == 0) return;
         if (want > mirror_cap_) {
             dfree(mirror_);
             mirror_ = nullptr;
             mirror_cap_ = std::min<uint64_t>(cap, want + want / 8);  // room for growth
             mirror_ = dalloc<uint32_t>(mirror_cap_);
         }
-        GraphView g = gview();  // without a mirror: the fill reads the host store
+        if (rest > tail_cap_) {
+            hfree(tail_);
+            tail_ = nullptr;
+            tail_cap_ = rest + rest / 8;
+            tail_ = halloc<uint32_t>(tail_cap_);
+        }
+        GraphView g = gview();  // without copies: the fill reads the host store
         g.mirror = nullptr;
         g.mlive = 0;
+        g.tail = nullptr;
         const uint64_t nt = ntiles();
         hipLaunchKernelGGL(k_mirror_fill, dim3(sweep_grid(k_mirror_fill, nt)), dim3(kSweepBlock), 0, stream_, g,
-                           (const unsigned long long *)dpre_, tiles_, mirror_, want);
+                           (const unsigned long long *)dpre_, tiles_, mirror_, want, rest ? tail_ : nullptr);
         SG_LAUNCHED();
         if (!tsplit_) tsplit_ = dalloc<unsigned long long>(1);
         hipLaunchKernelGGL(k_tile_split, dim3(1), dim3(1), 0, stream_, tiles_, (const unsigned long long *)dpre_, nv_,
@@ -2138,6 +2153,7 @@ public:
     const unsigned long long *tile_split() const {
         return host_edges_ && mirror_live_ && mirror_live_ < ne_ ? tsplit_ : nullptr;
     }
+
     // Mirror capacity (entries) and the live edges it holds.
     void mirror_counts(uint64_t &slots, uint64_t &live) {
         slots = mirror_ ? mirror_cap_ : 0;
@@ -2235,8 +2251,10 @@ static void transpose_csr_device(hipStream_t st, uint32_t nv, uint64_t ne, const
 // are built on the host for small graphs, on the device from 2^24 edges.
 // mirror_extra_bytes: HBM beyond the edge budget for the store's mirror when
 // its edges live in host memory.
+// tail: the owner's pull rounds read every in-edge (a gap-free host copy of
+// the edges past the mirror pays for its rebuild).
 static std::unique_ptr<GraphStore> make_in_store(Engine &owner, const uint64_t *row, const uint32_t *col,
-                                                 uint64_t mirror_extra_bytes = 0) {
+                                                 uint64_t mirror_extra_bytes = 0, bool tail = false) {
     std::vector<uint64_t> trow;
     std::vector<uint32_t> tcol;
     if (owner.ne_ >= (1ull << 24)) transpose_csr_device(owner.stream_, owner.nv_, owner.ne_, row, col, trow, tcol);
@@ -2250,6 +2268,7 @@ static std::unique_ptr<GraphStore> make_in_store(Engine &owner, const uint64_t *
     in->init_device(owner.stream_);
     in->load_csr(owner.nv_, owner.ne_, trow.data(), tcol.data());
     in->mirror_extra_bytes_ = mirror_extra_bytes;
+    in->want_tail_ = tail;
     in->refresh_tiles();
     in->refresh_mirror();
     return in;

@@ -56,6 +56,10 @@ struct GraphView {
     // read by the tile sweeps; list walks and updates use col. mlive = 0: none.
     const uint32_t *mirror;
     uint64_t mlive;
+    // Gap-free copy of the live edges [mlive, E) in pinned host memory (tail[k
+    // - mlive] = edge k), read by the tile sweeps instead of col's gappy
+    // slots; null: none.
+    const uint32_t *tail;
 };
 
 // Where a relax walk reads u's list: its cached copy when there is one.

@@ -652,7 +652,8 @@ __device__ __forceinline__ void sweep_tile(const GraphView &g, const unsigned lo
                 ++n_edges;
                 const uint64_t slot = !wide ? off0 + (uint64_t)((long long)sh.base[o] + (long long)p)
                                             : g.off[src] + (e0 + p - dpre[src]);
-                vv[j] = e0 + p < g.mlive ? g.mirror[e0 + p] : colp[slot - slot_lo];
+                const unsigned long long k = e0 + p;
+                vv[j] = k < g.mlive ? g.mirror[k] : g.tail ? g.tail[k - g.mlive] : colp[slot - slot_lo];
             }
         }
     }

@@ -321,13 +321,14 @@ __global__ void __launch_bounds__(kSweepBlock) k_trav_pull(GraphView rg, const u
     block_count<kSweepWaves>(n_done, s_red, io.stats + 1);
 }
 
-// ---- HBM mirror of a host-resident in-edge store ---------------------------------
-// mirror[k] = live edge k (merge-path order) for k < mlive, read from the
-// store in host memory through the tile sweep (live slots only, coalesced).
-// g must carry no mirror itself. Tiles are in edge order, so a wave stops at
-// its first tile past mlive.
+// ---- read copies of a host-resident in-edge store ----------------------------------
+// mirror[k] = live edge k (merge-path order) for k < mlive and, with a tail,
+// tail[k - mlive] = edge k for the rest, read from the store in host memory
+// through the tile sweep (live slots only, coalesced). g must carry no copy
+// itself. Tiles are in edge order, so without a tail a wave stops at its
+// first tile past mlive.
 __global__ void __launch_bounds__(kSweepBlock) k_mirror_fill(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
-                                                             uint32_t *mirror, uint64_t mlive) {
+                                                             uint32_t *mirror, uint64_t mlive, uint32_t *tail) {
     __shared__ SweepWave s_wave[kSweepWaves];
     const unsigned long long E = dpre[g.nv];
     const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
@@ -341,14 +342,18 @@ __global__ void __launch_bounds__(kSweepBlock) k_mirror_fill(GraphView g, const 
     tp.start(g, dpre, tiles);
     while (tp.t < tp.tend) {
         const TileHead h = tp.h;
-        if (h.e0 >= mlive) break;
+        if (h.e0 >= mlive && !tail) break;
         uint32_t vv[kSweepPerLane];
         sweep_tile(g, dpre, h, tp.pre, g.col, 0, 0u, g.nv, sh, vv, n_edges);
         tp.advance(g, dpre, tiles);
 #pragma unroll
         for (int j = 0; j < kSweepPerLane; ++j) {
             const uint32_t p = j * kWave + lane;
-            if (p < h.ne && h.e0 + p < mlive) mirror[h.e0 + p] = vv[j];
+            const unsigned long long k = h.e0 + p;
+            if (p < h.ne) {
+                if (k < mlive) mirror[k] = vv[j];
+                else if (tail) tail[k - mlive] = vv[j];
+            }
         }
         __builtin_amdgcn_wave_barrier();
     }
