@@ -50,10 +50,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy), or None."""
-    path = os.path.join(ROOT, "profiles", "k_sweep_edges_sssp_bfs_rmat22_ef16_b100k.pmc.txt")
+def measured_traffic(kernel, summary="k_sweep_edges_sssp_bfs_rmat22_ef16_b100k.pmc.txt"):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC summary
+    under profiles/ (FETCH_SIZE + WRITE_SIZE, calibrated against a plain
+    copy), or None."""
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None, None
     vals = []
@@ -669,6 +670,8 @@ def main():
 
     pulls = c3["pulls"]
     ms_pull = c3["ms_pull"]
+    # memory traffic per dense pull launch, rocprofv3 FETCH_SIZE + WRITE_SIZE at C3 (committed summary)
+    traffic_c3, traffic_c3_src = measured_traffic("sg::k_pr_pull", "k_pr_pull_c3_prcc_rmat27.pmc.txt")
     out = {
         "metric": "updated-edge GTEPS + p50/p99 snapshot latency, 1 MI355X, batched vs plain",
         "value": round(c3["batched"], 4),
@@ -706,7 +709,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(c3["achieved"] / HBM_PEAK_GBS, 4) if c3["achieved"] else None,
-            "traffic": None,
+            "traffic": traffic_c3,
+            "traffic_source": traffic_c3_src,
             "launches": pulls,
             "avg_launch_us": round(ms_pull * 1e3 / pulls, 1) if pulls else None,
             "algorithmic_bytes_per_launch": round(c3["b_pull"] / pulls) if pulls else None,
