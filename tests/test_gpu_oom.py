@@ -80,3 +80,21 @@ def test_cache_serves_edges():
     st = e.stats()
     assert st["edges_from_cache"] > 0
     e.close()
+
+
+def test_device_budget_refuses_with_ebudget():
+    # device_budget_gb caps the engine's accounted device memory: a load that
+    # needs more fails with SG_EBUDGET and a message naming the budget, and a
+    # fresh engine with room for the load works
+    nv, row, col = sgraph.gen_rmat(14, 16, seed=61)
+    e = sgraph.Engine("sssp", device_budget_gb=1e-4)  # 100 KiB: below the vertex state alone
+    with pytest.raises(sgraph.SgError) as x:
+        e.load_csr(row, col)
+    assert x.value.status == sgraph.SG_EBUDGET and "budget" in str(x.value)
+    e.close()
+    f = sgraph.Engine("sssp", device_budget_gb=1.0)
+    f.load_csr(row, col)
+    f.init(0)
+    assert np.array_equal(f.values(), O.sssp(row, col, 0))
+    assert f.stats()["device_bytes"] <= 2**30
+    f.close()
