@@ -1032,6 +1032,9 @@ sg_status sg_create(const sg_config *cfg, sg_engine **out) {
     } catch (const std::bad_alloc &) {
         sg::g_create_error = "host allocation failed";
         return SG_ENOMEM;
+    } catch (...) {
+        sg::g_create_error = "unexpected exception";
+        return SG_EDEVICE;
     }
     *out = new (std::nothrow) sg_engine{std::move(impl)};
     if (!*out) { sg::g_create_error = "host allocation failed"; return SG_ENOMEM; }
