@@ -97,7 +97,8 @@ public:
     virtual bool wants_in_store() const { return cfg_.pull_divisor != 0xFFFFFFFFu; }
     void after_load(const uint64_t *row, const uint32_t *col) override {
         if (!wants_in_store()) return;
-        rev_ = make_in_store(*this, row, col, in_cache_bytes_, cfg_.pull_divisor != 0xFFFFFFFFu);
+        const bool pulling = cfg_.pull_divisor != 0xFFFFFFFFu;
+        rev_ = make_in_store(*this, row, col, in_cache_bytes_, pulling, pulling);
         if (cfg_.pull_divisor == 0xFFFFFFFFu) return;  // the store serves another program only
         const uint64_t words = ((uint64_t)nv_ + 31) / 32;
         for (auto &b : fb_) {
@@ -120,7 +121,7 @@ public:
         return std::max<unsigned long long>(1ull, ne_ / div);
     }
     bool pulls() const { return pv_ != nullptr; }
-    PullTake take_state(uint32_t R) const { return PullTake{fb_[R & 1], pv_}; }
+    PullTake take_state(uint32_t R) const { return PullTake{fb_[R & 1], pv_, rev_ ? rev_->relabel_ : nullptr}; }
     // Round R's frontier bitmap starts empty (it last served round R - 2).
     void clear_take(uint32_t R) {
         if (!pulls()) return;
@@ -426,7 +427,7 @@ public:
         if (q == SG_Q_IN_DEGREE || q == SG_Q_IN_OFFSETS || q == SG_Q_IN_SLOTS) {
             if (!rev_) fail(SG_EINVAL, "this engine keeps no in-edge store (pull_divisor = UINT32_MAX)");
             sync();
-            rev_->gather_graph(q - SG_Q_IN_DEGREE + SG_Q_DEGREE, out, bytes);
+            rev_->gather_in(q - SG_Q_IN_DEGREE + SG_Q_DEGREE, out, bytes);
             return;
         }
         gather_graph(q, out, bytes);

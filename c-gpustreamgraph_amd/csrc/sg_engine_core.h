@@ -2081,6 +2081,17 @@ public:
     }
 };
 
+// A batch's (u, v) pairs as a relabelled in-edge store records them: u
+// replaced by its id there (ids >= nv stay, to be dropped as invalid).
+__global__ void k_relabel_pairs(const uint32_t *in, uint64_t n, const uint32_t *relabel, uint32_t nv, uint32_t *out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t u = in[2 * i];
+        out[2 * i] = u < nv ? relabel[u] : u;
+        out[2 * i + 1] = in[2 * i + 1];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // In-edge store: a PMA of the reversed graph owned by another engine, on that
 // engine's stream, updated by the owner's batches read as (v, u). No vertex
@@ -2091,8 +2102,29 @@ class GraphStore final : public Engine {
 public:
     using Engine::Engine;
     SweepTiles tiles_{};  // merge-path tiles over the in-lists (refreshed after every batch)
+    // With relabelled sources (make_in_store): relabel_[u] is the id the
+    // in-lists record for source u, unlabel_ the inverse; null: ids as given.
+    uint32_t *relabel_ = nullptr, *unlabel_ = nullptr;
+    uint32_t *rl_ins_ = nullptr, *rl_del_ = nullptr;
+    size_t rl_ins_cap_ = 0, rl_del_cap_ = 0;
     void init(const uint32_t *, uint32_t) override { fail(SG_EINVAL, "a graph store has no vertex program"); }
     void apply(const uint32_t *d_ins, uint64_t nins, const uint32_t *d_del, uint64_t ndel) override {
+        if (relabel_) {  // the batch's sources as the in-lists record them
+            if (nins) {
+                ensure(rl_ins_, rl_ins_cap_, 2 * nins);
+                hipLaunchKernelGGL(k_relabel_pairs, dim3(grid_for(nins, 256, 4096)), dim3(256), 0, stream_, d_ins, nins,
+                                   (const uint32_t *)relabel_, nv_, rl_ins_);
+                SG_LAUNCHED();
+                d_ins = rl_ins_;
+            }
+            if (ndel) {
+                ensure(rl_del_, rl_del_cap_, 2 * ndel);
+                hipLaunchKernelGGL(k_relabel_pairs, dim3(grid_for(ndel, 256, 4096)), dim3(256), 0, stream_, d_del, ndel,
+                                   (const uint32_t *)relabel_, nv_, rl_del_);
+                SG_LAUNCHED();
+                d_del = rl_del_;
+            }
+        }
         const bool overflow = prepare(d_ins, nins, d_del, ndel);
         commit(overflow);
         finish_batch();
@@ -2161,7 +2193,18 @@ public:
     }
     void refresh_tiles() { tiles_ = sweep_tiles(); }
     uint64_t ntiles() const { return ((uint64_t)nv_ + ne_ + kSweepTile - 1) / kSweepTile; }
-    void gather(uint32_t q, void *out, size_t bytes) override { gather_graph(q, out, bytes); }
+    void gather(uint32_t q, void *out, size_t bytes) override { gather_in(q, out, bytes); }
+    // The store's arrays with sources as vertex ids (relabelled slots mapped back).
+    void gather_in(uint32_t q, void *out, size_t bytes) {
+        gather_graph(q, out, bytes);
+        if (q != SG_Q_SLOTS || !unlabel_) return;
+        std::vector<uint32_t> un(nv_);
+        SG_HIP(hipMemcpyAsync(un.data(), unlabel_, (size_t)nv_ * 4, hipMemcpyDeviceToHost, stream_));
+        sync();
+        uint32_t *slots = (uint32_t *)out;
+        for (uint64_t i = 0; i < cap_; ++i)
+            if (slots[i] != kGap) slots[i] = un[slots[i]];
+    }
 };
 
 // In-lists of a CSR (counting sort by target; sources ascending in each list).
@@ -2177,13 +2220,15 @@ static void transpose_csr(uint32_t nv, uint64_t ne, const uint64_t *row, const u
 }
 
 // keys[e] = target << 32 | source of every edge e (one wave per source).
+// relabel (null: identity): the id the in-lists record for source v.
 __global__ void __launch_bounds__(kBlock) k_edge_keys_t(uint32_t nv, const uint64_t *row, const uint32_t *col,
-                                                        unsigned long long *keys) {
+                                                        const uint32_t *relabel, unsigned long long *keys) {
     const uint64_t gw = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) / kWave;
     for (uint64_t v = gw; v < nv; v += nw) {
         const uint64_t b = row[v], d = row[v + 1] - b;
-        for (uint64_t k = lane_id(); k < d; k += kWave) keys[b + k] = ((unsigned long long)col[b + k] << 32) | v;
+        const uint32_t id = relabel ? relabel[v] : (uint32_t)v;
+        for (uint64_t k = lane_id(); k < d; k += kWave) keys[b + k] = ((unsigned long long)col[b + k] << 32) | id;
     }
 }
 // Sorted keys (hi << 32 | lo) as a CSR: row[v] = first key with hi >= v, col[i] = lo.
@@ -2207,10 +2252,10 @@ __global__ void k_keys_to_csr(uint32_t nv, uint64_t ne, const unsigned long long
 // transpose_csr gives). Transient device memory of about 20 bytes per edge,
 // outside the engine's budget (a load-time scratch, freed before returning).
 static void transpose_csr_device(hipStream_t st, uint32_t nv, uint64_t ne, const uint64_t *row, const uint32_t *col,
-                                 std::vector<uint64_t> &trow, std::vector<uint32_t> &tcol) {
-    void *d_row = nullptr, *d_col = nullptr, *d_k0 = nullptr, *d_k1 = nullptr, *d_tmp = nullptr;
+                                 const uint32_t *rank, std::vector<uint64_t> &trow, std::vector<uint32_t> &tcol) {
+    void *d_row = nullptr, *d_col = nullptr, *d_k0 = nullptr, *d_k1 = nullptr, *d_tmp = nullptr, *d_rank = nullptr;
     auto release = [&]() {
-        for (void *p : {d_row, d_col, d_k0, d_k1, d_tmp})
+        for (void *p : {d_row, d_col, d_k0, d_k1, d_tmp, d_rank})
             if (p) (void)hipFree(p);
     };
     try {
@@ -2220,8 +2265,13 @@ static void transpose_csr_device(hipStream_t st, uint32_t nv, uint64_t ne, const
         SG_HIP(hipMalloc(&d_k1, (size_t)ne * 8));
         SG_HIP(hipMemcpyAsync(d_row, row, ((size_t)nv + 1) * 8, hipMemcpyHostToDevice, st));
         SG_HIP(hipMemcpyAsync(d_col, col, (size_t)ne * 4, hipMemcpyHostToDevice, st));
+        if (rank) {
+            SG_HIP(hipMalloc(&d_rank, (size_t)nv * 4));
+            SG_HIP(hipMemcpyAsync(d_rank, rank, (size_t)nv * 4, hipMemcpyHostToDevice, st));
+        }
         hipLaunchKernelGGL(k_edge_keys_t, dim3(grid_for(nv, kWavesPerBlock, 16384)), dim3(kBlock), 0, st, nv,
-                           (const uint64_t *)d_row, (const uint32_t *)d_col, (unsigned long long *)d_k0);
+                           (const uint64_t *)d_row, (const uint32_t *)d_col, (const uint32_t *)d_rank,
+                           (unsigned long long *)d_k0);
         SG_LAUNCHED();
         const int kb = 32 - __builtin_clz(std::max<uint32_t>(nv, 1));
         size_t tb = 0;
@@ -2253,12 +2303,38 @@ static void transpose_csr_device(hipStream_t st, uint32_t nv, uint64_t ne, const
 // its edges live in host memory.
 // tail: the owner's pull rounds read every in-edge (a gap-free host copy of
 // the edges past the mirror pays for its rebuild).
+// relabel: the in-lists record each source by its rank in descending
+// out-degree instead of its id (hubs first), so the per-in-edge gathers of a
+// pull pass (frontier bit and push of the source) fall on a few hot words
+// that stay in the caches; the store translates batches and queries.
 static std::unique_ptr<GraphStore> make_in_store(Engine &owner, const uint64_t *row, const uint32_t *col,
-                                                 uint64_t mirror_extra_bytes = 0, bool tail = false) {
+                                                 uint64_t mirror_extra_bytes = 0, bool tail = false, bool relabel = false) {
+    const uint32_t nv = owner.nv_;
+    std::vector<uint32_t> rank, unrank;
+    if (relabel) {  // counting sort by out-degree, descending; ties by id
+        uint64_t maxd = 0;
+        for (uint32_t u = 0; u < nv; ++u) maxd = std::max<uint64_t>(maxd, row[u + 1] - row[u]);
+        std::vector<uint64_t> start(maxd + 2, 0);
+        for (uint32_t u = 0; u < nv; ++u) ++start[maxd - (row[u + 1] - row[u]) + 1];
+        for (uint64_t d = 1; d < start.size(); ++d) start[d] += start[d - 1];
+        rank.resize(nv);
+        unrank.resize(nv);
+        for (uint32_t u = 0; u < nv; ++u) {
+            const uint32_t r = (uint32_t)start[maxd - (row[u + 1] - row[u])]++;
+            rank[u] = r;
+            unrank[r] = u;
+        }
+    }
     std::vector<uint64_t> trow;
     std::vector<uint32_t> tcol;
-    if (owner.ne_ >= (1ull << 24)) transpose_csr_device(owner.stream_, owner.nv_, owner.ne_, row, col, trow, tcol);
-    else transpose_csr(owner.nv_, owner.ne_, row, col, trow, tcol);
+    if (owner.ne_ >= (1ull << 24)) {
+        transpose_csr_device(owner.stream_, nv, owner.ne_, row, col, relabel ? rank.data() : nullptr, trow, tcol);
+    } else {
+        transpose_csr(nv, owner.ne_, row, col, trow, tcol);
+        if (relabel) {
+            for (uint64_t i = 0; i < owner.ne_; ++i) tcol[i] = rank[tcol[i]];
+        }
+    }
     sg_config rc = owner.cfg_;
     rc.flags &= SG_FLAG_HOST_EDGES;  // no timing, graphs or deferral of its own
     rc.cache_level = 0;
@@ -2267,6 +2343,13 @@ static std::unique_ptr<GraphStore> make_in_store(Engine &owner, const uint64_t *
     in->budget_peer_ = &owner;
     in->init_device(owner.stream_);
     in->load_csr(owner.nv_, owner.ne_, trow.data(), tcol.data());
+    if (relabel) {
+        in->relabel_ = in->dalloc<uint32_t>(nv);
+        in->unlabel_ = in->dalloc<uint32_t>(nv);
+        SG_HIP(hipMemcpyAsync(in->relabel_, rank.data(), (size_t)nv * 4, hipMemcpyHostToDevice, owner.stream_));
+        SG_HIP(hipMemcpyAsync(in->unlabel_, unrank.data(), (size_t)nv * 4, hipMemcpyHostToDevice, owner.stream_));
+        in->sync();  // the host vectors die with this call
+    }
     in->mirror_extra_bytes_ = mirror_extra_bytes;
     in->want_tail_ = tail;
     in->refresh_tiles();
@@ -2311,7 +2394,7 @@ struct TravProgram {
     // never, UINT32_MAX - 1... = a round pulls above E / divisor).
     void enable_pull(Engine &e, GraphStore *in, uint32_t divisor) {
         if constexpr (has_merge<Op>::value) {
-            if (!in || divisor == 0xFFFFFFFFu || pv) return;
+            if (!in || in->relabel_ || divisor == 0xFFFFFFFFu || pv) return;
             pull_in = in;
             pull_div = divisor ? divisor : kTravPullDivisor;
             const uint64_t words = ((uint64_t)e.nv_ + 31) / 32;
@@ -2375,7 +2458,7 @@ struct TravProgram {
             // round R's frontier bitmap starts empty (it last served round R - 2)
             const uint64_t words = ((uint64_t)e.nv_ + 31) / 32;
             e.round_launch(k_fill_u32, dim3(grid_for(words, 256, 2048)), dim3(256), fb[R & 1], words, 0u);
-            e.launch_pass(io, VCombinePull<Op>{op, value, buffer, PullTake{fb[R & 1], pv}},
+            e.launch_pass(io, VCombinePull<Op>{op, value, buffer, PullTake{fb[R & 1], pv, nullptr}},
                           ERelax<Op>{op, buffer, e.stamp_, R + 1, e.cfg_.weight_mode, e.cfg_.weight_seed}, do_time);
             // the pull kernel returns at once unless the round is dense (device flag)
             const uint64_t nt = pull_in->ntiles();
@@ -2425,7 +2508,8 @@ struct TravProgram {
                                    e.nv_, (const uint32_t *)e.inval_, list, lcnt, e.nv_, e.stats_ + 3);
                 SG_LAUNCHED();
                 PassIO io = e.seed_io(R, list, lcnt, 0);
-                const EPullSeed<Op> ef{op, buffer, e.stamp_, R, e.cfg_.weight_mode, e.cfg_.weight_seed};
+                const EPullSeed<Op> ef{op, buffer, e.stamp_, R, e.cfg_.weight_mode, e.cfg_.weight_seed,
+                                       (const uint32_t *)in->unlabel_};
                 const GraphView ig = in->gview();
                 e.launch_pass_view(ig, io, VAll{}, ef, false);
                 e.drain_seed_spill_view(ig, R, ef);  // long in-lists cut into chunks
@@ -2551,7 +2635,7 @@ public:
         if (q == SG_Q_IN_DEGREE || q == SG_Q_IN_OFFSETS || q == SG_Q_IN_SLOTS) {
             if (!in_) fail(SG_EINVAL, "this engine keeps no in-edge store (SG_FLAG_IN_EDGES)");
             sync();
-            return in_->gather_graph(q - SG_Q_IN_DEGREE + SG_Q_DEGREE, out, bytes);
+            return in_->gather_in(q - SG_Q_IN_DEGREE + SG_Q_DEGREE, out, bytes);
         }
         gather_graph(q, out, bytes);
     }

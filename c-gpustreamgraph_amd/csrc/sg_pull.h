@@ -31,11 +31,13 @@ namespace sg {
 // bitmaps alternate between rounds; round R's is cleared before round R (it
 // last served round R - 2).
 struct PullTake {
-    uint32_t *bits;  // [ceil(nv / 32)] this round's frontier
-    uint32_t *pv;    // [nv] push bits of the frontier's vertices
+    uint32_t *bits;           // [ceil(nv / 32)] this round's frontier
+    uint32_t *pv;             // [nv] push bits of the frontier's vertices
+    const uint32_t *relabel;  // the id the in-lists record for a vertex (null: its own)
     __device__ void take(uint32_t u, uint32_t push) const {
-        pv[u] = push;
-        atomicOr(&bits[u >> 5], 1u << (u & 31));
+        const uint32_t r = relabel ? relabel[u] : u;
+        pv[r] = push;
+        atomicOr(&bits[r >> 5], 1u << (r & 31));
     }
 };
 
@@ -392,9 +394,12 @@ struct EPullSeed {
     uint32_t stamp_val;
     uint32_t wmode;
     uint64_t wseed;
-    __device__ uint64_t load(uint32_t u) const { return ld_peek(&buffer[u]); }
-    // v: the invalidated vertex whose in-list is walked; u: one in-neighbour
-    __device__ bool apply(uint32_t v, uint32_t u, uint32_t, uint64_t bu) const {
+    const uint32_t *unlabel;  // vertex of each in-list id (null: ids are vertices)
+    __device__ uint32_t vertex(uint32_t x) const { return unlabel ? unlabel[x] : x; }
+    __device__ uint64_t load(uint32_t x) const { return ld_peek(&buffer[vertex(x)]); }
+    // v: the invalidated vertex whose in-list is walked; x: one in-list entry
+    __device__ bool apply(uint32_t v, uint32_t x, uint32_t, uint64_t bu) const {
+        const uint32_t u = vertex(x);
         const uint32_t w = Op::kWeighted ? sg_weight(wmode, wseed, u, v) : 0u;
         const auto p = op.PushValue(bu);
         if (!op.MayImprove(u, w, p, ld_peek(&buffer[v]))) return false;
