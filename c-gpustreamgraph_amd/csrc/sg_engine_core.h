@@ -1130,8 +1130,9 @@ public:
     uint32_t *mirror_ = nullptr;
     uint64_t mirror_live_ = 0;
     uint32_t *tail_ = nullptr;  // gap-free host copy of the live edges past the mirror (GraphStore)
+    uint32_t mbits_ = 32;       // bits per id in the mirror and the tail
     GraphView gview() const {
-        GraphView g{nv_, off_, deg_, col_, nullptr, nullptr, cap_, nullptr, mirror_, std::min(mirror_live_, ne_), tail_};
+        GraphView g{nv_, off_, deg_, col_, nullptr, nullptr, cap_, nullptr, mirror_, std::min(mirror_live_, ne_), tail_, mbits_};
         if (cache_slots_) {
             g.ccol = ccol_[ccur_];
             g.coff = coff_;
@@ -2144,25 +2145,43 @@ public:
     // instead of short lists scattered between gaps. Rebuilt with the mirror.
     bool want_tail_ = false;
     uint64_t tail_cap_ = 0;
+    // Ids are packed at ceil(log2 nv) bits (27 at 2^27 vertices): the same
+    // HBM holds 32 / 27 as many edges, and the host link moves that much less
+    // per tail edge. Packed words are ORed together at run ends, so the copies
+    // are zeroed first; the tail is packed in HBM and copied out in one piece.
+    uint32_t *tail_stage_ = nullptr;
+    static uint64_t packed_words(uint64_t n, uint32_t b) { return (n * b + 31) / 32 + 1; }
     void refresh_mirror() {
         mirror_live_ = 0;
         if (!host_edges_) return;
+        const uint32_t b = nv_ > 1 ? 32 - (uint32_t)__builtin_clz(nv_ - 1) : 1;
+        mbits_ = b > 31 ? 32 : b;
         const double bytes = std::max(0.0, cfg_.edge_budget_gb) * 1073741824.0 + (double)mirror_extra_bytes_;
-        const uint64_t cap = (uint64_t)(bytes / 4.0);
+        const uint64_t cap_words = (uint64_t)(bytes / 4.0);
+        const uint64_t cap = cap_words > 1 ? (cap_words - 1) * 32 / mbits_ : 0;  // ids that fit
         const uint64_t want = std::min<uint64_t>(ne_, cap);
         const uint64_t rest = want_tail_ ? ne_ - want : 0;
         if (want == 0 && rest == 0) return;
-        if (want > mirror_cap_) {
+        const uint64_t mwords = want ? packed_words(want, mbits_) : 0;
+        if (mwords > mirror_cap_) {
             dfree(mirror_);
             mirror_ = nullptr;
-            mirror_cap_ = std::min<uint64_t>(cap, want + want / 8);  // room for growth
+            mirror_cap_ = std::min<uint64_t>(cap_words, mwords + mwords / 8);  // room for growth
             mirror_ = dalloc<uint32_t>(mirror_cap_);
         }
-        if (rest > tail_cap_) {
+        const uint64_t twords = rest ? packed_words(rest, mbits_) : 0;
+        if (twords > tail_cap_) {
             hfree(tail_);
+            dfree(tail_stage_);
             tail_ = nullptr;
-            tail_cap_ = rest + rest / 8;
+            tail_stage_ = nullptr;
+            tail_cap_ = twords + twords / 8;
             tail_ = halloc<uint32_t>(tail_cap_);
+            tail_stage_ = dalloc<uint32_t>(tail_cap_);
+        }
+        if (mbits_ < 32) {
+            if (mwords) fill<uint32_t>(mirror_, mwords, 0u);
+            if (twords) fill<uint32_t>(tail_stage_, twords, 0u);
         }
         GraphView g = gview();  // without copies: the fill reads the host store
         g.mirror = nullptr;
@@ -2170,8 +2189,9 @@ public:
         g.tail = nullptr;
         const uint64_t nt = ntiles();
         hipLaunchKernelGGL(k_mirror_fill, dim3(sweep_grid(k_mirror_fill, nt)), dim3(kSweepBlock), 0, stream_, g,
-                           (const unsigned long long *)dpre_, tiles_, mirror_, want, rest ? tail_ : nullptr);
+                           (const unsigned long long *)dpre_, tiles_, mirror_, want, rest ? tail_stage_ : nullptr, mbits_);
         SG_LAUNCHED();
+        if (twords) SG_HIP(hipMemcpyAsync(tail_, tail_stage_, twords * 4, hipMemcpyDeviceToHost, stream_));
         if (!tsplit_) tsplit_ = dalloc<unsigned long long>(1);
         hipLaunchKernelGGL(k_tile_split, dim3(1), dim3(1), 0, stream_, tiles_, (const unsigned long long *)dpre_, nv_,
                            (unsigned long long)want, tsplit_);
@@ -2186,9 +2206,9 @@ public:
         return host_edges_ && mirror_live_ && mirror_live_ < ne_ ? tsplit_ : nullptr;
     }
 
-    // Mirror capacity (entries) and the live edges it holds.
+    // Mirror capacity (ids) and the live edges it holds.
     void mirror_counts(uint64_t &slots, uint64_t &live) {
-        slots = mirror_ ? mirror_cap_ : 0;
+        slots = mirror_ && mirror_cap_ > 1 ? (mirror_cap_ - 1) * 32 / mbits_ : 0;
         live = std::min(mirror_live_, ne_);
     }
     void refresh_tiles() { tiles_ = sweep_tiles(); }

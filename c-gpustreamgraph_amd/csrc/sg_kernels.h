@@ -53,14 +53,26 @@ struct GraphView {
     uint8_t *hot;          // [nv] activations this snapshot (null when unused)
     // HBM copy of the live edges [0, mlive) of a col in pinned host memory, in
     // live-edge order (edge k of the merge path, vertex by vertex: no gaps),
-    // read by the tile sweeps; list walks and updates use col. mlive = 0: none.
+    // ids packed at mbits bits each (packed_at), read by the tile sweeps; list
+    // walks and updates use col. mlive = 0: none.
     const uint32_t *mirror;
     uint64_t mlive;
-    // Gap-free copy of the live edges [mlive, E) in pinned host memory (tail[k
-    // - mlive] = edge k), read by the tile sweeps instead of col's gappy
+    // Gap-free packed copy of the live edges [mlive, E) in pinned host memory
+    // (id k - mlive = edge k), read by the tile sweeps instead of col's gappy
     // slots; null: none.
     const uint32_t *tail;
+    uint32_t mbits;  // bits per packed id (32: plain words)
 };
+
+// Id i of an array of b-bit ids packed back to back from bit 0 (b in [1, 32];
+// the array holds one spare word past its last id).
+__device__ __forceinline__ uint32_t packed_at(const uint32_t *p, unsigned long long i, uint32_t b) {
+    const unsigned long long bit = i * b;
+    const unsigned long long w = bit >> 5;
+    const uint64_t x = ((uint64_t)p[w + 1] << 32) | p[w];
+    const uint32_t v = (uint32_t)(x >> (bit & 31u));
+    return b == 32 ? v : v & ((1u << b) - 1u);
+}
 
 // Where a relax walk reads u's list: its cached copy when there is one.
 __device__ __forceinline__ uint64_t list_start(const GraphView &g, uint32_t u) {

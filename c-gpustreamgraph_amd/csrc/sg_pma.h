@@ -653,7 +653,9 @@ __device__ __forceinline__ void sweep_tile(const GraphView &g, const unsigned lo
                 const uint64_t slot = !wide ? off0 + (uint64_t)((long long)sh.base[o] + (long long)p)
                                             : g.off[src] + (e0 + p - dpre[src]);
                 const unsigned long long k = e0 + p;
-                vv[j] = k < g.mlive ? g.mirror[k] : g.tail ? g.tail[k - g.mlive] : colp[slot - slot_lo];
+                vv[j] = k < g.mlive ? packed_at(g.mirror, k, g.mbits)
+                        : g.tail    ? packed_at(g.tail, k - g.mlive, g.mbits)
+                                    : colp[slot - slot_lo];
             }
         }
     }

@@ -324,13 +324,60 @@ __global__ void __launch_bounds__(kSweepBlock) k_trav_pull(GraphView rg, const u
 }
 
 // ---- read copies of a host-resident in-edge store ----------------------------------
-// mirror[k] = live edge k (merge-path order) for k < mlive and, with a tail,
-// tail[k - mlive] = edge k for the rest, read from the store in host memory
-// through the tile sweep (live slots only, coalesced). g must carry no copy
-// itself. Tiles are in edge order, so without a tail a wave stops at its
-// first tile past mlive.
+// Lanes [l0, l0 + n) of the wave hold (in v) the ids of indices i0 .. i0 + n - 1
+// of an array of b-bit ids packed back to back (b <= 31 here, so the words
+// they cover fit one per lane; the array is zeroed before the fill). Each
+// lane assembles one word from the up to 32 / b + 2 ids that overlap it;
+// whole words are stored, the two end words - shared with the neighbouring
+// runs - are ORed in. Every lane of the wave calls it (the ids move by
+// shuffles).
+__device__ __forceinline__ void pack_run(uint32_t *arr, unsigned long long i0, uint32_t l0, uint32_t n, uint32_t v,
+                                         uint32_t b) {
+    if (n == 0) return;
+    const uint32_t lane = lane_id();
+    const unsigned long long bit0 = i0 * b, bit1 = (i0 + n) * b;
+    const unsigned long long w0 = bit0 >> 5, w1 = (bit1 - 1) >> 5;
+    const unsigned long long w = w0 + lane;
+    const unsigned long long wb = w << 5;
+    unsigned long long q = wb / b;  // first id overlapping word w
+    if (q < i0) q = i0;
+    uint32_t word = 0;
+    const uint32_t nt = 32 / b + 2;
+    for (uint32_t t = 0; t < nt; ++t, ++q) {
+        const long long rel = (long long)(q - i0);
+        const uint32_t src = l0 + (uint32_t)(rel < (long long)n ? rel : (long long)n - 1);
+        const uint32_t x = __shfl(v, (int)src, kWave);
+        if (w <= w1 && q < i0 + n && q * b < wb + 32) {
+            const long long sh = (long long)(q * b) - (long long)wb;
+            word |= sh >= 0 ? (x << (uint32_t)sh) : (x >> (uint32_t)(-sh));
+        }
+    }
+    if (w <= w1) {
+        if (w == w0 || w == w1) atomicOr(&arr[w], word);
+        else arr[w] = word;
+    }
+}
+
+// Store the ids of lanes [l0, l0 + n) at indices i0 .. of a copy: packed at
+// b bits, or plain words (b = 32).
+__device__ __forceinline__ void copy_run(uint32_t *arr, unsigned long long i0, uint32_t l0, uint32_t n, uint32_t v,
+                                         uint32_t b) {
+    if (b == 32) {
+        const uint32_t lane = lane_id();
+        if (lane >= l0 && lane < l0 + n) arr[i0 + (lane - l0)] = v;
+    } else {
+        pack_run(arr, i0, l0, n, v, b);
+    }
+}
+
+// mirror id k = live edge k (merge-path order) for k < mlive and, with a
+// tail, tail id k - mlive = edge k for the rest, read from the store in host
+// memory through the tile sweep (live slots only, coalesced); packed arrays
+// must be zero. g must carry no copy itself. Tiles are in edge order, so
+// without a tail a wave stops at its first tile past mlive.
 __global__ void __launch_bounds__(kSweepBlock) k_mirror_fill(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
-                                                             uint32_t *mirror, uint64_t mlive, uint32_t *tail) {
+                                                             uint32_t *mirror, uint64_t mlive, uint32_t *tail,
+                                                             uint32_t b) {
     __shared__ SweepWave s_wave[kSweepWaves];
     const unsigned long long E = dpre[g.nv];
     const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
@@ -350,12 +397,13 @@ __global__ void __launch_bounds__(kSweepBlock) k_mirror_fill(GraphView g, const 
         tp.advance(g, dpre, tiles);
 #pragma unroll
         for (int j = 0; j < kSweepPerLane; ++j) {
-            const uint32_t p = j * kWave + lane;
-            const unsigned long long k = h.e0 + p;
-            if (p < h.ne) {
-                if (k < mlive) mirror[k] = vv[j];
-                else if (tail) tail[k - mlive] = vv[j];
-            }
+            // positions [j * 64, j * 64 + nrun) of the tile: ids k0 .. k0 + nrun - 1
+            const uint32_t pj = j * kWave;
+            const uint32_t nrun = h.ne > pj ? min(h.ne - pj, (uint32_t)kWave) : 0u;
+            const unsigned long long k0 = h.e0 + pj;
+            const uint32_t nm = k0 < mlive ? (uint32_t)min_u64(nrun, mlive - k0) : 0u;
+            copy_run(mirror, k0, 0, nm, vv[j], b);
+            if (tail) copy_run(tail, k0 + nm - mlive, nm, nrun - nm, vv[j], b);
         }
         __builtin_amdgcn_wave_barrier();
     }

@@ -114,7 +114,7 @@ def test_prcc_host_edges():
 def test_prcc_host_edges_with_hbm_mirror():
     # the edge budget (below the edge array) holds an HBM copy of part of the
     # in-store: CC's in-list seeds and PageRank's pulls read those slots there
-    st = prcc_stream(12, 16, 3, 2000, seed=93, edge_budget_gb=1e-4, segments=7, cache_level=0)
+    st = prcc_stream(12, 16, 3, 2000, seed=93, edge_budget_gb=2e-5, segments=7, cache_level=0)
     assert st["host_edges"] == 1 and 0 < st["in_mirror_live"] < st["ne"]
 
 

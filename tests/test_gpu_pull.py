@@ -126,17 +126,19 @@ def test_pr_pull_host_edges(mode, stream):
     assert (st["streamed_bytes"] > 0) == stream
 
 
-@pytest.mark.parametrize("budget,cache", [(1e-4, 0), (1.0, 0), (1e-4, 1)])
+@pytest.mark.parametrize("budget,cache", [(2e-5, 0), (1.0, 0), (2e-5, 1)])
 def test_pr_pull_host_edges_with_hbm_mirror(budget, cache):
     # both stores in pinned host memory; the edge budget (plus, for PageRank,
     # the -cache budget) holds an HBM copy of the in-store's leading live
-    # edges - part of them, or all - rebuilt after every batch
+    # edges, ids packed at ceil(log2 nv) bits - part of them, or all -
+    # rebuilt after every batch; the rest come from a packed host copy
     nv, row, col = sgraph.gen_rmat(12, 16, seed=65)
     ins, dele = sgraph.gen_updates(row, col, 12, 3, 2000, 2000, seed=66, hub_frac=0.8, hub_top=0.01)
     st = pr_stream(row, col, list(zip(ins, dele)), check_store=True, edge_budget_gb=budget, cache_level=cache,
                    flags=sgraph.FLAG_HOST_EDGES, segments=7, pull_divisor=sgraph.PULL_ALWAYS)
     assert st["host_edges"] == 1 and st["pull_rounds"] > 0
-    partial = budget * 2**30 / 4 < st["ne"] and cache == 0
+    ids = (int(budget * 2**30) // 4 - 1) * 32 // 12  # 12-bit ids at 2^12 vertices
+    partial = ids < st["ne"] and cache == 0
     assert 0 < st["in_mirror_live"] and (st["in_mirror_live"] < st["ne"]) == partial
 
 
