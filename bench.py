@@ -526,8 +526,10 @@ def run_c3(args):
     sb = res["prcc"]["stats"]
     n_pull, ms_pull, b_pull = res["prcc"]["kt"]["pull"]
     pulls = sb["pull_rounds"]
-    # bytes of a pull that cross the host link: the live in-edges outside the HBM mirror
-    host_per_pull = 4 * (sb["ne"] - sb["in_mirror_live"])
+    # bytes of a pull that cross the host link: the live in-edges outside the HBM
+    # copy, read from the packed host copy (ceil(log2 nv) bits per id)
+    id_bits = max(1, (nv - 1).bit_length())
+    host_per_pull = (sb["ne"] - sb["in_mirror_live"]) * id_bits / 8
     achieved = b_pull / (ms_pull * 1e-3) / 1e9 if ms_pull else None
     return {
         "nv": nv, "ne0": len(col), "plain": plain, "batched": batched, "res": res, "pulls": pulls,
@@ -703,7 +705,7 @@ def main():
                   "latency_p99_ms": pct([a + b for a, b in zip(res["pr"]["lat"], res["cc"]["lat"])], 99)},
         "roofline": {
             "kernel": "k_pr_pull + k_pr_pull_fix (PageRank pull rounds over the in-edge store: 8 B per in-edge + "
-                      "24 B per vertex); the in-lists beyond the HBM mirror are read over the host link",
+                      "24 B per vertex); in-lists read from a packed HBM copy, the rest from a packed host copy",
             "bound": "hbm",
             "achieved": round(c3["achieved"], 2) if c3["achieved"] else None,
             "peak": HBM_PEAK_GBS,
@@ -717,7 +719,8 @@ def main():
         },
         "host_link": {
             "pull_rounds_gbs": round(c3["host_per_pull"] * pulls / (ms_pull * 1e-3) / 1e9, 2) if ms_pull else None,
-            "pull_rounds_what": "bytes of live in-edges outside the HBM mirror per pull round x pull rounds / pull time",
+            "pull_rounds_what": "packed bytes of the live in-edges outside the HBM copy per pull round x pull rounds "
+                                "/ pull time",
             "cc_streamed_sweep_gbs": round(res["cc"]["stats"]["streamed_bytes"] / (res["cc"]["stats"]["stream_ms"] * 1e-3)
                                            / 1e9, 2) if res["cc"]["stats"]["stream_ms"] else None,
             "cache_fill_gbs": round(res["prcc"]["stats"]["cache_fill_bytes"] / (res["prcc"]["stats"]["cache_fill_ms"] * 1e-3)

@@ -113,11 +113,12 @@ public:
 
     // Frontier out-edges above which a round is a pull. A resident pull reads
     // every in-edge at HBM speed without atomics (about 40x cheaper per edge
-    // than an atomic push): it pays above E / 24. Over the host link both
-    // directions move about the same bytes per edge read, so a pull pays only
-    // when the frontier holds a large share of the edges (E / 4).
+    // than an atomic push): it pays above E / 24. With the edges in host
+    // memory a push reads the frontier's out-lists over the host link while a
+    // pull reads the in-store mostly from its HBM copy; measured at 2^27
+    // vertices, a pull pays above E / 8.
     unsigned long long dense_thr() const {
-        const uint64_t div = cfg_.pull_divisor ? cfg_.pull_divisor : (rev_ && rev_->host_edges_ ? 4 : 24);
+        const uint64_t div = cfg_.pull_divisor ? cfg_.pull_divisor : (rev_ && rev_->host_edges_ ? 8 : 24);
         return std::max<unsigned long long>(1ull, ne_ / div);
     }
     bool pulls() const { return pv_ != nullptr; }
@@ -184,6 +185,23 @@ public:
         end_timed(timed);
         st_.streamed_bytes += rev_->st_.streamed_bytes - b0;  // the in-store's segments crossed the host link
         st_.stream_ms += rev_->st_.stream_ms - ms0;
+    }
+    // The touched sources' amend pushes (type -1: cancel, +1: compensate)
+    // delivered by one pull over the in-store as it stands; what crosses
+    // epsilon joins round R's frontier (stamp R).
+    void amend_pull(uint32_t R, int type) {
+        const uint64_t words = ((uint64_t)nv_ + 31) / 32;
+        const PullTake t = take_state(R - 1);  // the pull of "round R - 1" reads it and stamps R
+        fill<uint32_t>(t.bits, words, 0u);
+        hipLaunchKernelGGL(k_pr_amend_take, dim3(grid_for(nv_, 256, 4096)), dim3(256), 0, stream_,
+                           (const uint32_t *)touched_, (const uint32_t *)&ctl_->tcount, nv_, op_, type, value_, buffer_,
+                           (const uint32_t *)deg_, t);
+        SG_LAUNCHED();
+        PassIO io = seed_io(R, nullptr, nullptr, 0);
+        io.round = R - 1;
+        const bool timed = timing(SG_KCLASS_PULL);
+        pull_resident(io, 1, timed);
+        if (timed) kt_bytes_[SG_KCLASS_PULL] += 8ull * ne_ + 24ull * nv_;
     }
     void count_pull_bytes(bool timed) {
         if (timed && timing(SG_KCLASS_PULL)) kt_bytes_[SG_KCLASS_PULL] += 8ull * ne_ + 24ull * nv_;
@@ -330,19 +348,27 @@ public:
         frontier_reset();
         const uint32_t R = R_;
         const uint32_t *tc = &ctl_->tcount;
-        {
-            PassIO io = seed_io(R, touched_, tc, 0);
-            launch_pass(io, VAmend<api::PageRank>{op_, -1, value_, buffer_},
-                        ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0}, false);
-            // long old lists were cut into chunks: walk them before the commit moves them
-            drain_seed_spill(R, ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0});
-        }
-        commit(overflow);
-        if (rev_) rev_->apply(d_ins, nins, d_del, ndel);
-        {
-            PassIO io = seed_io(R, touched_, tc, 0);
-            launch_pass(io, VAmend<api::PageRank>{op_, +1, value_, buffer_},
-                        ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0}, false);
+        if (pulls()) {
+            // cancel over the old in-lists, compensate over the new ones: two pulls
+            amend_pull(R, -1);
+            commit(overflow);
+            rev_->apply(d_ins, nins, d_del, ndel);
+            amend_pull(R, +1);
+        } else {
+            {
+                PassIO io = seed_io(R, touched_, tc, 0);
+                launch_pass(io, VAmend<api::PageRank>{op_, -1, value_, buffer_},
+                            ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0}, false);
+                // long old lists were cut into chunks: walk them before the commit moves them
+                drain_seed_spill(R, ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0});
+            }
+            commit(overflow);
+            if (rev_) rev_->apply(d_ins, nins, d_del, ndel);
+            {
+                PassIO io = seed_io(R, touched_, tc, 0);
+                launch_pass(io, VAmend<api::PageRank>{op_, +1, value_, buffer_},
+                            ERelax<api::PageRank>{op_, buffer_, stamp_, R, 0, 0}, false);
+            }
         }
         if (cfg_.flags & SG_FLAG_DEFER_CONVERGE) {
             // the long new lists were cut into chunks for the first round: walk

@@ -58,6 +58,23 @@ struct VPageRank {
     __device__ void take(uint32_t u, uint32_t push) const { t.take(u, push); }
 };
 
+// Cancel / compensate of a PageRank snapshot as a take for a pull pass: each
+// touched source records the push its CombineValueBufferAmend makes (type -1
+// over the old out-degree before the commit, +1 over the new one after it).
+// The pull over the in-edge store as it stands then delivers it along exactly
+// the lists the push form walks (old in-lists before the store's update, new
+// ones after), one read of the store instead of the hub out-lists twice.
+__global__ void k_pr_amend_take(const uint32_t *list, const uint32_t *cnt, uint32_t cap, api::PageRank op, int type,
+                                float *value, float *buffer, const uint32_t *deg, PullTake t) {
+    const uint32_t n = min(*cnt, cap);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t u = list[i];
+        const auto p = op.CombineValueBufferAmend(u, type, &value[u], &buffer[u], deg[u]);
+        if (p.scheduled) t.take(u, __float_as_uint(p.value));
+    }
+}
+
 // State a pull pass reads and updates.
 struct PullPR {
     const uint32_t *bits;          // this round's frontier bitmap

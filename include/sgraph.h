@@ -76,7 +76,7 @@ typedef struct sg_config {
     uint32_t pull_divisor;    /* a round whose frontier's out-edges exceed E /
                                  pull_divisor is a pull round over the in-edge store
                                  (pull_functor.h:22-369, sync_pull_dd.cuh:1-357).
-                                 PageRank: 0 = 24, or 4 with the store in host memory;
+                                 PageRank: 0 = 24, or 8 with the store in host memory;
                                  UINT32_MAX = push rounds only, no in-edge store.
                                  SSSP / BFS / CC and custom operators with the pull
                                  method, with SG_FLAG_IN_EDGES: 0 = 2; UINT32_MAX =
