@@ -2148,18 +2148,30 @@ public:
     // Ids are packed at ceil(log2 nv) bits (27 at 2^27 vertices): the same
     // HBM holds 32 / 27 as many edges, and the host link moves that much less
     // per tail edge. Packed words are ORed together at run ends, so the copies
-    // are zeroed first; the tail is packed in HBM and copied out in one piece.
+    // are zeroed first. The tail is packed in windows of an HBM stage (at most
+    // kTailStageWords, taken out of the mirror's budget) and each window
+    // copied out before the next is packed, so no more of the store than the
+    // budget allows is held in HBM.
+    static constexpr uint64_t kTailStageWords = 16ull << 20;  // 64 MiB
+    static constexpr uint64_t kTailStageMinWords = 1024;
     uint32_t *tail_stage_ = nullptr;
+    uint64_t tail_stage_cap_ = 0;
     static uint64_t packed_words(uint64_t n, uint32_t b) { return (n * b + 31) / 32 + 1; }
+    static uint64_t ids_in_words(uint64_t words, uint32_t b) { return words > 1 ? (words - 1) * 32 / b : 0; }
     void refresh_mirror() {
         mirror_live_ = 0;
         if (!host_edges_) return;
         const uint32_t b = nv_ > 1 ? 32 - (uint32_t)__builtin_clz(nv_ - 1) : 1;
         mbits_ = b > 31 ? 32 : b;
         const double bytes = std::max(0.0, cfg_.edge_budget_gb) * 1073741824.0 + (double)mirror_extra_bytes_;
-        const uint64_t cap_words = (uint64_t)(bytes / 4.0);
-        const uint64_t cap = cap_words > 1 ? (cap_words - 1) * 32 / mbits_ : 0;  // ids that fit
-        const uint64_t want = std::min<uint64_t>(ne_, cap);
+        uint64_t cap_words = (uint64_t)(bytes / 4.0);
+        // a tail needs a stage: its words come out of the mirror's share
+        uint64_t stage_words = 0;
+        if (want_tail_ && ne_ > ids_in_words(cap_words, mbits_)) {
+            stage_words = std::max(kTailStageMinWords, std::min(kTailStageWords, cap_words / 2));
+            cap_words = cap_words > stage_words ? cap_words - stage_words : 0;
+        }
+        const uint64_t want = std::min<uint64_t>(ne_, ids_in_words(cap_words, mbits_));
         const uint64_t rest = want_tail_ ? ne_ - want : 0;
         if (want == 0 && rest == 0) return;
         const uint64_t mwords = want ? packed_words(want, mbits_) : 0;
@@ -2172,26 +2184,44 @@ public:
         const uint64_t twords = rest ? packed_words(rest, mbits_) : 0;
         if (twords > tail_cap_) {
             hfree(tail_);
-            dfree(tail_stage_);
             tail_ = nullptr;
-            tail_stage_ = nullptr;
             tail_cap_ = twords + twords / 8;
             tail_ = halloc<uint32_t>(tail_cap_);
-            tail_stage_ = dalloc<uint32_t>(tail_cap_);
         }
-        if (mbits_ < 32) {
-            if (mwords) fill<uint32_t>(mirror_, mwords, 0u);
-            if (twords) fill<uint32_t>(tail_stage_, twords, 0u);
+        if (rest) {
+            stage_words = std::min(stage_words, twords);
+            if (stage_words > tail_stage_cap_) {
+                dfree(tail_stage_);
+                tail_stage_ = nullptr;
+                tail_stage_ = dalloc<uint32_t>(stage_words);
+                tail_stage_cap_ = stage_words;
+            }
         }
         GraphView g = gview();  // without copies: the fill reads the host store
         g.mirror = nullptr;
         g.mlive = 0;
         g.tail = nullptr;
-        const uint64_t nt = ntiles();
-        hipLaunchKernelGGL(k_mirror_fill, dim3(sweep_grid(k_mirror_fill, nt)), dim3(kSweepBlock), 0, stream_, g,
-                           (const unsigned long long *)dpre_, tiles_, mirror_, want, rest ? tail_stage_ : nullptr, mbits_);
-        SG_LAUNCHED();
-        if (twords) SG_HIP(hipMemcpyAsync(tail_, tail_stage_, twords * 4, hipMemcpyDeviceToHost, stream_));
+        const uint32_t grid = sweep_grid(k_mirror_fill, ntiles());
+        if (want) {
+            if (mbits_ < 32) fill<uint32_t>(mirror_, mwords, 0u);
+            hipLaunchKernelGGL(k_mirror_fill, dim3(grid), dim3(kSweepBlock), 0, stream_, g,
+                               (const unsigned long long *)dpre_, tiles_, mirror_, (uint64_t)want, (uint32_t *)nullptr,
+                               (uint64_t)0, (uint64_t)0, mbits_);
+            SG_LAUNCHED();
+        }
+        // tail windows of win ids (a multiple of 32: each starts on a word)
+        const uint64_t win = rest ? std::max<uint64_t>(32, ids_in_words(stage_words, mbits_) & ~31ull) : 0;
+        for (uint64_t lo = 0; lo < rest; lo += win) {
+            const uint64_t hi = std::min(rest, lo + win);
+            const uint64_t used = ((hi - lo) * mbits_ + 31) / 32;  // words holding the window's bits
+            if (used > tail_stage_cap_) fail(SG_EDEVICE, "tail window exceeds its stage");
+            if (mbits_ < 32) fill<uint32_t>(tail_stage_, used, 0u);
+            hipLaunchKernelGGL(k_mirror_fill, dim3(grid), dim3(kSweepBlock), 0, stream_, g,
+                               (const unsigned long long *)dpre_, tiles_, (uint32_t *)nullptr, (uint64_t)want,
+                               tail_stage_, lo, hi, mbits_);
+            SG_LAUNCHED();
+            SG_HIP(hipMemcpyAsync(tail_ + lo * mbits_ / 32, tail_stage_, used * 4, hipMemcpyDeviceToHost, stream_));
+        }
         if (!tsplit_) tsplit_ = dalloc<unsigned long long>(1);
         hipLaunchKernelGGL(k_tile_split, dim3(1), dim3(1), 0, stream_, tiles_, (const unsigned long long *)dpre_, nv_,
                            (unsigned long long)want, tsplit_);

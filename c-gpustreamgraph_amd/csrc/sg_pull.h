@@ -387,40 +387,60 @@ __device__ __forceinline__ void copy_run(uint32_t *arr, unsigned long long i0, u
     }
 }
 
-// mirror id k = live edge k (merge-path order) for k < mlive and, with a
-// tail, tail id k - mlive = edge k for the rest, read from the store in host
-// memory through the tile sweep (live slots only, coalesced); packed arrays
-// must be zero. g must carry no copy itself. Tiles are in edge order, so
-// without a tail a wave stops at its first tile past mlive.
+// First tile whose edges reach past live-edge index x (every tile before it
+// ends at or below x); nt when none.
+__device__ __forceinline__ unsigned long long tile_reaching(const SweepTiles &tiles, unsigned long long nt,
+                                                            unsigned long long x) {
+    unsigned long long lo = 0, hi = nt;
+    while (lo < hi) {
+        const unsigned long long mid = (lo + hi) / 2;
+        if (tiles.e[mid + 1] > x) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Copies of the store's live edges, read from the store in host memory
+// through the tile sweep (live slots only, coalesced); g must carry no copy
+// itself, and packed arrays must be zero.
+//  - mirror (tail null): mirror id k = live edge k (merge-path order) for
+//    k < mlive; the tiles are in edge order, so a wave stops at its first
+//    tile past mlive.
+//  - tail window (mirror null): stage id i = live edge mlive + wlo + i for
+//    mlive + wlo <= k < mlive + whi; waves start at the first tile reaching
+//    the window. wlo is a multiple of 32, so the window's packed bits start
+//    on a word boundary and windows share no word.
 __global__ void __launch_bounds__(kSweepBlock) k_mirror_fill(GraphView g, const unsigned long long *dpre, SweepTiles tiles,
                                                              uint32_t *mirror, uint64_t mlive, uint32_t *tail,
-                                                             uint32_t b) {
+                                                             uint64_t wlo, uint64_t whi, uint32_t b) {
     __shared__ SweepWave s_wave[kSweepWaves];
     const unsigned long long E = dpre[g.nv];
-    const unsigned long long ntiles = (g.nv + E + kSweepTile - 1) / kSweepTile;
-    const uint32_t w = threadIdx.x / kWave, lane = lane_id();
+    const unsigned long long ntiles = min_u64((g.nv + E + kSweepTile - 1) / kSweepTile, tiles.cap - 1);
+    const uint32_t w = threadIdx.x / kWave;
     SweepWave &sh = s_wave[w];
     uint32_t n_edges = 0;
+    const unsigned long long k_lo = tail ? mlive + wlo : 0, k_hi = tail ? mlive + whi : mlive;
     TilePipe tp;
-    tp.tend = min_u64(ntiles, tiles.cap - 1);
+    tp.tend = ntiles;
     tp.nw = (unsigned long long)gridDim.x * kSweepWaves;
-    tp.t = (unsigned long long)blockIdx.x * kSweepWaves + w;
+    tp.t = (tail ? tile_reaching(tiles, ntiles, k_lo) : 0ull) + (unsigned long long)blockIdx.x * kSweepWaves + w;
     tp.start(g, dpre, tiles);
     while (tp.t < tp.tend) {
         const TileHead h = tp.h;
-        if (h.e0 >= mlive && !tail) break;
+        if (h.e0 >= k_hi) break;
         uint32_t vv[kSweepPerLane];
         sweep_tile(g, dpre, h, tp.pre, g.col, 0, 0u, g.nv, sh, vv, n_edges);
         tp.advance(g, dpre, tiles);
 #pragma unroll
         for (int j = 0; j < kSweepPerLane; ++j) {
-            // positions [j * 64, j * 64 + nrun) of the tile: ids k0 .. k0 + nrun - 1
+            // positions [j * 64, j * 64 + nrun) of the tile: edges k0 .. k0 + nrun - 1, of which
+            // lanes [l0, l1) fall in [k_lo, k_hi)
             const uint32_t pj = j * kWave;
             const uint32_t nrun = h.ne > pj ? min(h.ne - pj, (uint32_t)kWave) : 0u;
             const unsigned long long k0 = h.e0 + pj;
-            const uint32_t nm = k0 < mlive ? (uint32_t)min_u64(nrun, mlive - k0) : 0u;
-            copy_run(mirror, k0, 0, nm, vv[j], b);
-            if (tail) copy_run(tail, k0 + nm - mlive, nm, nrun - nm, vv[j], b);
+            const uint32_t l0 = k0 < k_lo ? (uint32_t)min_u64(nrun, k_lo - k0) : 0u;
+            const uint32_t l1 = k0 < k_hi ? (uint32_t)min_u64(nrun, k_hi - k0) : 0u;
+            if (l1 > l0) copy_run(tail ? tail : mirror, k0 + l0 - k_lo, l0, l1 - l0, vv[j], b);
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -432,13 +452,7 @@ __global__ void k_tile_split(SweepTiles tiles, const unsigned long long *dpre, u
                              unsigned long long *out) {
     const unsigned long long E = dpre[nv];
     const unsigned long long nt = min_u64((nv + E + kSweepTile - 1) / kSweepTile, tiles.cap - 1);
-    unsigned long long lo = 0, hi = nt;
-    while (lo < hi) {
-        const unsigned long long mid = (lo + hi) / 2;
-        if (tiles.e[mid + 1] > mlive) hi = mid;
-        else lo = mid + 1;
-    }
-    *out = lo;
+    *out = tile_reaching(tiles, nt, mlive);
 }
 
 // ---- traversal recompute from the in-lists of invalidated vertices -------------
