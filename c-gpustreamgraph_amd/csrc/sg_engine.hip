@@ -199,12 +199,8 @@ public:
         SG_LAUNCHED();
         PassIO io = seed_io(R, nullptr, nullptr, 0);
         io.round = R - 1;
-        const bool timed = timing(SG_KCLASS_PULL);
-        pull_resident(io, 1, timed);
-        if (timed) kt_bytes_[SG_KCLASS_PULL] += 8ull * ne_ + 24ull * nv_;
-    }
-    void count_pull_bytes(bool timed) {
-        if (timed && timing(SG_KCLASS_PULL)) kt_bytes_[SG_KCLASS_PULL] += 8ull * ne_ + 24ull * nv_;
+        // counted as a pull round on the device: the next pr_rounds adds its bytes
+        pull_resident(io, 1, timing(SG_KCLASS_PULL));
     }
 
     // One round: push (k_round) or, when the frontier's out-edges pass the
@@ -241,6 +237,7 @@ public:
     // the whole store over the host link, so a push round must not start one).
     uint32_t pr_rounds_host(bool timed) {
         const bool do_time = timed && timing(SG_KCLASS_RELAX);
+        const uint64_t p0 = st_.pull_rounds;
         if (timed) fill<unsigned long long>(stats_ + 8, 8, 0ull);
         const uint32_t maxr = cfg_.max_rounds ? cfg_.max_rounds : 1000000u;
         read_block();
@@ -257,10 +254,7 @@ public:
             clear_take(R);
             launch_pass(io, VPageRank{op_, value_, buffer_, take_state(R)},
                         ERelax<api::PageRank>{op_, buffer_, stamp_, R + 1, 0, 0}, do_time);
-            if (dense) {
-                pull_streamed(io, do_time && timing(SG_KCLASS_PULL));
-                count_pull_bytes(timed);
-            }
+            if (dense) pull_streamed(io, do_time && timing(SG_KCLASS_PULL));
             ++R_;
             ++rounds;
             read_block();
@@ -274,6 +268,8 @@ public:
             }
         }
         finish_rounds(timed);
+        if (timed && timing(SG_KCLASS_PULL))
+            kt_bytes_[SG_KCLASS_PULL] += (st_.pull_rounds - p0) * (8ull * ne_ + 24ull * nv_);
         return rounds;
     }
 
