@@ -45,7 +45,7 @@
 // Waves of a pull pass, in 64ths, that take the tiles beyond the in-store's
 // HBM mirror (a tuning constant; the host link saturates with few waves).
 #ifndef SG_PULL_HOST_WAVES64
-#define SG_PULL_HOST_WAVES64 12
+#define SG_PULL_HOST_WAVES64 8
 #endif
 
 static_assert(sizeof(sg_config) == 104, "sg_config layout");

@@ -2154,6 +2154,7 @@ public:
     // budget allows is held in HBM.
     static constexpr uint64_t kTailStageWords = 16ull << 20;  // 64 MiB
     static constexpr uint64_t kTailStageMinWords = 1024;
+    static constexpr uint64_t kCopySlack = 4;
     uint32_t *tail_stage_ = nullptr;
     uint64_t tail_stage_cap_ = 0;
     static uint64_t packed_words(uint64_t n, uint32_t b) { return (n * b + 31) / 32 + 1; }
@@ -2171,21 +2172,22 @@ public:
             stage_words = std::max(kTailStageMinWords, std::min(kTailStageWords, cap_words / 2));
             cap_words = cap_words > stage_words ? cap_words - stage_words : 0;
         }
-        const uint64_t want = std::min<uint64_t>(ne_, ids_in_words(cap_words, mbits_));
+        // the tile sweeps read whole 16-byte groups: 4 words of slack past the last id
+        const uint64_t want = std::min<uint64_t>(ne_, ids_in_words(cap_words > kCopySlack ? cap_words - kCopySlack : 0, mbits_));
         const uint64_t rest = want_tail_ ? ne_ - want : 0;
         if (want == 0 && rest == 0) return;
         const uint64_t mwords = want ? packed_words(want, mbits_) : 0;
-        if (mwords > mirror_cap_) {
+        if (mwords && mwords + kCopySlack > mirror_cap_) {
             dfree(mirror_);
             mirror_ = nullptr;
-            mirror_cap_ = std::min<uint64_t>(cap_words, mwords + mwords / 8);  // room for growth
+            mirror_cap_ = std::min<uint64_t>(cap_words, mwords + std::max<uint64_t>(kCopySlack, mwords / 8));  // room for growth
             mirror_ = dalloc<uint32_t>(mirror_cap_);
         }
         const uint64_t twords = rest ? packed_words(rest, mbits_) : 0;
-        if (twords > tail_cap_) {
+        if (twords && twords + kCopySlack > tail_cap_) {
             hfree(tail_);
             tail_ = nullptr;
-            tail_cap_ = twords + twords / 8;
+            tail_cap_ = twords + twords / 8 + kCopySlack;
             tail_ = halloc<uint32_t>(tail_cap_);
         }
         if (rest) {

@@ -533,6 +533,8 @@ struct SweepWave {
     int32_t base[kSweepTile + 1];
     uint32_t out[kSweepOutStage];
 };
+// base[] also holds a tile's packed words (ids of at most 31 bits, 16-byte groups)
+static_assert(kSweepTile + 1 >= (kSweepTile * 31 + 31) / 32 + 8, "base[] holds a tile's packed words");
 
 // Bounds of tile t, loaded by lanes 0-3 with vector loads so that the load
 // stays in flight while the wave works on the current tile: lane 0 v[t],
@@ -592,6 +594,7 @@ __device__ __forceinline__ TilePre tile_pre(const GraphView &g, const unsigned l
 // [src_lo, src_hi) are walked. pre = tile_pre(h, lane). vv[j] = target at tile
 // position j * 64 + lane (kGap when none); the owner index of a position stays
 // in sh.own.
+template <bool kStagePacked = false>
 __device__ __forceinline__ void sweep_tile(const GraphView &g, const unsigned long long *dpre, const TileHead &h,
                                            const TilePre &pre, const uint32_t *colp, uint64_t slot_lo, uint32_t src_lo,
                                            uint32_t src_hi, SweepWave &sh, uint32_t vv[kSweepPerLane],
@@ -641,6 +644,60 @@ __device__ __forceinline__ void sweep_tile(const GraphView &g, const unsigned lo
         sh.own[lane * kSweepPerLane + j] = (uint16_t)prev;
     }
     __builtin_amdgcn_wave_barrier();
+    // kStagePacked: a tile whose edges all lie in one packed copy (the HBM
+    // mirror or the host tail) loads the words it spans with at most two
+    // 16-byte loads per lane into LDS (base[] is not needed for such a tile)
+    // and unpacks its ids there: each word crosses the bus once, where reading
+    // id by id loads two overlapping words per id. The copies keep 4 words of
+    // slack past their last id for the aligned loads. Kernels that seldom
+    // read a copy leave it off (it costs registers).
+    const uint32_t b = g.mbits;
+    const bool in_mirror = g.mlive && e0 + ne <= g.mlive;
+    const bool packed_tile = kStagePacked && b < 32 && ne && (in_mirror || (g.tail && e0 >= g.mlive));
+    uint32_t *const lw = reinterpret_cast<uint32_t *>(sh.base);
+    uint32_t bit0 = 0;  // bit of the tile's first id in lw
+    if (packed_tile) {
+        const uint32_t *cp = in_mirror ? g.mirror : g.tail;
+        const unsigned long long i0 = in_mirror ? e0 : e0 - g.mlive;
+        const unsigned long long wa = ((i0 * b) >> 5) & ~3ull;
+        // words [wa, end]: end is the word after the one holding the last bit (<= 501 words)
+        const uint32_t nw = (uint32_t)(((i0 + ne) * b + 31) / 32 + 1 - wa);
+        bit0 = (uint32_t)(i0 * b - wa * 32);
+        const uint32_t q0 = lane * 4, q1 = q0 + kWave * 4;
+        uint4 x0 = make_uint4(0u, 0u, 0u, 0u), x1 = x0;
+        if (q0 < nw) x0 = *reinterpret_cast<const uint4 *>(cp + wa + q0);
+        if (q1 < nw) x1 = *reinterpret_cast<const uint4 *>(cp + wa + q1);
+        if (q0 < nw) {
+            lw[q0] = x0.x;
+            lw[q0 + 1] = x0.y;
+            lw[q0 + 2] = x0.z;
+            lw[q0 + 3] = x0.w;
+        }
+        if (q1 < nw) {
+            lw[q1] = x1.x;
+            lw[q1 + 1] = x1.y;
+            lw[q1 + 2] = x1.z;
+            lw[q1 + 3] = x1.w;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (packed_tile) {
+#pragma unroll
+        for (int j = 0; j < kSweepPerLane; ++j) {
+            const uint32_t p = j * kWave + lane;
+            vv[j] = kGap;
+            if (p < ne) {
+                const uint32_t src = v0 + sh.own[p];
+                if (src >= src_lo && src < src_hi) {
+                    ++n_edges;
+                    const uint32_t bit = bit0 + p * b;
+                    const uint64_t x = ((uint64_t)lw[(bit >> 5) + 1] << 32) | lw[bit >> 5];
+                    vv[j] = (uint32_t)(x >> (bit & 31u)) & ((1u << b) - 1u);
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < kSweepPerLane; ++j) {
         const uint32_t p = j * kWave + lane;
@@ -653,8 +710,8 @@ __device__ __forceinline__ void sweep_tile(const GraphView &g, const unsigned lo
                 const uint64_t slot = !wide ? off0 + (uint64_t)((long long)sh.base[o] + (long long)p)
                                             : g.off[src] + (e0 + p - dpre[src]);
                 const unsigned long long k = e0 + p;
-                vv[j] = k < g.mlive ? packed_at(g.mirror, k, g.mbits)
-                        : g.tail    ? packed_at(g.tail, k - g.mlive, g.mbits)
+                vv[j] = k < g.mlive ? packed_at(g.mirror, k, b)
+                        : g.tail    ? packed_at(g.tail, k - g.mlive, b)
                                     : colp[slot - slot_lo];
             }
         }

@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_pr_pull(GraphView rg, const uns
     while (tp.t < tp.tend) {
         uint32_t vv[kSweepPerLane];
         const TileHead h = tp.h;
-        sweep_tile(rg, dpre, h, tp.pre, colp, slot_lo, dst_lo, dst_hi, sh, vv, n_edges);
+        sweep_tile<true>(rg, dpre, h, tp.pre, colp, slot_lo, dst_lo, dst_hi, sh, vv, n_edges);
         tp.advance(rg, dpre, tiles);
         uint32_t bw[kSweepPerLane];
 #pragma unroll
