@@ -2166,14 +2166,15 @@ public:
         mbits_ = b > 31 ? 32 : b;
         const double bytes = std::max(0.0, cfg_.edge_budget_gb) * 1073741824.0 + (double)mirror_extra_bytes_;
         uint64_t cap_words = (uint64_t)(bytes / 4.0);
+        // the tile sweeps read whole 16-byte groups: 4 words of slack past the last id
+        auto fits = [&](uint64_t words) { return ids_in_words(words > kCopySlack ? words - kCopySlack : 0, mbits_); };
         // a tail needs a stage: its words come out of the mirror's share
         uint64_t stage_words = 0;
-        if (want_tail_ && ne_ > ids_in_words(cap_words, mbits_)) {
+        if (want_tail_ && ne_ > fits(cap_words)) {
             stage_words = std::max(kTailStageMinWords, std::min(kTailStageWords, cap_words / 2));
             cap_words = cap_words > stage_words ? cap_words - stage_words : 0;
         }
-        // the tile sweeps read whole 16-byte groups: 4 words of slack past the last id
-        const uint64_t want = std::min<uint64_t>(ne_, ids_in_words(cap_words > kCopySlack ? cap_words - kCopySlack : 0, mbits_));
+        const uint64_t want = std::min<uint64_t>(ne_, fits(cap_words));
         const uint64_t rest = want_tail_ ? ne_ - want : 0;
         if (want == 0 && rest == 0) return;
         const uint64_t mwords = want ? packed_words(want, mbits_) : 0;
